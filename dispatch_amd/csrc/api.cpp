@@ -1,0 +1,182 @@
+// C ABI of libm3drt (include/m3drt.h): context, error state, memory helpers and
+// argument checking in front of the kernels.  Status conventions follow the
+// reference's HOP C ABI (HOP/standalone/hop_c_api.f90:1-18): int status, message
+// by m3drt_last_error, no process exit in library code.
+#include "m3drt_internal.hpp"
+#include <cstring>
+#include <cstdio>
+#include <string>
+
+namespace m3drt {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return set_error(e == hipErrorOutOfMemory ? M3DRT_ENOMEM : M3DRT_EDEVICE, m);
+}
+
+int ensure_scratch(m3drt_ctx *ctx, size_t bytes) {
+  if (bytes <= ctx->scratch_bytes) return M3DRT_OK;
+  if (ctx->scratch) {
+    M3DRT_HIP(hipStreamSynchronize(ctx->stream));
+    M3DRT_HIP(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  const size_t rounded = (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+  M3DRT_HIP(hipMalloc(&ctx->scratch, rounded));
+  ctx->scratch_bytes = rounded;
+  return M3DRT_OK;
+}
+
+}  // namespace m3drt
+
+using m3drt::set_error;
+
+extern "C" {
+
+int m3drt_last_error(char *buf, int buflen) {
+  if (!buf || buflen <= 0) return M3DRT_EINVAL;
+  const std::string &m = m3drt::g_last_error;
+  const size_t n = std::min<size_t>(m.size(), (size_t)buflen - 1);
+  std::memcpy(buf, m.data(), n);
+  buf[n] = '\0';
+  return M3DRT_OK;
+}
+
+int m3drt_create(m3drt_ctx **ctx, int device, int flags) {
+  (void)flags;
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_create: ctx is NULL");
+  *ctx = nullptr;
+  int n = 0;
+  M3DRT_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_error(M3DRT_EINVAL, "m3drt_create: no such device");
+  M3DRT_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  M3DRT_HIP(hipGetDeviceProperties(&prop, device));
+  m3drt_ctx *c = new m3drt_ctx();
+  c->device = device;
+  c->stream = nullptr;
+  c->lds_max = (int)prop.sharedMemPerBlockOptin > 0 ? (int)prop.sharedMemPerBlockOptin : (int)prop.maxSharedMemoryPerMultiProcessor;
+  if (c->lds_max <= 0 || c->lds_max > 160 * 1024) c->lds_max = (int)prop.sharedMemPerBlock;
+  *ctx = c;
+  return M3DRT_OK;
+}
+
+int m3drt_destroy(m3drt_ctx *ctx) {
+  if (!ctx) return M3DRT_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->scratch) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->scratch);
+  }
+  delete ctx;
+  return M3DRT_OK;
+}
+
+int m3drt_set_stream(m3drt_ctx *ctx, void *hip_stream) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_set_stream: ctx is NULL");
+  ctx->stream = static_cast<hipStream_t>(hip_stream);
+  return M3DRT_OK;
+}
+
+int m3drt_synchronize(m3drt_ctx *ctx) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_synchronize: ctx is NULL");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  M3DRT_HIP(hipStreamSynchronize(ctx->stream));
+  return M3DRT_OK;
+}
+
+int m3drt_device_alloc(m3drt_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr) return set_error(M3DRT_EINVAL, "m3drt_device_alloc: NULL argument");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  M3DRT_HIP(hipMalloc(ptr, bytes));
+  return M3DRT_OK;
+}
+
+int m3drt_device_free(m3drt_ctx *ctx, void *ptr) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_device_free: ctx is NULL");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  M3DRT_HIP(hipFree(ptr));
+  return M3DRT_OK;
+}
+
+int m3drt_copy_to_device(m3drt_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return set_error(M3DRT_EINVAL, "m3drt_copy_to_device: NULL argument");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  M3DRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  M3DRT_HIP(hipStreamSynchronize(ctx->stream));
+  return M3DRT_OK;
+}
+
+int m3drt_copy_to_host(m3drt_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return set_error(M3DRT_EINVAL, "m3drt_copy_to_host: NULL argument");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  M3DRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  M3DRT_HIP(hipStreamSynchronize(ctx->stream));
+  return M3DRT_OK;
+}
+
+int m3drt_native_strides(m3drt_sweep_desc *d) {
+  if (!d) return set_error(M3DRT_EINVAL, "m3drt_native_strides: desc is NULL");
+  const int64_t nx = d->nx, ny = d->ny, nf = d->nf;
+  d->cs_x = 1;
+  d->cs_y = nx;
+  d->cs_f = nx * ny;
+  d->cs_z = nx * ny * nf;
+  d->gs_x = 1;
+  d->gs_y = nx;
+  d->gs_f = nx * ny;
+  if (d->eta_ld < nf) d->eta_ld = nf;
+  return M3DRT_OK;
+}
+
+int m3drt_reference_strides(m3drt_sweep_desc *d) {
+  if (!d) return set_error(M3DRT_EINVAL, "m3drt_reference_strides: desc is NULL");
+  const int64_t nx = d->nx, ny = d->ny, nf = d->nf;
+  d->cs_f = 1;
+  d->cs_x = nf;
+  d->cs_y = nf * nx;
+  d->cs_z = nf * nx * ny;
+  d->gs_f = 1;
+  d->gs_x = nf;
+  d->gs_y = nf * nx;
+  if (d->eta_ld < nf) d->eta_ld = nf;
+  return M3DRT_OK;
+}
+
+int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  if (!ctx || !d || !b) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: NULL argument");
+  if (d->nf < 1 || d->nx < 1 || d->ny < 1 || d->nz < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: empty grid");
+  if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must be +1 or -1");
+  if (d->prec != M3DRT_F32 && d->prec != M3DRT_F64) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: bad prec");
+  if (d->solver != M3DRT_LINEAR2)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: solver must be M3DRT_LINEAR2");
+  if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0)) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: ds must be > 0");
+  if (d->vec[2] == 0.0) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: horizontal ray (vec(3) = 0)");
+  if ((d->vec[2] > 0) != (d->dir > 0))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must follow the sign of vec(3)");
+  if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I || !b->chi500 || !b->g500)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing input array");
+  if (d->nx * (int64_t)d->ny > 1 && !b->chi500_up)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: short_char2: chi500 at the upwind points is missing");
+  if (b->eta_row && (!b->eta_mask || d->eta_ld < d->nf))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: eta_row without eta_mask / eta_ld < nf");
+  if ((d->flags & M3DRT_SPLIT_RATES) && (!b->Irem_out || !b->escape_out))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
+  if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: strides must be >= 1 (see m3drt_native_strides)");
+  // every in-plane offset of a cube is formed in 64 bits; the planes themselves must not overlap
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  if (d->nx == 1 && d->ny == 1) return m3drt::sweep_lc2(ctx, d, b);
+  return m3drt::sweep_sc2(ctx, d, b);
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Internal declarations of libm3drt: context, error state, device helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include "m3drt.h"
+
+struct m3drt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *scratch = nullptr;   // per-sweep scratch (dt500, rc per cell); grown on demand
+  size_t scratch_bytes = 0;
+  int lds_max = 0;           // dynamic LDS bytes available per workgroup
+};
+
+namespace m3drt {
+
+// error state of the calling thread (m3drt_last_error)
+int set_error(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+int ensure_scratch(m3drt_ctx *ctx, size_t bytes);
+
+#define M3DRT_HIP(call)                                         \
+  do {                                                          \
+    hipError_t e_ = (call);                                     \
+    if (e_ != hipSuccess) return ::m3drt::hip_check(e_, #call); \
+  } while (0)
+
+int sweep_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
+int sweep_lc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
+
+}  // namespace m3drt
+
+// ------------------------------------------------------------------ device helpers
+#ifdef __HIPCC__
+namespace m3drt {
+namespace dev {
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
+// Weights of the linear step over optical depth t (the quantities of
+// linear_weights, SRC/rt_solvers_mod.f90:785-797): E = exp(-t), Psi = 1 - u0/t,
+// W = u0 - Psi, Esc = 1 - Psi with u0 = 1 - E.
+//
+// FP32 form for gfx950: E from the hardware exponential, E = 2^(-t*log2 e) with t
+// clamped at 87 as the reference clamps it.  For |t| <= 1, where 1 - E and 1 - u0/t
+// cancel, Psi = t*P(t) with the reference's degree-7 minimax P (lw_series,
+// SRC/rt_solvers_mod.f90:803-809) and u0 = t*(1 - Psi), an identity of the exact
+// weights that involves no cancellation; above, u0 = 1 - E and u0/t by the
+// hardware reciprocal.  Both forms are evaluated and one is selected, so the
+// wavefront never diverges.
+__device__ __forceinline__ void lin_weights(float t, float &E, float &W, float &Psi, float &Esc) {
+  const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
+                   t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
+                   t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
+  const float tc = fminf(t, 87.0f);
+  E = __builtin_amdgcn_exp2f(tc * -1.44269504088896341f);
+  const float esc_thin = 1.0f - ps;
+  const float u0_thin = t * esc_thin;
+  const float u0 = 1.0f - E;
+  const float ut = u0 * rcp(t);
+  const bool thin = fabsf(t) <= 1.0f;
+  Psi = thin ? ps : 1.0f - ut;
+  Esc = thin ? esc_thin : ut;
+  W = thin ? u0_thin - ps : ut - E;
+}
+
+// FP64 form: exact weights, the plain FP64 reference of the sweep.
+__device__ __forceinline__ void lin_weights(double t, double &E, double &W, double &Psi, double &Esc) {
+  E = exp(-t);
+  const double u0 = -expm1(-t);
+  if (fabs(t) < 1e-2) {
+    const double ps = t * (0.5 + t * (-1.0 / 6 + t * (1.0 / 24 + t * (-1.0 / 120 + t * (1.0 / 720 +
+                      t * (-1.0 / 5040 + t * (1.0 / 40320)))))));
+    Psi = ps;
+    Esc = 1.0 - ps;
+    W = u0 - ps;
+  } else {
+    const double ut = u0 / t;
+    Psi = 1.0 - ut;
+    Esc = ut;
+    W = ut - E;
+  }
+}
+
+// Waits for this wave's LDS traffic and meets the workgroup at a barrier; global
+// loads issued earlier stay in flight across it (only LDS is shared here).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ int wrap(int v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+// periodic index without a division for -n <= v < 2n; planes narrower than the
+// 4-point stencil (n < 4) take the modulo
+__device__ __forceinline__ int wrap_near(int v, int n) {
+  if (n < 4) return wrap(v, n);
+  return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
+}  // namespace dev
+}  // namespace m3drt
+#endif

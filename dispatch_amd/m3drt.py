@@ -1,0 +1,171 @@
+"""Python binding of libm3drt (include/m3drt.h) for PyTorch-ROCm callers.
+
+This is how tumult-side Python reaches the same kernels that dispatch.x reaches
+through the Fortran module (dispatch_amd/fortran/m3drt_mod.f90): ctypes over the
+C ABI, with torch tensors providing device memory and the current HIP stream.
+There is no CPU fallback: a missing library raises, and so does a failing call.
+
+Layouts (element strides are passed to the kernels, both layouts work):
+  native     tensor shape (nz, nf, ny, nx)   x fastest, then y, f, z  (fast path)
+  reference  tensor shape (nz, ny, nx, nf)   work(f,x,y,z), f fastest
+Per-cell arrays (chi500 ...) are (nz, ny, nx); planes (ghost, exit) are
+(nf, ny, nx) native or (ny, nx, nf) reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libm3drt.so")
+
+M3DRT_OK = 0
+M3DRT_F32, M3DRT_F64 = 0, 1
+M3DRT_LINEAR2 = 0
+M3DRT_SPLIT_RATES, M3DRT_GET_RATES = 1, 2
+
+
+class M3drtError(RuntimeError):
+    pass
+
+
+class SweepDesc(C.Structure):
+    _fields_ = [("nf", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dir", C.c_int32), ("prec", C.c_int32), ("solver", C.c_int32), ("flags", C.c_int32),
+                ("ds", C.c_double * 3), ("vec", C.c_double * 3),
+                ("maser_dtau_max", C.c_double), ("maser_Inu_max", C.c_double), ("wj", C.c_double),
+                ("cs_f", C.c_int64), ("cs_x", C.c_int64), ("cs_y", C.c_int64), ("cs_z", C.c_int64),
+                ("gs_f", C.c_int64), ("gs_x", C.c_int64), ("gs_y", C.c_int64), ("eta_ld", C.c_int64),
+                ("eta_nrow", C.c_int64)]
+
+
+class SweepBufs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "chi", "S", "ghost_chi", "ghost_S", "ghost_I", "chi500", "chi500_up", "g500",
+        "eta_row", "eta_mask", "J", "PsiJ", "I_out", "Psi_out", "Irem_out", "escape_out", "exit_I")]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libm3drt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise M3drtError(f"{LIB_PATH} is missing: build it with `python -m dispatch_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.m3drt_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+        L.m3drt_destroy.argtypes = [vp]
+        L.m3drt_set_stream.argtypes = [vp, vp]
+        L.m3drt_synchronize.argtypes = [vp]
+        L.m3drt_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.m3drt_sc_sweep.argtypes = [vp, C.POINTER(SweepDesc), C.POINTER(SweepBufs)]
+        L.m3drt_native_strides.argtypes = [C.POINTER(SweepDesc)]
+        L.m3drt_reference_strides.argtypes = [C.POINTER(SweepDesc)]
+        for n in ("m3drt_create", "m3drt_destroy", "m3drt_set_stream", "m3drt_synchronize",
+                  "m3drt_last_error", "m3drt_sc_sweep", "m3drt_native_strides",
+                  "m3drt_reference_strides"):
+            getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().m3drt_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(status: int, what: str):
+    if status != M3DRT_OK:
+        raise M3drtError(f"{what} failed ({status}): {last_error()}")
+
+
+class Context:
+    """One m3drt context on a device; calls run on torch's current HIP stream."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self._h = C.c_void_p()
+        _check(lib().m3drt_create(C.byref(self._h), device, 0), "m3drt_create")
+
+    def close(self):
+        if self._h:
+            lib().m3drt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        _check(lib().m3drt_set_stream(self._h, C.c_void_p(s.cuda_stream)), "m3drt_set_stream")
+
+    def synchronize(self):
+        _check(lib().m3drt_synchronize(self._h), "m3drt_synchronize")
+
+    def sc_sweep(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, *, dir, ds, vec,
+                 layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None, Irem_out=None,
+                 escape_out=None, exit_I=None, eta_row=None, eta_mask=None, split_rates=False,
+                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30):
+        """One directional formal solution (short_char2 in 3D, long_char2 when
+        nx = ny = 1) with J += wj*I and PsiJ += wj*Psi fused."""
+        if chi.dtype not in (torch.float32, torch.float64):
+            raise M3drtError("chi must be float32 or float64")
+        nz = chi.shape[0]
+        if layout == "native":
+            _, nf, ny, nx = chi.shape
+        elif layout == "reference":
+            _, ny, nx, nf = chi.shape
+        else:
+            raise M3drtError(f"unknown layout {layout!r}")
+        cubes = [t for t in (chi, S, J, PsiJ, I_out, Psi_out, Irem_out, escape_out) if t is not None]
+        planes = [t for t in (ghost_chi, ghost_S, ghost_I, exit_I) if t is not None]
+        for t in cubes + planes + [chi500, chi500_up, g500]:
+            if t is not None and (not t.is_cuda or not t.is_contiguous()):
+                raise M3drtError("every array must be a contiguous device tensor")
+        for t in cubes:
+            if t.dtype != chi.dtype or tuple(t.shape) != tuple(chi.shape):
+                raise M3drtError("cubes must share chi's dtype and shape")
+        for t in planes:
+            if t.dtype != chi.dtype or t.numel() != nf * nx * ny:
+                raise M3drtError("planes must be (nf, ny, nx) / (ny, nx, nf) of chi's dtype")
+        for t in (chi500, chi500_up):
+            if t is not None and (t.dtype != torch.float32 or t.numel() != nx * ny * nz):
+                raise M3drtError("chi500 arrays are float32 of nx*ny*nz cells")
+        if g500.dtype != torch.float32 or g500.numel() != nx * ny:
+            raise M3drtError("g500 is float32 of nx*ny cells")
+        if get_rates is None:
+            get_rates = PsiJ is not None or Psi_out is not None
+        d = SweepDesc()
+        d.nf, d.nx, d.ny, d.nz = nf, nx, ny, nz
+        d.dir = int(dir)
+        d.prec = M3DRT_F64 if chi.dtype == torch.float64 else M3DRT_F32
+        d.solver = M3DRT_LINEAR2
+        d.flags = (M3DRT_SPLIT_RATES if split_rates else 0) | (M3DRT_GET_RATES if get_rates else 0)
+        d.ds = (C.c_double * 3)(*[float(v) for v in ds])
+        d.vec = (C.c_double * 3)(*[float(v) for v in vec])
+        d.maser_dtau_max, d.maser_Inu_max, d.wj = maser_dtau_max, maser_Inu_max, float(wj)
+        d.eta_ld = 0
+        if eta_mask is not None:
+            if eta_mask.dtype != torch.uint8 or eta_mask.dim() != 2 or eta_mask.shape[1] < nf:
+                raise M3drtError("eta_mask is uint8 of shape (nrow, eta_ld >= nf)")
+            d.eta_ld = eta_mask.shape[1]
+            d.eta_nrow = eta_mask.shape[0]
+        if eta_row is not None and (eta_row.dtype != torch.int32 or eta_row.numel() != nx * ny * nz):
+            raise M3drtError("eta_row is int32 of nx*ny*nz cells")
+        strides = lib().m3drt_native_strides if layout == "native" else lib().m3drt_reference_strides
+        _check(strides(C.byref(d)), "strides")
+        p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        b = SweepBufs(p(chi), p(S), p(ghost_chi), p(ghost_S), p(ghost_I), p(chi500), p(chi500_up), p(g500),
+                      p(eta_row), p(eta_mask), p(J), p(PsiJ), p(I_out), p(Psi_out), p(Irem_out),
+                      p(escape_out), p(exit_I))
+        self._bind_stream()
+        _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")

@@ -1,0 +1,123 @@
+/*
+ * m3drt.h -- C ABI of the MI355X radiative-transfer kernels (libm3drt.so).
+ *
+ * This is the drop-in boundary for OpenMULTI3D's radiative-transfer hot path
+ * (experiments/OpenMULTI3D/ of Bergemann-Group-development/dispatch).  Each entry
+ * point names the reference procedure it replaces.  Conventions follow the
+ * reference's own C ABI, HOP/standalone/hop_c_api.f90:1-18: every call returns an
+ * int status (0 ok, >0 error), the message of the last error is read with
+ * m3drt_last_error, arrays are allocated by the caller, and nothing is retained
+ * by reference after a call returns.
+ *
+ * Array arguments are DEVICE pointers (memory of the context's device), unless a
+ * helper below says otherwise.  Cubes are addressed through element strides
+ * (f, x, y, z), so the reference's Fortran layout work(f,x,y,z) (f fastest) and
+ * the native frequency-sliced layout (x fastest, then y, then f, then z) are both
+ * accepted; the native one is the fast path on MI355X (see m3drt_native_strides).
+ * chi500-like per-cell arrays are (x, y, z) with x fastest, as in the reference.
+ */
+#ifndef M3DRT_H
+#define M3DRT_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define M3DRT_OK 0
+#define M3DRT_EINVAL 1    /* invalid argument or configuration */
+#define M3DRT_EDEVICE 2   /* HIP runtime error */
+#define M3DRT_ENOMEM 3    /* device allocation failed */
+#define M3DRT_ELIMIT 4    /* size beyond a documented kernel limit */
+
+/* element precision of the floating-point cubes of a call */
+#define M3DRT_F32 0
+#define M3DRT_F64 1
+
+/* formal solvers (rt_iteration_params solver=..., SRC/rt_params_mod.f90) */
+#define M3DRT_LINEAR2 0   /* short_char2 in 3D, long_char2 in 1D (the default) */
+
+/* sweep flags */
+#define M3DRT_SPLIT_RATES 1  /* fill Irem and escape ('largest' rate formulation) */
+#define M3DRT_GET_RATES 2    /* form Psi (Lambda* diagonal / chi) for Psi_out / PsiJ */
+
+typedef struct m3drt_ctx m3drt_ctx;
+
+/* Context: one per device and HIP stream.  flags reserved (0).  The reference has
+ * no counterpart (its kernels are Fortran procedures of rt_patch_t); HOP's
+ * hop_init (HOP/standalone/libhop.h) is the analogue. */
+int m3drt_create(m3drt_ctx **ctx, int device, int flags);
+int m3drt_destroy(m3drt_ctx *ctx);
+/* Runs the context's work on an existing hipStream_t (NULL: the null stream). */
+int m3drt_set_stream(m3drt_ctx *ctx, void *hip_stream);
+int m3drt_synchronize(m3drt_ctx *ctx);
+/* Copies the last error message of the calling thread; as hop_last_error,
+ * HOP/standalone/libhop.h. */
+int m3drt_last_error(char *buf, int buflen);
+
+/* Device memory helpers for hosts without HIP bindings (the Fortran side). */
+int m3drt_device_alloc(m3drt_ctx *ctx, size_t bytes, void **ptr);
+int m3drt_device_free(m3drt_ctx *ctx, void *ptr);
+int m3drt_copy_to_device(m3drt_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int m3drt_copy_to_host(m3drt_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+
+/* Geometry and parameters of one directional sweep through a column of slabs. */
+typedef struct {
+  int32_t nf, nx, ny, nz;  /* frequencies, plane size, planes swept */
+  int32_t dir;             /* +1: z increasing (up), -1: down (rt_patch_t%dir) */
+  int32_t prec;            /* M3DRT_F32 or M3DRT_F64 for every float cube */
+  int32_t solver;          /* M3DRT_LINEAR2 */
+  int32_t flags;           /* M3DRT_SPLIT_RATES | M3DRT_GET_RATES */
+  double ds[3];            /* cell size in cm (self%ds * scaling%l) */
+  double vec[3];           /* ray direction (rt_iter%vec(:,i_mu)) */
+  double maser_dtau_max;   /* &nlte_params, default 2 (SRC/m3d_patch_mod.f90:319) */
+  double maser_Inu_max;    /* default 1e30 (SRC/m3d_patch_mod.f90:320) */
+  double wj;               /* weight of J += wj*I and PsiJ += wj*Psi (2*wmu) */
+  int64_t cs_f, cs_x, cs_y, cs_z;  /* element strides of every cube */
+  int64_t gs_f, gs_x, gs_y;        /* element strides of the ghost and exit planes */
+  int64_t eta_ld;                  /* leading dimension of eta_mask (>= nf) */
+  int64_t eta_nrow;                /* rows of eta_mask; eta_row values lie in 0..eta_nrow */
+} m3drt_sweep_desc;
+
+/* Buffers of one sweep.  Optional pointers may be NULL. */
+typedef struct {
+  const void *chi, *S;          /* (nf,nx,ny,nz) opacity and source function */
+  const void *ghost_chi;        /* upstream ghost plane (nf,nx,ny): work(:,:,:,0|nz+1,1:3) */
+  const void *ghost_S;
+  const void *ghost_I;
+  const float *chi500;          /* (nx,ny,nz) composition%chi500 */
+  const float *chi500_up;       /* (nx,ny,nz) chi500 at the upwind point of this direction */
+  const float *g500;            /* (nx,ny) chi500 of the ghost plane */
+  const int32_t *eta_row;       /* (nx,ny,nz) row of the cell's eta flags, 0 none; or NULL */
+  const uint8_t *eta_mask;      /* (eta_ld, eta_nrow) logical(1) flags of the eta step */
+  void *J, *PsiJ;               /* accumulated in place (integrate_intensity) */
+  void *I_out, *Psi_out;        /* full cubes of Inu and Psi/chi */
+  void *Irem_out, *escape_out;  /* with M3DRT_SPLIT_RATES */
+  void *exit_I;                 /* (nf,nx,ny) Inu on the last plane (exit plane / emergent) */
+} m3drt_sweep_bufs;
+
+/*
+ * One formal solution along direction vec through nz planes, fused with the
+ * angle integration of J and PsiJ.  Replaces, per direction:
+ *   rt_patch_t%short_char2   SRC/rt_solvers_mod.f90:224-507 (nx*ny > 1)
+ *   rt_patch_t%long_char2    SRC/rt_solvers_mod.f90:527-663 (nx = ny = 1)
+ *   rt_patch_t%integrate_intensity  SRC/rt_patch_mod.f90:3399-3431 (J, PsiJ)
+ * over a whole column of slabs (rt_column_t%execute, SRC/rt_column_mod.f90:252-321
+ * hands the exit plane downstream; here the sweep continues through it).
+ * No amr grading (amr='').
+ * Limits (M3DRT_ELIMIT): nx <= 1024; the plane must fit the 160 KB LDS with the
+ * intensity at least (nx*ny*sizeof(elem) <= 160 KB).
+ */
+int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs);
+
+/* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
+ * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */
+int m3drt_native_strides(m3drt_sweep_desc *desc);
+/* Strides of the reference layout work(f,x,y,z), f fastest. */
+int m3drt_reference_strides(m3drt_sweep_desc *desc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,216 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py import this module; it is the checker, never the
+thing measured or shipped.  The functions restate the reference algorithms of
+OpenMULTI3D (experiments/OpenMULTI3D/ of Bergemann-Group-development/dispatch);
+each C function cites the reference file:line it follows (oracle/m3d_oracle.c).
+
+Cubes use the reference's Fortran order with frequency fastest.  numpy arrays of
+shape (nz, ny, nx, nf) in C order have exactly that memory layout, so the
+wrappers take and return them in that shape.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class ScParams(C.Structure):
+    _fields_ = [("nf", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dir", C.c_int), ("ds", C.c_double * 3), ("vec", C.c_double * 3),
+                ("maser_dtau_max", C.c_double), ("maser_Inu_max", C.c_double),
+                ("split_rates", C.c_int), ("get_rates", C.c_int), ("nf_stride", C.c_int)]
+
+
+class RayT(C.Structure):
+    _fields_ = [("xkern", C.c_double * 4), ("ykern", C.c_double * 4), ("dsup", C.c_double),
+                ("upo1", C.c_int), ("upo2", C.c_int)]
+
+
+def build():
+    """Compile liboracle.so with gcc (oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp = C.c_void_p
+    for n in ("orc_linear_weights_n", "orc_parab_moments_n"):
+        getattr(L, n).argtypes = [C.c_int, f32p, f32p, f32p, f32p, f32p]
+        getattr(L, n).restype = None
+    L.orc_exp_neg_n.argtypes = [C.c_int, f32p, f32p, f32p]
+    L.orc_exp_neg_n.restype = None
+    L.orc_cc_kernel.argtypes = [C.c_double, C.POINTER(C.c_double)]
+    L.orc_set_stencil.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(RayT)]
+    L.orc_gauss_quadrature.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+    L.orc_gauss_quadrature.restype = C.c_int
+    for n in ("orc_short_char2_f32", "orc_short_char2_f64"):
+        getattr(L, n).argtypes = [C.POINTER(ScParams)] + [vp] * 11
+        getattr(L, n).restype = None
+    for n in ("orc_long_char2_f32", "orc_long_char2_f64"):
+        getattr(L, n).argtypes = [C.POINTER(ScParams)] + [vp] * 10
+        getattr(L, n).restype = None
+    L.orc_short_char_f32.argtypes = [C.POINTER(ScParams)] + [vp] * 6
+    L.orc_long_char_quadratic_f32.argtypes = [C.POINTER(ScParams)] + [vp] * 4
+    L.orc_integrate_intensity.argtypes = [C.c_int] * 4 + [C.c_float] + [vp] * 4
+    L.orc_sweep_windows_f32.argtypes = [C.POINTER(ScParams), C.c_int] + [vp] * 6 + [C.c_float, vp]
+    L.orc_sweep_windows_f32.restype = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def linear_weights(t):
+    t = np.ascontiguousarray(t, np.float32)
+    out = [np.empty_like(t) for _ in range(4)]
+    lib().orc_linear_weights_n(t.size, t, *out)
+    return tuple(out)  # E, W, Psi, Esc
+
+
+def parab_moments(t):
+    t = np.ascontiguousarray(t, np.float32)
+    out = [np.empty_like(t) for _ in range(4)]
+    lib().orc_parab_moments_n(t.size, t, *out)
+    return tuple(out)  # E, U0, U1, q
+
+
+def exp_neg(t):
+    t = np.ascontiguousarray(t, np.float32)
+    E, u0 = np.empty_like(t), np.empty_like(t)
+    lib().orc_exp_neg_n(t.size, t, E, u0)
+    return E, u0
+
+
+def cc_kernel(s):
+    k = (C.c_double * 4)()
+    lib().orc_cc_kernel(float(s), k)
+    return np.array(k[:])
+
+
+def set_stencil(ds, vec):
+    r = RayT()
+    d = (C.c_double * 3)(*ds)
+    v = (C.c_double * 3)(*vec)
+    lib().orc_set_stencil(d, v, C.byref(r))
+    return dict(xkern=np.array(r.xkern[:]), ykern=np.array(r.ykern[:]), dsup=r.dsup,
+                upo1=r.upo1, upo2=r.upo2)
+
+
+def gauss_quadrature(ntheta, nphi, jnu_weights=True, one_dimensional=False):
+    n = 2 * ntheta * nphi
+    vec = np.zeros(3 * n)
+    wts = np.zeros(n)
+    m = lib().orc_gauss_quadrature(ntheta, nphi, int(jnu_weights), int(one_dimensional), vec, wts)
+    if m < 0:
+        raise ValueError("ntheta out of range")
+    return vec[:3 * m].reshape(m, 3), wts[:m]
+
+
+def _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max=2.0, maser_Inu_max=1e30,
+            split_rates=False, get_rates=True, nf_stride=0):
+    return ScParams(nf, nx, ny, nz, dir, (C.c_double * 3)(*ds), (C.c_double * 3)(*vec),
+                    maser_dtau_max, maser_Inu_max, int(split_rates), int(get_rates), nf_stride)
+
+
+def short_char2(chi, S, I, chi500, chi500_up, g500, dir, ds, vec, eta_row=None, eta_mask=None,
+                split_rates=False, get_rates=True, maser_dtau_max=2.0, maser_Inu_max=1e30,
+                fp64=False):
+    """Restated short_char2 on work cubes of shape (nz+2, ny, nx, nf) including the
+    ghost planes; I must hold the upstream ghost plane.  Returns (I, Psi, Irem,
+    escape) with Psi/Irem/escape of shape (nz, ny, nx, nf)."""
+    dt = np.float64 if fp64 else np.float32
+    chi, S = np.ascontiguousarray(chi, dt), np.ascontiguousarray(S, dt)
+    I = np.array(I, dt, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    c5 = np.ascontiguousarray(chi500, dt)
+    cu = np.ascontiguousarray(chi500_up, dt)
+    g5 = np.ascontiguousarray(g500, dt)
+    psi = np.zeros((nz, ny, nx, nf), dt)
+    irem = np.zeros((nz, ny, nx, nf), dt) if split_rates else None
+    esc = np.zeros((nz, ny, nx, nf), dt) if split_rates else None
+    er = None if eta_row is None else np.ascontiguousarray(eta_row, np.int32)
+    em = None if eta_mask is None else np.ascontiguousarray(eta_mask, np.uint8)
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, split_rates, get_rates)
+    fn = lib().orc_short_char2_f64 if fp64 else lib().orc_short_char2_f32
+    fn(C.byref(p), _p(chi), _p(S), _p(I), _p(c5), _p(cu), _p(g5), _p(er), _p(em),
+       _p(psi), _p(irem), _p(esc))
+    return I, psi, irem, esc
+
+
+def long_char2(chi, S, I, chi500, g500, dir, ds, vec, eta_row=None, eta_mask=None,
+               split_rates=False, get_rates=True, maser_dtau_max=2.0, maser_Inu_max=1e30,
+               fp64=False):
+    dt = np.float64 if fp64 else np.float32
+    chi, S = np.ascontiguousarray(chi, dt), np.ascontiguousarray(S, dt)
+    I = np.array(I, dt, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    c5 = np.ascontiguousarray(chi500, dt)
+    g5 = np.ascontiguousarray(g500, dt)
+    psi = np.zeros((nz, ny, nx, nf), dt)
+    irem = np.zeros((nz, ny, nx, nf), dt) if split_rates else None
+    esc = np.zeros((nz, ny, nx, nf), dt) if split_rates else None
+    er = None if eta_row is None else np.ascontiguousarray(eta_row, np.int32)
+    em = None if eta_mask is None else np.ascontiguousarray(eta_mask, np.uint8)
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, split_rates, get_rates)
+    fn = lib().orc_long_char2_f64 if fp64 else lib().orc_long_char2_f32
+    fn(C.byref(p), _p(chi), _p(S), _p(I), _p(c5), _p(g5), _p(er), _p(em), _p(psi), _p(irem), _p(esc))
+    return I, psi, irem, esc
+
+
+def short_char(chi, S, I, dir, ds, vec, split_rates=False, get_rates=True,
+               maser_dtau_max=2.0, maser_Inu_max=1e30):
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    irem = np.zeros_like(psi) if split_rates else None
+    esc = np.zeros_like(psi) if split_rates else None
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, split_rates, get_rates)
+    lib().orc_short_char_f32(C.byref(p), _p(chi), _p(S), _p(I), _p(psi), _p(irem), _p(esc))
+    return I, psi, irem, esc
+
+
+def long_char_quadratic(chi, S, I, dir, ds, vec, get_rates=True, maser_dtau_max=2.0,
+                        maser_Inu_max=1e30):
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, False, get_rates)
+    lib().orc_long_char_quadratic_f32(C.byref(p), _p(chi), _p(S), _p(I), _p(psi))
+    return I, psi
+
+
+def sweep_windows(chi, S, I, chi500, chi500_up, g500, dir, ds, vec, wmu2, J, nwin):
+    """CPU baseline: short_char2 + J += wmu2*I over nwin frequency windows on the
+    OpenMP threads (OMP_NUM_THREADS).  Arrays in the reference layout; I holds the
+    ghost plane and receives the intensities; J (nz, ny, nx, nf) is accumulated."""
+    nz2, ny, nx, nf = chi.shape
+    p = _params(nf, nx, ny, nz2 - 2, dir, ds, vec, split_rates=False, get_rates=False)
+    lib().orc_sweep_windows_f32(C.byref(p), int(nwin), _p(chi), _p(S), _p(I), _p(chi500),
+                                _p(chi500_up), _p(g500), float(wmu2), _p(J))

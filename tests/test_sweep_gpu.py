@@ -1,0 +1,152 @@
+"""GPU parity of the short-characteristics sweep (m3drt_sc_sweep) against the
+oracle's restatement of short_char2 / long_char2 on the same seeded inputs.
+
+FP32: the kernel forms the step weights with the hardware exponential and
+reciprocal, the oracle with the reference's exp_neg and IEEE division, so they
+agree to rounding, not bit for bit.  Tolerance: 1e-5 relative (the accuracy the
+FP32 sweep must hold against FP64), with a floor of 1e-6 of the field's maximum
+for values that cancel to near zero.  FP64: the kernel against the oracle's FP64
+twin to 1e-11.
+"""
+import numpy as np
+import pytest
+import torch
+
+from sweep_cases import interior, make_case, max_rel, plane_to_native, run_oracle, to_native
+
+pytestmark = pytest.mark.gpu
+RTOL32 = 1e-5
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def run_gpu(ctx, case, layout="native", dtype=torch.float32, split_rates=False, wj=0.25, full=True):
+    gz = case["gz"]
+    chi, S, I = case["chi"], case["S"], case["I"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(interior(a)), dtype)
+        pla = lambda a: dev(plane_to_native(a[gz]), dtype)
+    else:
+        cub = lambda a: dev(interior(a), dtype)
+        pla = lambda a: dev(a[gz], dtype)
+    c, s = cub(chi), cub(S)
+    J = torch.full_like(c, 0.125)
+    PsiJ = torch.full_like(c, 0.5)
+    Iout = torch.zeros_like(c) if full else None
+    Psi = torch.zeros_like(c) if full else None
+    Irem = torch.zeros_like(c) if split_rates else None
+    esc = torch.zeros_like(c) if split_rates else None
+    exitI = torch.zeros_like(pla(chi))
+    er = dev(case["eta_row"]) if "eta_row" in case else None
+    em = dev(case["eta_mask"]) if "eta_mask" in case else None
+    ctx.sc_sweep(c, s, pla(chi), pla(S), pla(I), dev(case["chi500"]), dev(case["chi500_up"]),
+                 dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"], layout=layout,
+                 wj=wj, J=J, PsiJ=PsiJ, I_out=Iout, Psi_out=Psi, Irem_out=Irem, escape_out=esc,
+                 exit_I=exitI, eta_row=er, eta_mask=em, split_rates=split_rates, get_rates=True)
+    torch.cuda.synchronize()
+    back = {}
+    for k, v in dict(I=Iout, Psi=Psi, J=J, PsiJ=PsiJ, Irem=Irem, esc=esc).items():
+        if v is None:
+            continue
+        a = v.double().cpu().numpy()
+        back[k] = np.moveaxis(a, 1, 3) if layout == "native" else a
+    e = exitI.double().cpu().numpy()
+    back["exit"] = np.moveaxis(e, 0, 2) if layout == "native" else e
+    return back
+
+
+def check32(got, ref, what, rtol=RTOL32):
+    ref = np.asarray(ref, np.float64)
+    floor = 1e-6 * np.max(np.abs(ref)) if ref.size else 0.0
+    err = max_rel(got, ref, floor)
+    assert err < rtol, f"{what}: max rel err {err:.3e}"
+
+
+CASES = [
+    dict(seed=1, vec=(0.3, -0.4, 0.866)),
+    dict(seed=2, vec=(-0.5, 0.2, -0.84)),
+    dict(seed=3, vec=(0.61481017, 0.0, 0.78867513)),
+    dict(seed=4, vec=(-0.48910, 0.84715, 0.21132487)),
+    dict(seed=5, vec=(0.9, 0.2, 0.39), nx=17, ny=13, nz=7),
+]
+
+
+@pytest.mark.parametrize("kw", CASES)
+@pytest.mark.parametrize("layout", ["native", "reference"])
+def test_sweep_fp32_matches_oracle(ctx, kw, layout):
+    kw = dict(kw)
+    case = make_case(nf=kw.pop("nf", 16), nx=kw.pop("nx", 12), ny=kw.pop("ny", 10), nz=kw.pop("nz", 9), **kw)
+    print(f"case seed={case['dir']} {kw} layout={layout}", flush=True)
+    I, psi, _, _ = run_oracle(case)
+    g = run_gpu(ctx, case, layout)
+    check32(g["I"], interior(I), "I")
+    check32(g["Psi"], psi, "Psi")
+    # fused angle integration: J += wj*I, PsiJ += wj*Psi
+    check32(g["J"], 0.125 + 0.25 * interior(I).astype(np.float64), "J")
+    check32(g["PsiJ"], 0.5 + 0.25 * psi.astype(np.float64), "PsiJ")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(g["exit"], I[last], "exit plane")
+
+
+@pytest.mark.parametrize("seed,vec,maser", [(7, (0.2, 0.9, 0.39), False), (8, (0.1, -0.97, -0.211), True)])
+def test_sweep_eta_and_split_rates(ctx, seed, vec, maser):
+    case = make_case(nf=16, nx=11, ny=9, nz=8, seed=seed, vec=vec, eta=True, maser=maser)
+    print(f"eta case seed={seed} maser={maser}", flush=True)
+    I, psi, irem, esc = run_oracle(case, split_rates=True)
+    g = run_gpu(ctx, case, split_rates=True)
+    check32(g["I"], interior(I), "I")
+    check32(g["Psi"], psi, "Psi")
+    check32(g["Irem"], irem, "Irem")
+    check32(g["esc"], esc, "escape")
+
+
+@pytest.mark.parametrize("kw", CASES[:3])
+def test_sweep_fp64_matches_oracle_fp64(ctx, kw):
+    kw = dict(kw)
+    case = make_case(nf=8, nx=12, ny=10, nz=9, **kw)
+    I, psi, _, _ = run_oracle(case, fp64=True)
+    g = run_gpu(ctx, case, dtype=torch.float64)
+    assert max_rel(g["I"], interior(I), 1e-12) < 1e-11
+    assert max_rel(g["Psi"], psi, 1e-12 * np.max(np.abs(psi))) < 1e-11
+
+
+@pytest.mark.parametrize("nf", [1, 63, 200])
+def test_sweep_1d_long_char2(ctx, nf):
+    case = make_case(nf=nf, nx=1, ny=1, nz=40, seed=nf, vec=(0.0, 0.0, 0.5), eta=(nf == 63))
+    print(f"1d nf={nf}", flush=True)
+    from oracle import oracle as O
+    I, psi, _, _ = O.long_char2(case["chi"], case["S"], case["I"], case["chi500"], case["g500"],
+                                case["dir"], case["ds"], case["vec"], eta_row=case.get("eta_row"),
+                                eta_mask=case.get("eta_mask"))
+    for layout in ("native", "reference"):
+        g = run_gpu(ctx, case, layout)
+        check32(g["I"], interior(I), "I 1d")
+        check32(g["Psi"], psi, "Psi 1d")
+
+
+def test_sweep_plane_at_lds_limit(ctx):
+    """The config-1 plane (120 x 120) on a few planes and frequencies: I and x sit
+    in LDS, S of the upwind plane is re-read from global memory."""
+    case = make_case(nf=4, nx=120, ny=120, nz=5, seed=12, vec=(-0.48910, 0.84715, 0.21132487))
+    I, psi, _, _ = run_oracle(case)
+    g = run_gpu(ctx, case)
+    check32(g["I"], interior(I), "I 120x120")
+    check32(g["Psi"], psi, "Psi 120x120")
+
+
+def test_sweep_rejects_bad_arguments(ctx):
+    from dispatch_amd import m3drt
+    case = make_case(nf=2, nx=4, ny=4, nz=3, seed=0)
+    c = dev(to_native(interior(case["chi"])))
+    p = dev(plane_to_native(case["chi"][0]))
+    with pytest.raises(m3drt.M3drtError, match="dir must follow"):
+        ctx.sc_sweep(c, c, p, p, p, dev(case["chi500"]), dev(case["chi500_up"]), dev(case["g500"]),
+                     dir=-1, ds=case["ds"], vec=case["vec"])
+    with pytest.raises(m3drt.M3drtError, match="upwind"):
+        ctx.sc_sweep(c, c, p, p, p, dev(case["chi500"]), None, dev(case["g500"]),
+                     dir=1, ds=case["ds"], vec=case["vec"])
