@@ -214,3 +214,131 @@ def sweep_windows(chi, S, I, chi500, chi500_up, g500, dir, ds, vec, wmu2, J, nwi
     p = _params(nf, nx, ny, nz2 - 2, dir, ds, vec, split_rates=False, get_rates=False)
     lib().orc_sweep_windows_f32(C.byref(p), int(nwin), _p(chi), _p(S), _p(I), _p(chi500),
                                 _p(chi500_up), _p(g500), float(wmu2), _p(J))
+
+
+# ---------------------------------------------------------------- QLG / opacity
+
+class QlgCoef(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("ih", "bL", "gG", "cZH", "cL2")]
+
+
+class QlgCell(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("D", "D2", "SQ", "D2h", "D2G", "D2L")]
+
+
+class Linelist(C.Structure):
+    _fields_ = [("nline", C.c_int), ("llam0", C.c_void_p)] + \
+               [(n, C.c_void_p) for n in ("lower_ev", "gpref", "grad", "gw1", "gw2", "gstk",
+                                          "unsold", "lev_i", "lev_j", "ipart", "used", "strength",
+                                          "reach")]
+
+
+def _qlg_declare():
+    L = lib()
+    if getattr(L, "_qlg_ok", False):
+        return L
+    L.orc_qlg_init.restype = None
+    L.orc_qlg_tables.argtypes = [f32p, f32p]
+    L.orc_qlg_ref_f64.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, f64p]
+    L.orc_qlg_bot_sweep.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, f32p]
+    L.orc_qlg_bot_cell.argtypes = [C.c_float, C.POINTER(QlgCell)]
+    L.orc_qlg_bot_line_sweep.argtypes = [C.POINTER(QlgCell), C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_int, f64p, C.c_double, C.c_double,
+                                         C.c_double, C.c_float, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_float), f32p]
+    L.orc_qlg_bot_profile.argtypes = [C.c_float] * 4
+    L.orc_qlg_bot_profile.restype = C.c_float
+    L.orc_add_opacity_equi.argtypes = ([C.POINTER(Linelist), C.c_int, f64p, C.c_int] + [C.c_void_p] * 11
+                                       + [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float])
+    L.orc_add_opacity_equi.restype = None
+    L._qlg_ok = True
+    return L
+
+
+def qlg_tables():
+    L = _qlg_declare()
+    tg, th = np.zeros(9020, np.float32), np.zeros(9020, np.float32)
+    L.orc_qlg_tables(tg, th)
+    return tg, th
+
+
+def qlg_ref_f64(a, v):
+    L = _qlg_declare()
+    a = np.atleast_1d(np.asarray(a, np.float64))
+    v = np.atleast_1d(np.asarray(v, np.float64))
+    out = np.zeros(1)
+    res = np.empty(np.broadcast(a, v).shape)
+    for i, (ai, vi) in enumerate(np.broadcast(a, v)):
+        L.orc_qlg_ref_f64(ai, vi, 0.0, 1, out)
+        res.flat[i] = out[0]
+    return res
+
+
+def qlg_bot(a, v):
+    """K(a, v) of QLG-bot, point by point (qlg_bot_sweep with n = 1)."""
+    L = _qlg_declare()
+    a = np.asarray(a, np.float32)
+    v = np.asarray(v, np.float32)
+    out = np.zeros(1, np.float32)
+    res = np.empty(np.broadcast(a, v).shape, np.float32)
+    for i, (ai, vi) in enumerate(np.broadcast(a, v)):
+        L.orc_qlg_bot_sweep(ai, vi, 0.0, 1, out)
+        res.flat[i] = out[0]
+    return res
+
+
+def qlg_line_sweep(D, gL, S, ref, lnq, llam, llc, pos, idl, cap):
+    L = _qlg_declare()
+    w = QlgCell()
+    L.orc_qlg_bot_cell(np.float32(D), C.byref(w))
+    nf = len(llam)
+    K = np.zeros(nf, np.float32)
+    fa, fb, hw = C.c_int(), C.c_int(), C.c_float()
+    L.orc_qlg_bot_line_sweep(C.byref(w), gL, S, ref, lnq, nf, np.ascontiguousarray(llam, np.float64),
+                             llc, pos, idl, cap, C.byref(fa), C.byref(fb), C.byref(hw), K)
+    return fa.value, fb.value, hw.value, K
+
+
+def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None, do_nlte=False,
+                     one_m_eps=None, chi_scat=None, tol=1e-4, line_reject_frac=0.0):
+    """Restated add_opacity_equi.  lines: dict of SoA arrays (llam0, lower_ev, gpref,
+    grad, gw1, gw2, gstk, unsold, lev_i, lev_j, optional ipart); chi/eta/cont/logref
+    (ncell, nf) float32, updated copies returned; cells: dict of per-cell arrays
+    (vproj, idf, vb2, ekt, ne, tothi, spe); lnbase (np, ncell); dep (nlev, ncell)."""
+    L = _qlg_declare()
+    nl = len(lines["llam0"])
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    used = np.zeros(nl, np.int32)
+    strength = np.zeros(nl, np.float32)
+    reach = np.zeros(nl, np.float32)
+    ll = Linelist(nl, arr(lines["llam0"], np.float64),
+                  *[arr(lines[k], np.float32) for k in ("lower_ev", "gpref", "grad", "gw1", "gw2", "gstk")],
+                  *[arr(lines[k], np.int32) for k in ("unsold", "lev_i", "lev_j")],
+                  arr(lines["ipart"], np.int32) if lines.get("ipart") is not None else None,
+                  used.ctypes.data_as(C.c_void_p), strength.ctypes.data_as(C.c_void_p),
+                  reach.ctypes.data_as(C.c_void_p))
+    chi = np.array(chi, np.float32, copy=True, order="C")
+    eta = np.array(eta, np.float32, copy=True, order="C")
+    ncell, nf = chi.shape
+    lnb = np.ascontiguousarray(lnbase, np.float32)
+    np_ = lnb.shape[0]
+    dep = np.ones((1, ncell), np.float32) if dep is None else np.ascontiguousarray(dep, np.float32)
+    csc = None if chi_scat is None else np.array(chi_scat, np.float32, copy=True, order="C")
+    L.orc_add_opacity_equi(C.byref(ll), nf, np.ascontiguousarray(llam, np.float64), ncell,
+                           chi.ctypes.data_as(C.c_void_p), eta.ctypes.data_as(C.c_void_p),
+                           arr(cont, np.float32), arr(logref, np.float32),
+                           *[arr(cells[k], np.float32) for k in ("vproj", "idf", "vb2", "ekt", "ne",
+                                                                  "tothi", "spe")],
+                           np_, lnb.ctypes.data_as(C.c_void_p), dep.shape[0],
+                           dep.ctypes.data_as(C.c_void_p), int(do_nlte),
+                           None if one_m_eps is None else arr(one_m_eps, np.float32),
+                           None if csc is None else csc.ctypes.data_as(C.c_void_p),
+                           int(csc is not None), tol, line_reject_frac)
+    return dict(chi=chi, eta=eta, used=used, strength=strength, reach=reach, chi_scat=csc)

@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden from the reference's own compiled code.
+
+Run in the build container only (it needs the reference tree and amdflang):
+
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py
+
+oracle/_ref/hop_golden.x is the harness of oracle/ref_harness/hop_golden.f90
+linked against HOP objects compiled from the reference sources by
+oracle/build_ref.sh (amdflang, -O2, -DHOP_STANDALONE, no fast-math).  Each fixture
+holds the seeded inputs and the reference's outputs, plus provenance.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+EXE = os.path.join(ROOT, "oracle", "_ref", "hop_golden.x")
+
+
+def provenance():
+    fc = subprocess.run(["amdflang", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    return dict(compiler=fc, flags="-cpp -DHOP_STANDALONE -O2 -fPIC -fopenmp",
+                sources="experiments/OpenMULTI3D/HOP/{qlg_mod,op_math_mod,linelist_mod,...}.f90, exp_neg.inc",
+                harness="oracle/ref_harness/hop_golden.f90")
+
+
+def run(kind, payload: bytes) -> bytes:
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        open(fi, "wb").write(payload)
+        subprocess.run([EXE, kind, fi, fo], check=True)
+        return open(fo, "rb").read()
+
+
+def save(name, **arrays):
+    prov = provenance()
+    h = hashlib.sha256()
+    for k in sorted(arrays):
+        if k.startswith("in_"):
+            h.update(np.ascontiguousarray(arrays[k]).tobytes())
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, input_sha256=h.hexdigest(), **{f"prov_{k}": v for k, v in prov.items()},
+                        **arrays)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def i32(*v):
+    return np.array(v, np.int32).tobytes()
+
+
+def gen_expneg(rng):
+    t = np.concatenate([np.linspace(-2, 90, 6001), rng.uniform(-2, 87, 4000),
+                        10 ** rng.uniform(-8, 0, 1000) * rng.choice([-1, 1], 1000)]).astype(np.float32)
+    out = np.frombuffer(run("expneg", i32(t.size) + t.tobytes()), np.float32)
+    save("hop_expneg.npz", in_t=t, E=out[:t.size], u0=out[t.size:])
+
+
+def gen_qlg(rng):
+    n = 20000
+    a = (10 ** rng.uniform(-4, 2, n)).astype(np.float32)
+    a[:200] = 0.0
+    v = np.concatenate([rng.uniform(-12, 12, n // 2), rng.uniform(-300, 300, n // 4),
+                        rng.uniform(-1, 1, n - n // 2 - n // 4)]).astype(np.float32)
+    K = np.frombuffer(run("qlgbot", i32(n) + a.tobytes() + v.tobytes()), np.float32)
+    m = 5000
+    ad = 10 ** rng.uniform(-4, 2, m)
+    vd = np.concatenate([rng.uniform(-20, 20, m // 2), rng.uniform(-100, 100, m - m // 2)])
+    Kr = np.frombuffer(run("qlgref", i32(m) + ad.tobytes() + vd.tobytes()), np.float64)
+    # line sweeps on a uniform ln(lambda) grid
+    nf, nc = 160, 60
+    llam = np.log(5000.0) + np.arange(nf) * 7e-6
+    idl = 1.0 / 7e-6
+    cases = []
+    for c in range(nc):
+        D = np.float32(rng.uniform(0.3, 4.0))
+        gL = np.float32(D * 10 ** rng.uniform(-4, 1.5))
+        S = np.float32(10 ** rng.uniform(-6, -2))
+        ref = np.float32(10 ** rng.uniform(-10, -6))
+        lnq = np.float32(max(0.0, np.log(S / D / ref)))
+        cap = np.float32(3.4e38 if c % 5 else rng.uniform(1, 30))
+        llc = llam[0] + rng.uniform(-20, nf + 20) * 7e-6
+        pos = (llc - llam[0]) * idl
+        cases.append((D, gL, S, ref, lnq, cap, llc, pos, idl))
+    payload = i32(nc, nf) + llam.tobytes()
+    for (D, gL, S, ref, lnq, cap, llc, pos, idl_) in cases:
+        payload += np.array([D, gL, S, ref, lnq, cap], np.float32).tobytes()
+        payload += np.array([llc, pos, idl_], np.float64).tobytes()
+    out = run("qlgline", payload)
+    rec = 4 + 4 + 4 + 4 * nf
+    fa, fb, hw, Ks = [], [], [], []
+    for c in range(nc):
+        r = out[c * rec:(c + 1) * rec]
+        fa.append(np.frombuffer(r[0:4], np.int32)[0])
+        fb.append(np.frombuffer(r[4:8], np.int32)[0])
+        hw.append(np.frombuffer(r[8:12], np.float32)[0])
+        Ks.append(np.frombuffer(r[12:], np.float32))
+    cs = np.array([c[:6] for c in cases], np.float32)
+    cd = np.array([c[6:] for c in cases], np.float64)
+    save("hop_qlg.npz", in_a=a, in_v=v, K=K, in_ad=ad, in_vd=vd, Kref=Kr, in_llam=llam,
+         in_case_f32=cs, in_case_f64=cd, line_fa=np.array(fa, np.int32), line_fb=np.array(fb, np.int32),
+         line_hw=np.array(hw, np.float32), line_K=np.array(Ks, np.float32))
+
+
+def gen_lineq(rng):
+    n, ns = 21, 40
+    As, bs, xs = [], [], []
+    payload = i32(n, ns)
+    for s in range(ns):
+        # diagonally dominant rate-matrix-like systems (columns summing to ~0 plus a
+        # conservation row), the shape update_populations solves
+        A = -rng.uniform(0, 1, (n, n)) * (rng.random((n, n)) < 0.5)
+        np.fill_diagonal(A, 0.0)
+        np.fill_diagonal(A, -A.sum(axis=0) + rng.uniform(0.1, 1, n))
+        A[rng.integers(n)] = rng.uniform(0.5, 2, n)
+        b = rng.standard_normal(n)
+        As.append(A)
+        bs.append(b)
+        payload += np.asfortranarray(A).tobytes(order="F") + b.tobytes()
+    x = np.frombuffer(run("lineq", payload), np.float64).reshape(ns, n)
+    save("hop_lineq.npz", in_A=np.array(As), in_b=np.array(bs), x=x)
+
+
+def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4):
+    ncell = int(np.prod(m))
+    dl = np.log(5010 / 5000) / (nf - 1)
+    llam = np.log(5000.0) + np.arange(nf) * dl
+    llam0 = np.sort(rng.uniform(llam[0] - 40 * dl, llam[-1] + 40 * dl, nl))
+    lower_ev = rng.uniform(0, 5, nl).astype(np.float32)
+    gpref = (rng.uniform(-3, 0, nl) * np.log(10) - 37.7).astype(np.float32)
+    gpref[:3] += 12.0     # a few very strong lines, whose cores raise the running floor
+    grad = (10 ** rng.uniform(-9, -7.5, nl)).astype(np.float32)
+    gw1 = (10 ** rng.uniform(-30, -28.5, nl)).astype(np.float32)
+    gw2 = rng.uniform(0.3, 0.4, nl).astype(np.float32)
+    gstk = (10 ** rng.uniform(-24, -22, nl) * rng.choice([1, -1], nl)).astype(np.float32)
+    unsold = (rng.random(nl) < 0.3).astype(np.int32)
+    lev_i = rng.integers(0, nlev + 1, nl).astype(np.int32)
+    lev_j = rng.integers(0, nlev + 1, nl).astype(np.int32)
+    ipart = rng.integers(1, np_ + 1, nl).astype(np.int32)
+    F = lambda a: np.asfortranarray(a.astype(np.float32))
+    cont = (1e-6 * 10 ** rng.uniform(-0.5, 0.5, (1, ncell)) * (1 + 0.1 * np.linspace(0, 1, nf))[:, None])
+    cont = cont.astype(np.float32)
+    with np.errstate(divide="ignore"):
+        logref = (np.float32(np.log(np.float32(1e-4) / np.float32(0.5641895835))) +
+                  np.log(cont)).astype(np.float32)
+    chi = (cont * rng.uniform(0, 0.5, (nf, ncell))).astype(np.float32)
+    eta = (chi * rng.uniform(0.5, 1.0, (nf, ncell))).astype(np.float32)
+    cell = lambda lo, hi: rng.uniform(lo, hi, ncell).astype(np.float32)
+    vproj = (rng.uniform(-1, 1, ncell) * 5 * dl).astype(np.float32)
+    idf = (10 ** rng.uniform(5.0, 5.5, ncell)).astype(np.float32)
+    vb2 = (10 ** rng.uniform(11.5, 12.5, ncell)).astype(np.float32)
+    ekt = cell(1.5, 2.5)
+    ne = (10 ** rng.uniform(12, 15, ncell)).astype(np.float32)
+    tothi = (10 ** rng.uniform(15, 17, ncell)).astype(np.float32)
+    spe = (ne * 3.0).astype(np.float32)
+    lnbase = rng.uniform(20, 36, (np_, ncell)).astype(np.float32)
+    dep = rng.uniform(0.5, 2.0, (nlev, ncell)).astype(np.float32)
+    hdr = i32(nl, nf, *m, np_, nlev, flags)
+    payload = hdr + llam0.tobytes()
+    for a in (lower_ev, gpref, grad, gw1, gw2, gstk):
+        payload += a.tobytes()
+    for a in (unsold, lev_i, lev_j, ipart):
+        payload += a.tobytes()
+    payload += llam.tobytes()
+    for a in (chi, eta, cont, logref):
+        payload += a.T.tobytes()       # (nf, ncell) in Fortran order = (ncell, nf) C
+    for a in (vproj, idf, vb2, ekt, ne, tothi, spe):
+        payload += a.tobytes()
+    payload += lnbase.tobytes() + dep.tobytes()   # (ncell, np) Fortran = (np, ncell) C
+    ome = csc = None
+    if flags & 4:
+        ome = rng.uniform(0, 1, (nf, ncell)).astype(np.float32)
+        csc = (cont * 0.1).astype(np.float32)
+        payload += ome.T.tobytes() + csc.T.tobytes()
+    out = run("opac_equi", payload)
+    n2 = nf * ncell
+    o = np.frombuffer(out, np.uint8)
+    chi_o = np.frombuffer(out[:4 * n2], np.float32).reshape(ncell, nf)
+    eta_o = np.frombuffer(out[4 * n2:8 * n2], np.float32).reshape(ncell, nf)
+    p = 8 * n2
+    used = np.frombuffer(out[p:p + 4 * nl], np.int32); p += 4 * nl
+    strength = np.frombuffer(out[p:p + 4 * nl], np.float32); p += 4 * nl
+    reach = np.frombuffer(out[p:p + 4 * nl], np.float32); p += 4 * nl
+    extra = {}
+    if flags & 4:
+        extra["chi_scat"] = np.frombuffer(out[p:p + 4 * n2], np.float32).reshape(ncell, nf)
+        extra["in_one_m_eps"] = ome.T.copy()
+        extra["in_chi_scat"] = csc.T.copy()
+    save(name, in_flags=np.int32(flags), in_m=np.array(m, np.int32), in_llam0=llam0, in_lower_ev=lower_ev,
+         in_gpref=gpref, in_grad=grad, in_gw1=gw1, in_gw2=gw2, in_gstk=gstk, in_unsold=unsold,
+         in_lev_i=lev_i, in_lev_j=lev_j, in_ipart=ipart, in_llam=llam, in_chi=chi.T.copy(),
+         in_eta=eta.T.copy(), in_cont=cont.T.copy(), in_logref=logref.T.copy(), in_vproj=vproj,
+         in_idf=idf, in_vb2=vb2, in_ekt=ekt, in_ne=ne, in_tothi=tothi, in_spe=spe, in_lnbase=lnbase,
+         in_dep=dep, chi=chi_o, eta=eta_o, used=used, strength=strength, reach=reach, **extra)
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit(f"{EXE} missing: run oracle/build_ref.sh first")
+    rng = np.random.default_rng(20261202)
+    gen_expneg(rng)
+    gen_qlg(rng)
+    gen_lineq(rng)
+    gen_opac(rng, "hop_opac_equi_lte.npz", flags=0)
+    gen_opac(rng, "hop_opac_equi_nlte.npz", flags=1 | 2, np_=2)
+    gen_opac(rng, "hop_opac_equi_scat.npz", flags=4, nl=30, nf=200, m=(3, 2, 2))
+
+
+if __name__ == "__main__":
+    main()
