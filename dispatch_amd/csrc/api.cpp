@@ -72,10 +72,9 @@ int m3drt_create(m3drt_ctx **ctx, int device, int flags) {
 int m3drt_destroy(m3drt_ctx *ctx) {
   if (!ctx) return M3DRT_OK;
   (void)hipSetDevice(ctx->device);
-  if (ctx->scratch) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->scratch);
-  }
+  if (ctx->scratch || ctx->qlg_tab) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->qlg_tab) (void)hipFree(ctx->qlg_tab);
   delete ctx;
   return M3DRT_OK;
 }
@@ -177,6 +176,27 @@ int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
   M3DRT_HIP(hipSetDevice(ctx->device));
   if (d->nx == 1 && d->ny == 1) return m3drt::sweep_lc2(ctx, d, b);
   return m3drt::sweep_sc2(ctx, d, b);
+}
+
+int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
+  if (!ctx || !ll || !o) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NULL argument");
+  if (ll->nline < 0 || o->ncell < 0 || o->nf < 0)
+    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: negative size");
+  if (ll->nline == 0 || o->ncell == 0 || o->nf < 2) return M3DRT_OK;  // as the reference: nothing to add
+  if (!ll->llam0 || !ll->lower_ev || !ll->gpref || !ll->grad || !ll->gw1 || !ll->gw2 || !ll->gstk ||
+      !ll->unsold || !ll->lev_i || !ll->lev_j)
+    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: missing line-list array");
+  if (!o->llam || !o->vproj || !o->inv_df || !o->vbar2 || !o->e_kT || !o->ne || !o->tothi ||
+      !o->stark_pe || !o->lnbase || !o->chi || !o->eta || !o->cont || !o->logref)
+    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: missing cell array");
+  if (o->npart < 1) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: npart < 1");
+  if ((o->flags & M3DRT_OPAC_NLTE) && (!o->dep || o->nlev < 1))
+    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NLTE needs dep and nlev");
+  if ((o->flags & M3DRT_OPAC_SCAT) && (!o->one_m_eps || !o->chi_scat))
+    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: scattering needs one_m_eps and chi_scat");
+  if (o->sf < 1 || o->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::line_opacity_equi(ctx, ll, o);
 }
 
 }  // extern "C"

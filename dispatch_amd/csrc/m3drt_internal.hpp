@@ -12,6 +12,7 @@ struct m3drt_ctx {
   void *scratch = nullptr;   // per-sweep scratch (dt500, rc per cell); grown on demand
   size_t scratch_bytes = 0;
   int lds_max = 0;           // dynamic LDS bytes available per workgroup
+  float *qlg_tab = nullptr;  // QLG-bot's two tables on the device (2 x 9020 floats)
 };
 
 namespace m3drt {
@@ -29,6 +30,8 @@ int ensure_scratch(m3drt_ctx *ctx, size_t bytes);
 
 int sweep_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_lc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
+int qlg_tables(m3drt_ctx *ctx);
+int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
 
 }  // namespace m3drt
 

@@ -1,0 +1,306 @@
+// Line opacity on an equidistant ln(lambda) grid: the summation of
+// add_opacity_equi (HOP/linelist_mod.f90:729-960) with the QLG-bot Voigt kernel
+// (HOP/qlg_mod.f90:223-256, 524-653), for gfx950.
+//
+// Mapping.  Each cell walks its line list in order, because whether a line is
+// kept depends on the opacity the earlier lines already deposited at its centre
+// (the running tolerance floor, HOP/linelist_mod.f90:853-870).  So one lane owns
+// one cell and the line loop is uniform across a wavefront: lanes are consecutive
+// cells, and with cells at unit stride (the native layout) every deposit of a
+// wavefront at one frequency is one contiguous run.  QLG-bot's two 9020-entry
+// tables (72 KB) are staged in LDS once per workgroup.
+//
+// Arithmetic.  FP32 as the reference, each operation separately rounded (no fused
+// multiply-add) so that the window edges and the snapped table index, integer
+// decisions taken from floating-point values, come out as the reference's; exp,
+// log and pow are taken in FP64 and rounded once.  The reciprocal 1/(1+u) of the
+// per-point evaluation uses the hardware reciprocal (1 ulp).
+#include "m3drt_internal.hpp"
+#include <cmath>
+#include <vector>
+#include <algorithm>
+
+#pragma clang fp contract(off)
+
+namespace m3drt {
+
+namespace {
+
+constexpr int QLG_NX = 9019;
+
+struct OpacArgs {
+  int nline;
+  const double *llam0;
+  const float *lower_ev, *gpref, *grad, *gw1, *gw2, *gstk;
+  const int32_t *unsold, *lev_i, *lev_j, *ipart;
+  int32_t *used;
+  float *strength, *reach;
+  int nf, ncell, npart, nlev, nlte, scat;
+  const double *llam;
+  const float *vproj, *idf, *vb2, *ekt, *ne, *tothi, *spe, *lnbase, *dep;
+  float *chi, *eta, *csc;
+  const float *cont, *logref, *ome;
+  int64_t sf, sc;
+  float tol, rej;
+  const float *tgx, *th2x;
+};
+
+struct Coef {
+  float ih, bL, gG, cZH, cL2;
+};
+
+// bot_setup (HOP/qlg_mod.f90:524-540): QLG-bot's per-line constants, the paper's fit
+__device__ __forceinline__ Coef bot_setup(float gL, float D, float SQ, float S) {
+  Coef c;
+  const float e = D + gL * 0.60602209212341318f;
+  c.ih = e / (gL * e + SQ);
+  const float r = gL * c.ih;
+  const float q = r * (1.0f - r);
+  const float sc = S * c.ih;
+  const float qs = q * sc;
+  const float x = 2.0f * r - 1.0f;
+  c.bL = (0.3183098862f * r) * sc;
+  c.gG = (0.4697186f * (1.0f - r)) * sc + qs * (-0.064924393191013452f + x * (-0.11019326278794868f));
+  c.cZH = qs * (0.38494195930078284f + x * (-0.10567135279610407f));
+  c.cL2 = qs * (0.04054415746208978f + x * (0.16241029253712097f));
+  return c;
+}
+
+__device__ __forceinline__ int nint(float v) { return (int)roundf(v); }
+__device__ __forceinline__ int nint(double v) { return (int)round(v); }
+
+// atomic max of non-negative floats through their ordered bit patterns
+__device__ __forceinline__ void atomic_max_pos(float *p, float v) {
+  atomicMax(reinterpret_cast<int *>(p), __float_as_int(v));
+}
+
+__global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
+  extern __shared__ float tab[];  // TGX then TH2X
+  for (int i = threadIdx.x; i <= QLG_NX; i += blockDim.x) {
+    tab[i] = a.tgx[i];
+    tab[QLG_NX + 1 + i] = a.th2x[i];
+  }
+  __syncthreads();
+  const float *TGX = tab, *TH2X = tab + QLG_NX + 1;
+  const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= a.ncell) return;
+
+  const float sqrt_pi = 1.7724538509f, ex_cutoff = -85.0f, voigt_peak = 0.5641895835f;
+  const int nf = a.nf;
+  const double llam1 = a.llam[0];
+  const double dllam = (a.llam[nf - 1] - a.llam[0]) / (double)(nf - 1);
+  const double inv_dllam = 1.0 / dllam;
+  const float sdl = (float)inv_dllam;
+
+  float *chi = a.chi + cidx * a.sc, *eta = a.eta + cidx * a.sc;
+  const float *cont = a.cont + cidx * a.sc, *logref = a.logref + cidx * a.sc;
+  const int64_t sf = a.sf;
+
+  // the cell's Doppler width in grid steps and its cell-invariant terms (qlg_bot_cell)
+  const float idf = a.idf[cidx];
+  const float D = sdl / idf, D2 = D * D, SQ = 0.8325546112f * D2;
+  const float D2h = 0.5f * D2, D2G = 1.5f * D2, D2L = 5.0f * D2;
+  const float vb2 = a.vb2[cidx];
+  const float unsold_c = (float)pow((double)vb2, (double)0.3f) * a.tothi[cidx];
+  const float lvb2 = (float)log((double)vb2);
+  const float ekt = a.ekt[cidx], ne_c = a.ne[cidx], tothi_c = a.tothi[cidx];
+  const float starkpe = a.spe[cidx], vproj_c = a.vproj[cidx];
+
+  int f0_prev = -1;
+  float refl_c = 0.0f, lref_c = 0.0f;
+  for (int l = 0; l < a.nline; ++l) {
+    const double llc = a.llam0[l] - (double)vproj_c;
+    const double pos = (llc - llam1) * inv_dllam;
+    int f0 = nint(pos) + 1;
+    f0 = min(nf, max(1, f0));
+    const float ex = a.lower_ev[l] * ekt;
+    const int ip = a.ipart ? a.ipart[l] : 1;
+    const float arg = a.lnbase[(int64_t)(ip - 1) * a.ncell + cidx] + a.gpref[l] - ex;
+    if (arg < ex_cutoff) continue;
+    if (f0 != f0_prev) {
+      f0_prev = f0;
+      refl_c = a.tol * cont[(f0 - 1) * sf];
+      const float flo = fmaxf(a.tol, a.rej) * a.tol * chi[(f0 - 1) * sf];
+      if (flo > refl_c) {
+        refl_c = flo;
+        lref_c = (float)log((double)(flo * sqrt_pi));
+      } else {
+        lref_c = logref[(f0 - 1) * sf];
+      }
+    }
+    if (arg <= lref_c) continue;
+    const float Rij = (float)exp((double)arg);
+    const float ref = refl_c;
+    if (voigt_peak * Rij <= ref) continue;
+
+    const bool nlte = a.nlte && a.lev_i[l] > 0 && a.lev_j[l] > 0;
+    float bi = 1.0f, bj = 1.0f;
+    if (nlte) {
+      bi = a.dep[(int64_t)(a.lev_i[l] - 1) * a.ncell + cidx];
+      bj = a.dep[(int64_t)(a.lev_j[l] - 1) * a.ncell + cidx];
+    }
+    // damping, physical then in grid steps (HOP/linelist_mod.f90:895-906)
+    float gam = a.grad[l];
+    if (a.unsold[l])
+      gam = gam + a.gw1[l] * unsold_c;
+    else
+      gam = gam + a.gw1[l] * (float)exp((double)(a.gw2[l] * lvb2)) * tothi_c;
+    if (a.gstk[l] >= 0.0f)
+      gam = gam + a.gstk[l] * ne_c;
+    else
+      gam = gam - a.gstk[l] * starkpe;
+    gam = gam * sdl;
+    if (a.strength) atomic_max_pos(&a.strength[l], voigt_peak * Rij / cont[(f0 - 1) * sf]);
+    const float lnq = arg - lref_c;
+
+    // qlg_bot_line_sweep (HOP/qlg_mod.f90:630-653): setup, window, sweep
+    const float S = Rij * D;
+    const int ic = nint(pos) + 1;
+    const int i1 = min(nf, max(1, ic));
+    const float x1 = (float)((a.llam[i1 - 1] - llc) * inv_dllam);
+    const Coef c = bot_setup(gam, D, SQ, S);
+    const float AL = 0.3183098862f * gam * S / ref;
+    const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
+                                       AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
+    if (a.reach) atomic_max_pos(&a.reach[l], (float)((double)hw * dllam));
+    const int noff = nint(fminf(fminf(hw, 3.40282347e38f), (float)(nf + abs(ic))));
+    const int fa = max(1, ic - noff);
+    const int fb = min(nf, ic + noff);
+    if (fa > fb) continue;
+    if (a.used) a.used[l] = 1;
+    const float x0 = x1 - (float)(i1 - fa);
+    const bool plain = !(a.scat && !nlte);
+    for (int f = fa; f <= fb; ++f) {
+      // qlg_bot_line_eval (HOP/qlg_mod.f90:542-556)
+      const float x = x0 + (float)(f - fa);
+      const float xs = fabsf(x) * c.ih;
+      const float u = xs * xs;
+      const float L = __builtin_amdgcn_rcpf(1.0f + u);
+      const int ig = nint(fminf(xs * 800.0f, (float)QLG_NX));
+      const float G = TGX[ig], W = TH2X[ig];
+      float p = c.cL2 + c.cZH * W;
+      p = c.bL + L * p;
+      const float lte = G * c.gG + L * p;
+      const int64_t o = (int64_t)(f - 1) * sf;
+      if (plain) {
+        chi[o] = chi[o] + lte * bi;
+        eta[o] = eta[o] + lte * bj;
+      } else {
+        chi[o] = chi[o] + lte;
+        eta[o] = eta[o] + lte;
+        a.csc[cidx * a.sc + o] = a.csc[cidx * a.sc + o] + a.ome[cidx * a.sc + o] * lte;
+      }
+    }
+    f0_prev = -1;
+  }
+}
+
+// ------------------------------------------------------------------ tables
+// QLG-bot's tables (HOP/qlg_mod.f90:364-375, 382-395): 2^-u and Z_0.2(u)/L^2 on
+// u_i = (i/800)^2, built in FP64 and rounded once.  Z_0.2 needs an FP64 Voigt
+// function: Weideman's (1994) N = 32 rational approximation, its coefficients
+// from the paper's DFT recipe, with Humlicek's region-I asymptote beyond
+// |v| + a = 15.
+
+struct W32 {
+  double a[32];
+  double L;
+};
+
+static W32 weideman32() {
+  W32 w;
+  const int N = 32, M = 2 * N, M2 = 2 * M;
+  const double PI = 3.14159265358979323846;
+  w.L = std::sqrt(N / std::sqrt(2.0));
+  std::vector<double> g(M2, 0.0), f(M2);
+  for (int k = -M + 1; k <= M - 1; k++) {
+    const double t = w.L * std::tan(k * PI / M / 2);
+    g[k + M] = std::exp(-t * t) * (w.L * w.L + t * t);
+  }
+  for (int i = 0; i < M2; i++) f[i] = g[(i + M) % M2];
+  for (int j = 1; j <= N; j++) {
+    double re = 0.0;
+    for (int i = 0; i < M2; i++) re += f[i] * std::cos(2 * PI * (double)i * j / M2);
+    w.a[N - j] = re / M2;
+  }
+  return w;
+}
+
+static double voigt_f64(const W32 &w, double a, double v) {
+  const double ISQPI = 0.564189583547756287;
+  const double av = std::fabs(v);
+  if (av + a >= 15.0) {
+    const double tr = a * a - av * av + 0.5, ti = -2.0 * a * av;
+    return ISQPI * ((a * tr - av * ti) / (tr * tr + ti * ti));
+  }
+  const double dr = w.L + a, nr0 = w.L - a, di = -v;
+  const double q = 1.0 / (dr * dr + di * di);
+  const double zr = (nr0 * dr + v * di) * q, zi = (v * dr - nr0 * di) * q;
+  double pr = w.a[0], pim = 0.0;
+  for (int j = 1; j < 32; j++) {
+    const double t = pr * zr - pim * zi + w.a[j];
+    pim = pr * zi + pim * zr;
+    pr = t;
+  }
+  const double ir = dr * q, ii = -di * q;
+  return 2.0 * (pr * (ir * ir - ii * ii) - pim * (2.0 * ir * ii)) + ISQPI * ir;
+}
+
+}  // namespace
+
+int qlg_tables(m3drt_ctx *ctx) {
+  if (ctx->qlg_tab) return M3DRT_OK;
+  const W32 w = weideman32();
+  std::vector<float> tab(2 * (QLG_NX + 1));
+  // anchor of the Z_0.2 slice: damping a* with a*/Gamma(a*) = 0.2, its half-width,
+  // peak erfcx(a*) and Lorentz share, as the fitted kernel defines them
+  const double A2 = 0.18741813853432507, GAM = 0.93709069267162530, KP = 0.81924853602069690;
+  const double ETA = 0.14697983683153040, QRS = 0.16, LN2 = 0.69314718055994531;
+  for (int i = 0; i <= QLG_NX; i++) {
+    const double x = (double)i / 800.0, u = x * x;
+    tab[i] = (float)std::pow(2.0, -u);
+    const double L = 1.0 / (1.0 + u), G = std::exp(-LN2 * u);
+    const double K = voigt_f64(w, A2, GAM * std::sqrt(u));
+    const double P = KP * (ETA * L + (1.0 - ETA) * G);
+    tab[QLG_NX + 1 + i] = (float)((K - P) / (QRS * KP) * ((1.0 + u) * (1.0 + u)));
+  }
+  void *d = nullptr;
+  M3DRT_HIP(hipMalloc(&d, tab.size() * sizeof(float)));
+  M3DRT_HIP(hipMemcpy(d, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->qlg_tab = static_cast<float *>(d);
+  return M3DRT_OK;
+}
+
+int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
+  int rc = qlg_tables(ctx);
+  if (rc) return rc;
+  OpacArgs a;
+  a.nline = ll->nline;
+  a.llam0 = ll->llam0;
+  a.lower_ev = ll->lower_ev; a.gpref = ll->gpref; a.grad = ll->grad;
+  a.gw1 = ll->gw1; a.gw2 = ll->gw2; a.gstk = ll->gstk;
+  a.unsold = ll->unsold; a.lev_i = ll->lev_i; a.lev_j = ll->lev_j; a.ipart = ll->ipart;
+  a.used = ll->used; a.strength = ll->strength; a.reach = ll->reach;
+  a.nf = o->nf; a.ncell = o->ncell; a.npart = o->npart; a.nlev = o->nlev;
+  a.nlte = (o->flags & M3DRT_OPAC_NLTE) ? 1 : 0;
+  a.scat = (o->flags & M3DRT_OPAC_SCAT) ? 1 : 0;
+  a.llam = o->llam;
+  a.vproj = o->vproj; a.idf = o->inv_df; a.vb2 = o->vbar2; a.ekt = o->e_kT; a.ne = o->ne;
+  a.tothi = o->tothi; a.spe = o->stark_pe; a.lnbase = o->lnbase; a.dep = o->dep;
+  a.chi = o->chi; a.eta = o->eta; a.csc = o->chi_scat;
+  a.cont = o->cont; a.logref = o->logref; a.ome = o->one_m_eps;
+  a.sf = o->sf; a.sc = o->sc;
+  a.tol = o->tolerance; a.rej = o->line_reject_frac;
+  a.tgx = ctx->qlg_tab;
+  a.th2x = ctx->qlg_tab + QLG_NX + 1;
+  const int threads = 256;
+  const int blocks = (o->ncell + threads - 1) / threads;
+  const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
+  M3DRT_HIP(hipFuncSetAttribute((const void *)opacity_equi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  hipLaunchKernelGGL(opacity_equi_kernel, dim3(blocks), dim3(threads), lds, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
+
+}  // namespace m3drt

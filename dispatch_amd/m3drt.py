@@ -169,3 +169,72 @@ class Context:
                       p(escape_out), p(exit_I))
         self._bind_stream()
         _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")
+
+
+# ------------------------------------------------------------------ line opacity
+
+M3DRT_OPAC_NLTE, M3DRT_OPAC_SCAT = 1, 2
+
+
+class Linelist(C.Structure):
+    _fields_ = [("nline", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk", "unsold", "lev_i", "lev_j",
+        "ipart", "used", "strength", "reach")]
+
+
+class OpacityArgs(C.Structure):
+    _fields_ = [("nf", C.c_int32), ("ncell", C.c_int32), ("npart", C.c_int32), ("nlev", C.c_int32),
+                ("flags", C.c_int32), ("tolerance", C.c_float), ("line_reject_frac", C.c_float),
+                ("sf", C.c_int64), ("sc", C.c_int64)] + [(n, C.c_void_p) for n in (
+        "llam", "vproj", "inv_df", "vbar2", "e_kT", "ne", "tothi", "stark_pe", "lnbase", "dep",
+        "chi", "eta", "cont", "logref", "one_m_eps", "chi_scat")]
+
+
+def _opacity_lib():
+    L = lib()
+    if not getattr(L, "_opac_ok", False):
+        L.m3drt_line_opacity_equi.argtypes = [C.c_void_p, C.POINTER(Linelist), C.POINTER(OpacityArgs)]
+        L.m3drt_line_opacity_equi.restype = C.c_int
+        L._opac_ok = True
+    return L
+
+
+def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None,
+                       nlte=False, one_m_eps=None, chi_scat=None, layout="native", tolerance=1e-4,
+                       line_reject_frac=0.0, used=None, strength=None, reach=None):
+    """add_opacity_equi of one isotope's line list on the device.  lines: dict of
+    device tensors (llam0 float64; lower_ev, gpref, grad, gw1, gw2, gstk float32;
+    unsold, lev_i, lev_j, optional ipart int32).  Spectra (chi, eta, cont, logref,
+    one_m_eps, chi_scat) are (nf, ncell) in the native layout (cells at unit
+    stride) or (ncell, nf) in the reference layout (f at unit stride)."""
+    L = _opacity_lib()
+    nl = int(lines["llam0"].numel())
+    if layout == "native":
+        nf, ncell = chi.shape
+        sf, sc = ncell, 1
+    elif layout == "reference":
+        ncell, nf = chi.shape
+        sf, sc = 1, nf
+    else:
+        raise M3drtError(f"unknown layout {layout!r}")
+    for t in (chi, eta, cont, logref, one_m_eps, chi_scat):
+        if t is not None and (t.dtype != torch.float32 or tuple(t.shape) != tuple(chi.shape)
+                              or not t.is_cuda or not t.is_contiguous()):
+            raise M3drtError("spectra must be contiguous float32 device tensors shaped like chi")
+    if llam.dtype != torch.float64 or llam.numel() != nf:
+        raise M3drtError("llam is float64 with nf nodes")
+    npart = lnbase.shape[0]
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    ll = Linelist(nl, *[p(lines.get(k)) for k in ("llam0", "lower_ev", "gpref", "grad", "gw1", "gw2",
+                                                  "gstk", "unsold", "lev_i", "lev_j", "ipart")],
+                  p(used), p(strength), p(reach))
+    flags = (M3DRT_OPAC_NLTE if nlte else 0) | (M3DRT_OPAC_SCAT if chi_scat is not None else 0)
+    o = OpacityArgs(nf, ncell, npart, 0 if dep is None else dep.shape[0], flags, tolerance,
+                    line_reject_frac, sf, sc, p(llam), *[p(cells[k]) for k in (
+                        "vproj", "idf", "vb2", "ekt", "ne", "tothi", "spe")],
+                    p(lnbase), p(dep), p(chi), p(eta), p(cont), p(logref), p(one_m_eps), p(chi_scat))
+    self._bind_stream()
+    _check(L.m3drt_line_opacity_equi(self._h, C.byref(ll), C.byref(o)), "m3drt_line_opacity_equi")
+
+
+Context.line_opacity_equi = _line_opacity_equi

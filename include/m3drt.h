@@ -111,6 +111,52 @@ typedef struct {
  */
 int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs);
 
+/* ------------------------------------------------------------------ line opacity */
+
+/* flags of m3drt_opacity_args */
+#define M3DRT_OPAC_NLTE 1   /* lines with both levels in dep take b_i, b_j (do_nlte) */
+#define M3DRT_OPAC_SCAT 2   /* LTE lines also add (1-eps)*K to chi_scat (do_scat) */
+
+/* One isotope's line list, structure of arrays, ascending in llam0
+ * (linelist_type, HOP/line_types_mod.f90:22-74).  Device pointers. */
+typedef struct {
+  int32_t nline;
+  const double *llam0;                /* ln(lambda0) of the line centres */
+  const float *lower_ev, *gpref;      /* E_low [eV]; loggf*ln10 + log(const/nu0) */
+  const float *grad, *gw1, *gw2, *gstk;  /* damping recipes, scaled by 1/(4 pi nu0) */
+  const int32_t *unsold;              /* nonzero: Unsold van der Waals, else Barklem */
+  const int32_t *lev_i, *lev_j;       /* 1-based levels in dep, 0: none */
+  const int32_t *ipart;               /* 1-based population group of each line; NULL: 1 */
+  int32_t *used;                      /* out (may be NULL): set where the line deposits */
+  float *strength, *reach;            /* out (may be NULL): maxima over the cells */
+} m3drt_linelist;
+
+/* The cells and spectra of one call.  (f, cell) arrays are addressed f*sf + cell*sc
+ * (elements); per-cell arrays are (ncell); lnbase (ncell, npart) and dep (ncell,
+ * nlev) cell-fastest, as the reference's (m1,m2,m3,n) arrays. */
+typedef struct {
+  int32_t nf, ncell, npart, nlev, flags;
+  float tolerance;                    /* bbtrans_mod tolerance, default 1e-4 */
+  float line_reject_frac;             /* default 0 */
+  int64_t sf, sc;
+  const double *llam;                 /* (nf) ascending ln(lambda) grid, equidistant */
+  const float *vproj, *inv_df, *vbar2, *e_kT, *ne, *tothi, *stark_pe;
+  const float *lnbase;                /* ln n - ln U + ln inv_df per population group */
+  const float *dep;                   /* departure coefficients (NLTE) */
+  float *chi, *eta;                   /* accumulated line opacity and emissivity */
+  const float *cont, *logref;         /* continuum and ln(tol cont / voigt_peak) */
+  const float *one_m_eps;             /* scattering fraction (M3DRT_OPAC_SCAT) */
+  float *chi_scat;                    /* accumulated scattering opacity (M3DRT_OPAC_SCAT) */
+} m3drt_opacity_args;
+
+/*
+ * Adds the line opacity of one isotope's line list to chi and eta with the
+ * QLG-bot Voigt kernel.  Replaces add_opacity_equi, HOP/linelist_mod.f90:729-960
+ * (without reach-based chunking), as add_particle_opacity (HOP/linelist_mod.f90:
+ * 970-1070) calls it per isotope.
+ */
+int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
+
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */
 int m3drt_native_strides(m3drt_sweep_desc *desc);
