@@ -199,4 +199,50 @@ int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3dr
   return m3drt::line_opacity_equi(ctx, ll, o);
 }
 
+static int check_atom(const m3drt_rate_solve_args *r, const char *who) {
+  char msg[160];
+  if (r->nk < 1 || r->nrad < 0 || r->nline < 0 || r->nline > r->nrad || r->npair < 0 || r->ncell < 0) {
+    snprintf(msg, sizeof msg, "%s: inconsistent sizes", who);
+    return set_error(M3DRT_EINVAL, msg);
+  }
+  if ((r->nrad && (!r->trans_i || !r->trans_j || !r->trans_active || !r->gamma_ij || !r->gamma_ji)) ||
+      !r->nstar) {
+    snprintf(msg, sizeof msg, "%s: missing transition or population array", who);
+    return set_error(M3DRT_EINVAL, msg);
+  }
+  return M3DRT_OK;
+}
+
+int m3drt_rate_solve_f64(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_rate_solve_f64: NULL argument");
+  int rc = check_atom(r, "m3drt_rate_solve_f64");
+  if (rc) return rc;
+  if (!r->g || !r->ev || !r->e_kT || !r->temp || !r->totn || !r->b || (r->nline && !r->Aji) ||
+      (r->npair && r->cup && (!r->pair_i || !r->pair_j)))
+    return set_error(M3DRT_EINVAL, "m3drt_rate_solve_f64: missing atom or cell array");
+  if (res) {
+    res->maxerr = 0;
+    res->epos = -1;
+    res->elvl = -1;
+    res->nclamp = 0;
+  }
+  if (r->ncell == 0) return M3DRT_OK;
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  rc = m3drt::rate_solve(ctx, r, res);
+  if (rc == M3DRT_OK && res && res->maxerr == 0.0) {
+    res->epos = -1;
+    res->elvl = -1;
+  }
+  return rc;
+}
+
+int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_normalize_rates: NULL argument");
+  int rc = check_atom(r, "m3drt_normalize_rates");
+  if (rc) return rc;
+  if (r->ncell == 0 || r->nrad == 0) return M3DRT_OK;
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::normalize_rates(ctx, r, wphi);
+}
+
 }  // extern "C"

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -238,3 +239,93 @@ def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase,
 
 
 Context.line_opacity_equi = _line_opacity_equi
+
+
+# ------------------------------------------------------------------ statistical equilibrium
+
+class RateSolveArgs(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("nrad", C.c_int32), ("nline", C.c_int32), ("npair", C.c_int32),
+                ("ncell", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "trans_i", "trans_j", "trans_active", "g", "ev", "Aji", "pair_i", "pair_j", "e_kT", "temp", "totn",
+        "nstar", "b", "gamma_ij", "gamma_ji", "cup")] + [(n, C.c_double) for n in (
+        "max_update", "min_pop_frac", "conv_pop_floor", "sor_w", "cswitch", "cut_x", "bf_cut_rate",
+        "tmax_update")]
+
+
+class RateSolveResult(C.Structure):
+    _fields_ = [("maxerr", C.c_double), ("epos", C.c_int32), ("elvl", C.c_int32), ("nclamp", C.c_int64),
+                ("lvlerr", C.c_void_p)]
+
+
+SE_DEFAULTS = dict(max_update=1e2, min_pop_frac=1e-30, conv_pop_floor=0.0, sor_w=1.0, cswitch=1.0,
+                   cut_x=70.0, bf_cut_rate=1.0, tmax_update=1e7)
+
+
+class AtomOnDevice:
+    """A model atom's constant arrays on the device (see synth.Atom)."""
+
+    def __init__(self, atom, active=None):
+        d = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+        self.nk, self.nline = atom.nk, atom.nline
+        self.nrad, self.npair = len(atom.ti), len(atom.pi)
+        self.ti = d(atom.ti, torch.int32)
+        self.tj = d(atom.tj, torch.int32)
+        self.active = d(np.ones(self.nrad) if active is None else active, torch.int32)
+        self.g = d(atom.g, torch.float64)
+        self.ev = d(atom.ev, torch.float64)
+        self.Aji = d(atom.Aji, torch.float64)
+        self.pi = d(atom.pi, torch.int32)
+        self.pj = d(atom.pj, torch.int32)
+
+
+def _rate_args(atom, e_kT, temp, totn, nstar, b, gamma_ij, gamma_ji, cup, params):
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    ncell = int(totn.numel())
+    for t, dt in ((e_kT, torch.float64), (temp, torch.float64), (totn, torch.float64), (nstar, torch.float64),
+                  (b, torch.float64), (gamma_ij, torch.float32), (gamma_ji, torch.float32)):
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+            raise M3drtError("rate-solve arrays: contiguous device tensors of the documented dtypes")
+    prm = {**SE_DEFAULTS, **params}
+    return RateSolveArgs(atom.nk, atom.nrad, atom.nline, atom.npair, ncell, p(atom.ti), p(atom.tj),
+                         p(atom.active), p(atom.g), p(atom.ev), p(atom.Aji), p(atom.pi), p(atom.pj),
+                         p(e_kT), p(temp), p(totn), p(nstar), p(b), p(gamma_ij), p(gamma_ji), p(cup),
+                         *[float(prm[k]) for k in ("max_update", "min_pop_frac", "conv_pop_floor", "sor_w",
+                                                    "cswitch", "cut_x", "bf_cut_rate", "tmax_update")])
+
+
+def _se_lib():
+    L = lib()
+    if not getattr(L, "_se_ok", False):
+        L.m3drt_rate_solve_f64.argtypes = [C.c_void_p, C.POINTER(RateSolveArgs), C.POINTER(RateSolveResult)]
+        L.m3drt_rate_solve_f64.restype = C.c_int
+        L.m3drt_normalize_rates.argtypes = [C.c_void_p, C.POINTER(RateSolveArgs), C.c_void_p]
+        L.m3drt_normalize_rates.restype = C.c_int
+        L._se_ok = True
+    return L
+
+
+def _rate_solve(self, atom, e_kT, temp, totn, nstar, b, gamma_ij, gamma_ji, cup=None, **params):
+    """update_populations on the device: b is updated in place; returns the
+    convergence measure (maxerr, epos, elvl, nclamp, lvlerr)."""
+    L = _se_lib()
+    a = _rate_args(atom, e_kT, temp, totn, nstar, b, gamma_ij, gamma_ji, cup, params)
+    lvl = np.zeros(atom.nk)
+    res = RateSolveResult(0.0, -1, -1, 0, lvl.ctypes.data_as(C.c_void_p))
+    self._bind_stream()
+    _check(L.m3drt_rate_solve_f64(self._h, C.byref(a), C.byref(res)), "m3drt_rate_solve_f64")
+    return dict(maxerr=res.maxerr, epos=res.epos, elvl=res.elvl, nclamp=res.nclamp, lvlerr=lvl)
+
+
+def _normalize_rates(self, atom, nstar, gamma_ij, gamma_ji, wphi=None):
+    """normalize_profiles' scaling of the rate sums, in place on the device."""
+    L = _se_lib()
+    totn = torch.zeros(nstar.shape[1], dtype=torch.float64, device="cuda")
+    a = _rate_args(atom, totn, totn, totn, nstar, totn, gamma_ij, gamma_ji, None, {})
+    a.b = None
+    self._bind_stream()
+    _check(L.m3drt_normalize_rates(self._h, C.byref(a), None if wphi is None else C.c_void_p(wphi.data_ptr())),
+           "m3drt_normalize_rates")
+
+
+Context.rate_solve = _rate_solve
+Context.normalize_rates = _normalize_rates

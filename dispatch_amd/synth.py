@@ -207,3 +207,99 @@ def ghost_planes(box: Box, chi_cube, dtype):
     zero = torch.zeros_like(chi_b)
     return dict(up=(chi_b.contiguous(), S_b.contiguous(), S_b.clone()),
                 down=(zero, zero.clone(), zero.clone()))
+
+
+# ------------------------------------------------------------------ model atom
+
+EV_K = 11604.518  # K per eV
+SAHA = 2.4146830e15  # (2 pi m_e k / h^2)^(3/2) [cm^-3 K^-3/2]
+
+
+@dataclass
+class Atom:
+    """A synthetic model atom: nk-1 bound levels of one ion plus the ground level
+    of the next ion (the continuum).  Transitions are lines first, then continua;
+    levels and transitions are 1-based as in the reference's atom files."""
+    nk: int
+    ev: np.ndarray        # (nk) level energies [eV]; the last is the ionisation limit
+    g: np.ndarray         # (nk) statistical weights
+    ion: np.ndarray       # (nk) ion stage (1 bound, 2 continuum)
+    ti: np.ndarray        # (nrad) lower level
+    tj: np.ndarray        # (nrad) upper level
+    nline: int
+    f: np.ndarray         # (nline) oscillator strengths
+    Aji: np.ndarray       # (nline) [1/s]
+    nu0: np.ndarray       # (nrad) transition frequency [Hz]
+    sigma0: np.ndarray    # (ncont) photo-ionisation cross section at the edge [cm^2]
+    pi: np.ndarray        # (npair) collisional pairs
+    pj: np.ndarray
+
+    @property
+    def nrad(self):
+        return len(self.ti)
+
+
+def make_atom(nbound=20, nline=80, ncont=20, seed=20261204, chi_ion=7.6):
+    """20 bound levels with energies sorted uniform in 0-7 eV plus the continuum; 80
+    bound-bound pairs with f log-uniform in 1e-4..1, 20 bound-free with hydrogenic
+    nu^-3 cross sections of edge value 1e-18 cm^2."""
+    rng = np.random.default_rng(seed)
+    ev = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 7.0, nbound - 1)), [chi_ion]])
+    g = np.concatenate([rng.integers(1, 10, nbound).astype(np.float64), [1.0]])
+    ion = np.concatenate([np.ones(nbound, np.int32), [2]])
+    pairs = [(i, j) for i in range(1, nbound + 1) for j in range(i + 1, nbound + 1)]
+    sel = rng.choice(len(pairs), size=nline, replace=False)
+    lines = sorted(pairs[s] for s in sel)
+    ti = [p[0] for p in lines] + list(range(1, ncont + 1))
+    tj = [p[1] for p in lines] + [nbound + 1] * ncont
+    f = 10 ** rng.uniform(-4, 0, nline)
+    nu0 = np.array([(ev[j - 1] - ev[i - 1]) * 2.417989242e14 for i, j in zip(ti, tj)])
+    gi = g[np.array(ti[:nline]) - 1]
+    gj = g[np.array(tj[:nline]) - 1]
+    Aji = 6.670e15 * gi / gj * f / (C_LIGHT / nu0[:nline] * 1e8) ** 2
+    pi_, pj_ = [], []
+    for i, j in zip(ti, tj):
+        pi_.append(i)
+        pj_.append(j)
+    return Atom(nbound + 1, ev, g, ion, np.array(ti, np.int32), np.array(tj, np.int32), nline, f, Aji,
+                nu0, np.full(ncont, 1e-18), np.array(pi_, np.int32), np.array(pj_, np.int32))
+
+
+def lte_populations(atom: Atom, T, ne, totn):
+    """Saha-Boltzmann LTE populations (nk, ncell) normalised to totn."""
+    T = np.asarray(T, np.float64)
+    kT = T / EV_K
+    bound = atom.ion == 1
+    w = np.empty((atom.nk, T.size))
+    w[bound] = atom.g[bound, None] * np.exp(-atom.ev[bound, None] / kT[None])
+    # next ion's ground, relative to the neutral ground (g = g_ground of the ion)
+    chi = atom.ev[-1]
+    w[-1] = (atom.g[-1] * 2.0 / atom.g[0]) * SAHA * T ** 1.5 * np.exp(-chi / kT) / ne * atom.g[0]
+    w *= totn / w.sum(axis=0)
+    return w
+
+
+def van_regemorter(atom: Atom, T, ne, nstar):
+    """Upward collision rates (npair, ncell): van Regemorter for the lines
+    (14.5 Allen Ryd/kT f max(0.2 e^-u, sqrt3/2pi E1(u)) / dE, SRC/atom_mod.f90:
+    1790-1838 with the recipe's constants) and a Seaton-type ionisation rate."""
+    from scipy.special import exp1
+    T = np.asarray(T, np.float64)
+    sqt = np.sqrt(T)
+    e_kT = EV_K / T
+    rydk, e_k = 157887.51, 11604.518
+    c0 = np.pi * (0.529177e-8) ** 2 * np.sqrt(8 * K_BOLTZ / 9.1093837e-28 / np.pi)
+    allen = c0 * rydk * ne / sqt
+    regem = 14.5 * allen * rydk / e_k
+    seaton = 1.55e13 * ne / sqt
+    out = np.zeros((len(atom.pi), T.size))
+    for q, (i, j) in enumerate(zip(atom.pi, atom.pj)):
+        dev = max(atom.ev[j - 1] - atom.ev[i - 1], 1e-8)
+        u0 = dev * e_kT
+        ex = np.exp(-u0)
+        if q < atom.nline:
+            f = max(atom.f[q], 1e-6 * atom.g[j - 1])
+            out[q] = regem * f * np.maximum(0.2 * ex, np.sqrt(3) / (2 * np.pi) * exp1(u0)) / dev
+        else:
+            out[q] = seaton * 0.1 * 0.1 / u0 * ex
+    return out

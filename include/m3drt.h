@@ -157,6 +157,58 @@ typedef struct {
  */
 int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
 
+/* ------------------------------------------------------------------ statistical equilibrium */
+
+/* A model atom and its cells, FP64 except the radiative rate sums.  Per-cell
+ * arrays are cell-fastest: nstar[i*ncell + c] (the reference's (x,y,z,level)),
+ * gamma_ij[kr*ncell + c], cup[q*ncell + c].  Levels and transitions 1-based. */
+typedef struct {
+  int32_t nk, nrad, nline, npair, ncell;   /* levels, transitions (lines first), pairs, cells */
+  const int32_t *trans_i, *trans_j;        /* (nrad) lower, upper level */
+  const int32_t *trans_active;             /* (nrad) */
+  const double *g, *ev;                    /* (nk) statistical weight, energy [eV] */
+  const double *Aji;                       /* (nline) Einstein A, the pair of a cut line */
+  const int32_t *pair_i, *pair_j;          /* (npair) collisional pairs i < j */
+  const double *e_kT, *temp, *totn;        /* (ncell) 1 eV/kT, T [K], atoms per cm^3 */
+  const double *nstar;                     /* (nk, ncell) LTE populations */
+  double *b;                               /* (nk, ncell) departure coefficients, updated */
+  float *gamma_ij, *gamma_ji;              /* (nrad, ncell) rate sums (thermal, deviation for lines) */
+  const double *cup;                       /* (npair, ncell) upward collision rates [1/s]; or NULL */
+  double max_update;                       /* &nlte_params defaults: 1e2 */
+  double min_pop_frac;                     /* 1e-30 */
+  double conv_pop_floor;                   /* 0 */
+  double sor_w;                            /* 1 */
+  double cswitch;                          /* 1 */
+  double cut_x;                            /* 70 */
+  double bf_cut_rate;                      /* 1 [1/s] */
+  double tmax_update;                      /* 1e7 K */
+} m3drt_rate_solve_args;
+
+typedef struct {
+  double maxerr;      /* signed largest (weighted) relative change of any level */
+  int32_t epos;       /* its cell (0-based, x fastest), -1 if none */
+  int32_t elvl;       /* its level (0-based) */
+  int64_t nclamp;     /* corrections cut by the limiter */
+  double *lvlerr;     /* host array (nk) of each level's signed largest change, or NULL */
+} m3drt_rate_solve_result;
+
+/*
+ * One statistical-equilibrium update of every cell: the rate matrix of
+ * setup_transition_matrix (SRC/atom_mod.f90:740-851) with line_rate_coefficients
+ * (862-884), solved by lineq_no_pivot (HOP/op_math_mod.f90:420-449), then
+ * update_populations' limiter, under-relaxation, conservation and floor
+ * (SRC/atom_mod.f90:439-732).  Returns the convergence measure the NLTE task
+ * compares with convlim (SRC/nlte_patch_mod.f90:711-770).  A cell without a
+ * finite solution fails the call with its index, as the reference stops there.
+ * Synchronizes the context's stream to return the result.
+ */
+int m3drt_rate_solve_f64(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, m3drt_rate_solve_result *res);
+
+/* normalize_profiles (SRC/atom_mod.f90:377-431): line rate sums times the inverted
+ * profile integrals wphi (nline, ncell; NULL: none), continuum downward sums times
+ * n*_i/n*_j.  Uses trans_*, nline, nrad, ncell, nstar, gamma_ij/ji of args. */
+int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, const float *wphi);
+
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */
 int m3drt_native_strides(m3drt_sweep_desc *desc);

@@ -342,3 +342,113 @@ def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=Non
                            None if csc is None else csc.ctypes.data_as(C.c_void_p),
                            int(csc is not None), tol, line_reject_frac)
     return dict(chi=chi, eta=eta, used=used, strength=strength, reach=reach, chi_scat=csc)
+
+
+# ---------------------------------------------------------------- statistical equilibrium
+
+class OrcAtom(C.Structure):
+    _fields_ = [("nk", C.c_int), ("nrad", C.c_int), ("nline", C.c_int), ("npair", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("ti", "tj", "active", "g", "ev", "Aji", "pi", "pj")]
+
+
+class SeParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("max_update", "min_pop_frac", "conv_pop_floor", "sor_w",
+                                          "cswitch", "cut_x", "bf_cut_rate", "tmax_update")]
+
+
+class SeResult(C.Structure):
+    _fields_ = [("maxerr", C.c_double), ("epos", C.c_int), ("elvl", C.c_int), ("nclamp", C.c_long),
+                ("lvlerr", C.c_void_p)]
+
+
+SE_DEFAULTS = dict(max_update=1e2, min_pop_frac=1e-30, conv_pop_floor=0.0, sor_w=1.0, cswitch=1.0,
+                   cut_x=70.0, bf_cut_rate=1.0, tmax_update=1e7)
+
+
+def _se_declare():
+    L = lib()
+    if not getattr(L, "_se_ok", False):
+        vp = C.c_void_p
+        L.orc_lineq_no_pivot.argtypes = [vp, vp, C.c_int]
+        L.orc_lineq_no_pivot.restype = None
+        L.orc_normalize_rates.argtypes = [C.POINTER(OrcAtom), C.c_int, vp, vp, vp, vp]
+        L.orc_normalize_rates.restype = None
+        L.orc_transition_matrix.argtypes = [C.POINTER(OrcAtom), C.POINTER(SeParams), C.c_int, C.c_int] + [vp] * 9
+        L.orc_transition_matrix.restype = None
+        L.orc_update_populations.argtypes = [C.POINTER(OrcAtom), C.POINTER(SeParams), C.c_int] + [vp] * 8 + \
+                                             [C.POINTER(SeResult)]
+        L.orc_update_populations.restype = C.c_int
+        L._se_ok = True
+    return L
+
+
+class _Keep(list):
+    def p(self, a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dt)
+        self.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+def _atom(atom, keep, active=None):
+    nrad = len(atom.ti)
+    act = np.ones(nrad, np.int32) if active is None else active
+    return OrcAtom(atom.nk, nrad, atom.nline, len(atom.pi), keep.p(atom.ti, np.int32), keep.p(atom.tj, np.int32),
+                   keep.p(act, np.int32), keep.p(atom.g, np.float64), keep.p(atom.ev, np.float64),
+                   keep.p(atom.Aji, np.float64), keep.p(atom.pi, np.int32), keep.p(atom.pj, np.int32))
+
+
+def lineq_no_pivot(A, b):
+    L = _se_declare()
+    a = np.asfortranarray(A, np.float64).copy(order="F")
+    x = np.array(b, np.float64, copy=True)
+    L.orc_lineq_no_pivot(a.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), len(x))
+    return x
+
+
+def normalize_rates(atom, nstar, gij, gji, wphi=None):
+    L = _se_declare()
+    keep = _Keep()
+    at = _atom(atom, keep)
+    gij = np.array(gij, np.float32, copy=True)
+    gji = np.array(gji, np.float32, copy=True)
+    ncell = gij.shape[1]
+    L.orc_normalize_rates(C.byref(at), ncell, keep.p(nstar, np.float64), keep.p(wphi, np.float32),
+                          gij.ctypes.data_as(C.c_void_p), gji.ctypes.data_as(C.c_void_p))
+    return gij, gji
+
+
+def transition_matrix(atom, cell, e_kT, totn, nstar, b, gij, gji, cup=None, **params):
+    L = _se_declare()
+    keep = _Keep()
+    at = _atom(atom, keep)
+    prm = SeParams(**{**SE_DEFAULTS, **params})
+    nk = atom.nk
+    w = np.zeros((nk, nk), np.float64, order="F")
+    e = np.zeros(nk)
+    ncell = np.asarray(totn).size
+    L.orc_transition_matrix(C.byref(at), C.byref(prm), ncell, cell, keep.p(e_kT, np.float64),
+                            keep.p(totn, np.float64), keep.p(nstar, np.float64), keep.p(b, np.float64),
+                            keep.p(gij, np.float32), keep.p(gji, np.float32), keep.p(cup, np.float64),
+                            w.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p))
+    return w, e
+
+
+def update_populations(atom, e_kT, temp, totn, nstar, b, gij, gji, cup=None, **params):
+    """Restated update_populations over all cells; returns (b, result dict)."""
+    L = _se_declare()
+    keep = _Keep()
+    at = _atom(atom, keep)
+    prm = SeParams(**{**SE_DEFAULTS, **params})
+    b = np.array(b, np.float64, copy=True)
+    lvl = np.zeros(atom.nk)
+    res = SeResult(0.0, -1, -1, 0, lvl.ctypes.data_as(C.c_void_p))
+    ncell = np.asarray(totn).size
+    st = L.orc_update_populations(C.byref(at), C.byref(prm), ncell, keep.p(e_kT, np.float64),
+                                  keep.p(temp, np.float64), keep.p(totn, np.float64),
+                                  keep.p(nstar, np.float64), b.ctypes.data_as(C.c_void_p),
+                                  keep.p(gij, np.float32), keep.p(gji, np.float32),
+                                  keep.p(cup, np.float64), C.byref(res))
+    return b, dict(status=st, maxerr=res.maxerr, epos=res.epos, elvl=res.elvl, nclamp=res.nclamp,
+                   lvlerr=lvl)
