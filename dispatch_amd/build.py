@@ -90,6 +90,17 @@ def build_fortran(verbose: bool = False):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    # the Fortran host driver the GPU tests run (tests/fortran)
+    drv = os.path.join(ROOT, "tests", "fortran", "sweep_driver.f90")
+    if os.path.exists(drv):
+        exe_dir = os.path.join(ROOT, "tests", "fortran", "_build")
+        os.makedirs(exe_dir, exist_ok=True)
+        exe = os.path.join(exe_dir, "sweep_driver.x")
+        cmd = [fc, "-O2", "-I", outdir, drv, obj, "-L", HERE, "-lm3drt",
+               "-Wl,-rpath," + os.path.relpath(HERE, exe_dir).join(["$ORIGIN/", ""]), "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=exe_dir)
     return obj
 
 
