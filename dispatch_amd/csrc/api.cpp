@@ -245,4 +245,25 @@ int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const 
   return m3drt::normalize_rates(ctx, r, wphi);
 }
 
+int m3drt_bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_bb_rates: NULL argument");
+  if (r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: inconsistent sizes");
+  if (r->nline == 0 || r->ncell == 0) return M3DRT_OK;
+  if (!r->iblue || !r->ired || !r->lev_i || !r->lev_j || !r->Aij || !r->lower_off || !r->q || !r->wnu ||
+      !r->g || !r->b || !r->nstar || !r->inv_df || !r->vproj || !r->e_kT || !r->adamp || !r->Inu ||
+      !r->Bnu || !r->gamma_ij || !r->gamma_ji)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: missing array");
+  if ((r->flags & M3DRT_RATES_PRECOND) && !(r->flags & M3DRT_RATES_LARGEST) &&
+      (!r->Psi || !r->beta || !r->Bij || !r->pop_ratio))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: preconditioning needs Psi, beta, Bij, pop_ratio");
+  if ((r->flags & M3DRT_RATES_LARGEST) && (r->flags & M3DRT_RATES_PRECOND))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: the preconditioned 'largest' formulation is not built");
+  if ((r->flags & M3DRT_RATES_OWN_NORM) && !r->wq)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: own normalisation needs wq");
+  if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_bb_rates: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::bb_rates(ctx, r);
+}
+
 }  // extern "C"

@@ -329,3 +329,55 @@ def _normalize_rates(self, atom, nstar, gamma_ij, gamma_ji, wphi=None):
 
 Context.rate_solve = _rate_solve
 Context.normalize_rates = _normalize_rates
+
+
+# ------------------------------------------------------------------ line rates
+
+M3DRT_RATES_PRECOND, M3DRT_RATES_OWN_NORM, M3DRT_RATES_LARGEST = 1, 2, 4
+
+BB_FIELDS = ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij", "lower_off", "q", "wnu", "wq", "g", "b",
+             "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp", "Inu", "Psi", "Bnu", "beta")
+BB_DTYPES = dict(iblue=torch.int32, ired=torch.int32, lev_i=torch.int32, lev_j=torch.int32,
+                 Aij=torch.float64, Bij=torch.float64, lower_off=torch.float32, q=torch.float32,
+                 wnu=torch.float64, wq=torch.float64, g=torch.float64, b=torch.float64, nstar=torch.float64,
+                 pop_ratio=torch.float64, inv_df=torch.float32, vproj=torch.float32, e_kT=torch.float32,
+                 adamp=torch.float32, Inu=torch.float32, Psi=torch.float32, Bnu=torch.float32,
+                 beta=torch.float32)
+
+
+class BbRatesArgs(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("nline", "nf", "ncell", "nk", "flags")]
+                + [("wmu", C.c_float), ("wtot", C.c_float), ("sf", C.c_int64), ("sc", C.c_int64)]
+                + [(n, C.c_void_p) for n in BB_FIELDS + ("gamma_ij", "gamma_ji")])
+
+
+def _bb_rates(self, d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, largest=False,
+              layout="native"):
+    """bound_bound_rates of one direction on the device.  d: dict of device tensors
+    (BB_FIELDS, dtypes BB_DTYPES); spectra Inu, Psi, Bnu, beta are (nf, ncell) in the
+    native layout or (ncell, nf) in the reference layout; gamma_ij, gamma_ji
+    (nline, ncell) float32 are accumulated in place."""
+    L = lib()
+    if not getattr(L, "_bb_ok", False):
+        L.m3drt_bb_rates.argtypes = [C.c_void_p, C.POINTER(BbRatesArgs)]
+        L.m3drt_bb_rates.restype = C.c_int
+        L._bb_ok = True
+    for k, t in d.items():
+        if t is not None and (t.dtype != BB_DTYPES[k] or not t.is_cuda or not t.is_contiguous()):
+            raise M3drtError(f"bb_rates: {k} must be a contiguous {BB_DTYPES[k]} device tensor")
+    for t in (gamma_ij, gamma_ji):
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+            raise M3drtError("bb_rates: gamma_ij/gamma_ji are contiguous float32 device tensors")
+    nline, nf = d["q"].shape
+    nk, ncell = d["b"].shape
+    sf, sc = (ncell, 1) if layout == "native" else (1, nf)
+    flags = ((M3DRT_RATES_PRECOND if precond else 0) | (M3DRT_RATES_OWN_NORM if own_norm else 0)
+             | (M3DRT_RATES_LARGEST if largest else 0))
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    a = BbRatesArgs(nline, nf, ncell, nk, flags, wmu, wtot, sf, sc, *[p(d.get(k)) for k in BB_FIELDS],
+                    p(gamma_ij), p(gamma_ji))
+    self._bind_stream()
+    _check(L.m3drt_bb_rates(self._h, C.byref(a)), "m3drt_bb_rates")
+
+
+Context.bb_rates = _bb_rates

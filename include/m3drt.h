@@ -209,6 +209,37 @@ int m3drt_rate_solve_f64(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, m3dr
  * n*_i/n*_j.  Uses trans_*, nline, nrad, ncell, nstar, gamma_ij/ji of args. */
 int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, const float *wphi);
 
+/* ------------------------------------------------------------------ line rates */
+
+#define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T ('original' formulation) */
+#define M3DRT_RATES_OWN_NORM 2  /* the line lies inside the window: own profile normalisation */
+#define M3DRT_RATES_LARGEST 4   /* rate_formulation='largest' (without preconditioning) */
+
+/* One direction's line-rate integrals.  (nline, nf) line arrays; (f, cell) fields at
+ * f*sf + cell*sc; (nk, ncell) and (nline, ncell) per-cell arrays cell-fastest. */
+typedef struct {
+  int32_t nline, nf, ncell, nk, flags;
+  float wmu, wtot;                          /* direction weight, sum of the weights */
+  int64_t sf, sc;
+  const int32_t *iblue, *ired;              /* (nline) 1-based window of each line */
+  const int32_t *lev_i, *lev_j;             /* (nline) 1-based levels */
+  const double *Aij, *Bij;                  /* (nline) */
+  const float *lower_off;                   /* (nline) E_low of the line minus level E_i [eV] */
+  const float *q;                           /* (nline, nf) profile abscissa (line%q) */
+  const double *wnu, *wq;                   /* (nline, nf) rate and profile quadrature weights */
+  const double *g, *b, *nstar, *pop_ratio;  /* (nk), (nk, ncell) x2, (ncell) */
+  const float *inv_df, *vproj, *e_kT;       /* (ncell) */
+  const float *adamp;                       /* (nline, ncell) damping in Doppler units */
+  const float *Inu, *Psi, *Bnu, *beta;      /* this direction's intensity and Psi; B_nu, beta */
+  float *gamma_ij, *gamma_ji;               /* (nline, ncell) th and dev, accumulated */
+} m3drt_bb_rates_args;
+
+/* bound_bound_rates for one direction (SRC/rt_patch_mod.f90:2852-3142), FP32 with
+ * Kahan-compensated frequency sums.  A line whose window is clipped to [1, nf]; a line
+ * whose level index lies outside [1, nk] is skipped.  M3DRT_RATES_LARGEST with
+ * M3DRT_RATES_PRECOND (the tracked split contributions) returns M3DRT_EINVAL. */
+int m3drt_bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *args);
+
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */
 int m3drt_native_strides(m3drt_sweep_desc *desc);

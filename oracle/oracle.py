@@ -452,3 +452,47 @@ def update_populations(atom, e_kT, temp, totn, nstar, b, gij, gji, cup=None, **p
                                   keep.p(cup, np.float64), C.byref(res))
     return b, dict(status=st, maxerr=res.maxerr, epos=res.epos, elvl=res.elvl, nclamp=res.nclamp,
                    lvlerr=lvl)
+
+
+# ------------------------------------------------------------------ line rates
+
+class BbArgs(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("nline", "nf", "ncell", "nk", "precond", "own_norm", "largest", "kahan")]
+                + [(n, C.c_void_p) for n in ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij", "lower_off", "q",
+                                              "wnu", "wq")]
+                + [("wmu", C.c_float), ("wtot", C.c_float)]
+                + [(n, C.c_void_p) for n in ("g", "b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp",
+                                              "Inu", "Psi", "Bnu", "beta")]
+                + [("sf", C.c_size_t), ("sc", C.c_size_t)])
+
+
+RATE_FIELDS = dict(  # name -> dtype of the line-rate inputs (one direction)
+    iblue=np.int32, ired=np.int32, lev_i=np.int32, lev_j=np.int32, Aij=np.float64, Bij=np.float64,
+    lower_off=np.float32, q=np.float32, wnu=np.float64, wq=np.float64, g=np.float64, b=np.float64,
+    nstar=np.float64, pop_ratio=np.float64, inv_df=np.float32, vproj=np.float32, e_kT=np.float32,
+    adamp=np.float32, Inu=np.float32, Psi=np.float32, Bnu=np.float32, beta=np.float32)
+
+
+def bb_rates(d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, largest=False, kahan=False):
+    """bound_bound_rates for one direction (SRC/rt_patch_mod.f90:2852-3142).  d holds
+    the arrays of RATE_FIELDS; spectra (Inu, Psi, Bnu, beta) are (nf, ncell).
+    Returns the accumulated (gamma_ij, gamma_ji), each (nline, ncell) float32."""
+    L = lib()
+    if not getattr(L, "_bb_ok", False):
+        L.orc_bb_rates.argtypes = [C.POINTER(BbArgs), C.c_void_p, C.c_void_p]
+        L.orc_bb_rates.restype = None
+        L._bb_ok = True
+    keep = _Keep()
+    nline, nf = d["q"].shape
+    nk, ncell = d["b"].shape
+    a = BbArgs(nline, nf, ncell, nk, int(precond), int(own_norm), int(largest), int(kahan),
+               *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij",
+                                                           "lower_off", "q", "wnu", "wq")],
+               wmu, wtot,
+               *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("g", "b", "nstar", "pop_ratio", "inv_df", "vproj",
+                                                           "e_kT", "adamp", "Inu", "Psi", "Bnu", "beta")],
+               ncell, 1)
+    gij = np.array(gamma_ij, np.float32, copy=True)
+    gji = np.array(gamma_ji, np.float32, copy=True)
+    L.orc_bb_rates(C.byref(a), gij.ctypes.data_as(C.c_void_p), gji.ctypes.data_as(C.c_void_p))
+    return gij, gji

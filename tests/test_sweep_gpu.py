@@ -102,7 +102,15 @@ def test_sweep_eta_and_split_rates(ctx, seed, vec, maser):
     check32(g["I"], interior(I), "I")
     check32(g["Psi"], psi, "Psi")
     check32(g["Irem"], irem, "Irem")
-    check32(g["esc"], esc, "escape")
+    # escape of a flagged cell adds 0.5*Psi*(x_up - x)*rxb, a difference of two
+    # nearly equal interpolated ratios: in FP32 it carries the cancellation of that
+    # difference (the oracle's own FP32 escape is 2e-5..5e-5 off its FP64 one on
+    # these cases).  Bar: the kernel is no further from FP64 than twice the
+    # reference arithmetic in FP32, plus the usual 1e-5.
+    esc64 = run_oracle(case, split_rates=True, fp64=True)[3]
+    floor = 1e-6 * np.max(np.abs(esc64))
+    own = max_rel(esc, esc64, floor)
+    assert max_rel(g["esc"], esc64, floor) < RTOL32 + 2 * own, "escape"
 
 
 @pytest.mark.parametrize("kw", CASES[:3])
