@@ -254,16 +254,63 @@ int m3drt_bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r) {
       !r->g || !r->b || !r->nstar || !r->inv_df || !r->vproj || !r->e_kT || !r->adamp || !r->Inu ||
       !r->Bnu || !r->gamma_ij || !r->gamma_ji)
     return set_error(M3DRT_EINVAL, "m3drt_bb_rates: missing array");
-  if ((r->flags & M3DRT_RATES_PRECOND) && !(r->flags & M3DRT_RATES_LARGEST) &&
-      (!r->Psi || !r->beta || !r->Bij || !r->pop_ratio))
-    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: preconditioning needs Psi, beta, Bij, pop_ratio");
-  if ((r->flags & M3DRT_RATES_LARGEST) && (r->flags & M3DRT_RATES_PRECOND))
-    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: the preconditioned 'largest' formulation is not built");
+  const bool precond = r->flags & M3DRT_RATES_PRECOND, largest = r->flags & M3DRT_RATES_LARGEST;
+  if (precond && (!r->beta || !r->Bij || !r->pop_ratio))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: preconditioning needs beta, Bij, pop_ratio");
+  if (precond && !largest && !r->Psi)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: preconditioning needs Psi");
+  if (precond && largest &&
+      (!r->Psi || !r->Irem || !r->escape || !r->chi || !r->split_opac1 || !r->split_opac2 ||
+       !r->split_chi1 || !r->split_chi2 || !r->split_id || !r->split_chi_id))
+    return set_error(M3DRT_EINVAL,
+                     "m3drt_bb_rates: 'largest' with preconditioning needs Psi, Irem, escape, chi and the "
+                     "tracked contributions");
+  if ((r->flags & M3DRT_RATES_SCAT) && (!r->chi_scat || !r->dJnu))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: line scattering needs chi_scat and dJnu");
+  if ((r->flags & M3DRT_RATES_FLAGS) && (!r->ntrans || !r->psi_flags_in || !r->psi_flags_out || r->npw < 1))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_rates: psi flags need ntrans, psi_flags_in/out and npw");
   if ((r->flags & M3DRT_RATES_OWN_NORM) && !r->wq)
     return set_error(M3DRT_EINVAL, "m3drt_bb_rates: own normalisation needs wq");
   if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_bb_rates: strides must be >= 1");
   M3DRT_HIP(hipSetDevice(ctx->device));
   return m3drt::bb_rates(ctx, r);
+}
+
+int m3drt_bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: NULL argument");
+  if (r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: inconsistent sizes");
+  if (r->nline == 0 || r->ncell == 0) return M3DRT_OK;
+  if (!r->iblue || !r->ired || !r->lev_i || !r->lev_j || !r->Bij || !r->lower_off || !r->q || !r->nstar ||
+      !r->pop_ratio || !r->inv_df || !r->vproj || !r->e_kT || !r->adamp || !r->xhat || !r->ehat)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: missing array");
+  if (!(r->flags & M3DRT_BBOP_LTE) && !r->b)
+    return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: departures b needed outside LTE");
+  if ((r->flags & M3DRT_BBOP_SPLIT) && (!r->split_opac1 || !r->split_opac2 || !r->split_chi1 ||
+                                        !r->split_chi2 || !r->split_id || !r->split_chi_id))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: split tracking needs its six arrays");
+  if (r->wphi_line && (!r->wphi || !r->wq))
+    return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: wphi needs wphi and wq");
+  if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_bb_opacity: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::bb_opacity(ctx, r);
+}
+
+int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_bf_rates: NULL argument");
+  if (r->ncont < 0 || r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_bf_rates: inconsistent sizes");
+  if (r->ncont == 0 || r->ncell == 0) return M3DRT_OK;
+  if (!r->iblue || !r->ired || !r->lev_i || !r->lev_j || !r->ev || !r->wnu || !r->alpha || !r->twohnu3_c2 ||
+      !r->e_kT || !r->Jnu || !r->beta || !r->gamma_ij || !r->gamma_ji)
+    return set_error(M3DRT_EINVAL, "m3drt_bf_rates: missing array");
+  if (r->precond && (!r->PsiJ || !r->nstar || !r->b))
+    return set_error(M3DRT_EINVAL, "m3drt_bf_rates: preconditioning needs PsiJ, nstar and b");
+  if (!r->precond && !r->PsiJ)
+    return set_error(M3DRT_EINVAL, "m3drt_bf_rates: PsiJ is read by the sums (pass zeros without preconditioning)");
+  if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_bf_rates: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::bf_rates(ctx, r);
 }
 
 }  // extern "C"

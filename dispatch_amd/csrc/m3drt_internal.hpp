@@ -35,6 +35,8 @@ int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opac
 int rate_solve(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res);
 int normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi);
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);
+int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r);
+int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r);
 
 }  // namespace m3drt
 

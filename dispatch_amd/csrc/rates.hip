@@ -1,15 +1,20 @@
-// Line-rate integrals of one direction, bound_bound_rates
-// (SRC/rt_patch_mod.f90:2852-3142): per line and cell, the frequency sums of the
-// QLG-bot profile times the intensity, Bnu/n_ref and the Lambda* correction, split
-// into the thermal (th) and deviation (dev) integrals that line_rate_coefficients
-// turns into rates.  Formulations: 'original' (with the preconditioning
-// correction) and 'largest' without preconditioning.
+// The ALI line and continuum terms of one direction / one window, for gfx950:
 //
-// Mapping.  One lane per cell; the line loop and each line's frequency window are
-// uniform across the wavefront, so the per-frequency line data (abscissa q and
-// quadrature weights) are scalar loads and the per-(f, cell) fields are one
-// contiguous run per wavefront in the native layout.  QLG-bot's tables sit in LDS.
-// The sums are FP32 with Kahan compensation.
+//  bb_opacity_kernel  add_bound_bound_opacities (SRC/rt_patch_mod.f90:2689-2831): the
+//                     model atom's lines into xhat/ehat, add_largest tracking
+//                     (3150-3169), the direction's wphi share;
+//  bb_rates_kernel    bound_bound_rates (2852-3142): 'original' with the
+//                     preconditioning correction, 'largest' without it, and
+//                     'largest' with it (Rybicki & Hummer integrands from Irem,
+//                     escape and other_contribution, 3174-3187; psi_flags);
+//  bf_rates_kernel    bound_free_rates (3193-3259).
+//
+// Mapping.  One lane per cell; the line (continuum) loop and each window's
+// frequency loop are uniform across the wavefront, so the per-frequency line data
+// are scalar loads and the per-(f, cell) fields are one contiguous run per
+// wavefront in the native layout.  A cell's lines are visited in list order, which
+// is what add_largest's order dependence needs; no two lanes touch one point.
+// QLG-bot's tables sit in LDS.  Line-rate sums are FP32 with Kahan compensation.
 #include "m3drt_internal.hpp"
 #include <cmath>
 #include <algorithm>
@@ -22,18 +27,6 @@ namespace {
 
 constexpr int QLG_NX = 9019;
 
-struct RatesArgs {
-  int nline, nf, ncell, nk, precond, own_norm, largest;
-  const int32_t *iblue, *ired, *lev_i, *lev_j;
-  const double *Aij, *Bij, *wnu, *wq, *g, *b, *nstar, *pop_ratio;
-  const float *lower_off, *q, *inv_df, *vproj, *e_kT, *adamp;
-  const float *Inu, *Psi, *Bnu, *beta;
-  int64_t sf, sc;
-  float wmu, wtot;
-  float *gij, *gji;
-  const float *tab;
-};
-
 struct Kahan {
   float s = 0.0f, c = 0.0f;
   __device__ __forceinline__ void add(float v) {
@@ -44,14 +37,68 @@ struct Kahan {
   }
 };
 
+// QLG-bot on Doppler units for damping a (qlg_bot_sweep's setup, HOP/qlg_mod.f90:230-240)
+// and the model-atom profile of qlg_bot_profile (HOP/qlg_mod.f90:682-689)
+struct Qlg {
+  float ih, bL, gG, cZH, cL2;
+  __device__ __forceinline__ explicit Qlg(float ad) {
+    const float d = 1.0f + ad * 0.60602209212341318f;
+    ih = d / (ad * d + 0.8325546112f);
+    const float r = ad * ih, qq = r * (1.0f - r), scl = 1.7724538509f * ih, qs = qq * scl;
+    const float x = 2.0f * r - 1.0f;
+    bL = (0.3183098862f * r) * scl;
+    gG = (0.4697186f * (1.0f - r)) * scl + qs * (-0.064924393191013452f + x * (-0.11019326278794868f));
+    cZH = qs * (0.38494195930078284f + x * (-0.10567135279610407f));
+    cL2 = qs * (0.04054415746208978f + x * (0.16241029253712097f));
+  }
+  __device__ __forceinline__ float profile(const float *TGX, const float *TH2X, float qf, float vpr,
+                                           float idf) const {
+    const float v = fabsf(qf - vpr) * idf;
+    const float xs = v * ih, u = xs * xs, L = 1.0f / (1.0f + u);
+    const int ig = (int)roundf(fminf(xs * 800.0f, (float)QLG_NX));
+    float p = cL2 + cZH * TH2X[ig];
+    p = bL + L * p;
+    const float K = TGX[ig] * gG + L * p;
+    return K * 0.5641895835f * idf * (qf + 1.0f);
+  }
+};
+
+__device__ __forceinline__ void load_tables(float *tab, const float *src) {
+  for (int i = threadIdx.x; i < 2 * (QLG_NX + 1); i += blockDim.x) tab[i] = src[i];
+  __syncthreads();
+}
+
+// other_contribution, SRC/rt_patch_mod.f90:3174-3187
+__device__ __forceinline__ float other_contribution(float value, int index, float largest, float rest,
+                                                    int largest_index) {
+  return index == largest_index ? rest : largest + (rest - value);
+}
+
+// ---------------------------------------------------------------- line rates
+
+struct RatesArgs {
+  int nline, nf, ncell, nk, precond, own_norm, largest, scat, use_flags, split_offset;
+  const int32_t *iblue, *ired, *lev_i, *lev_j;
+  const double *Aij, *Bij, *wnu, *wq, *g, *b, *nstar, *pop_ratio;
+  const float *lower_off, *q, *inv_df, *vproj, *e_kT, *adamp;
+  const float *Inu, *Psi, *Bnu, *beta;
+  const float *Irem, *esc, *chi, *so1, *so2, *sc1, *sc2, *chi_scat, *dJnu;
+  const int32_t *sid, *scid, *ntrans, *flags_in;
+  int32_t *flags_out;
+  int64_t sf, sc;
+  float wmu, wtot;
+  float *gij, *gji;
+  const float *tab;
+};
+
 __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
   extern __shared__ float tab[];
-  for (int i = threadIdx.x; i < 2 * (QLG_NX + 1); i += blockDim.x) tab[i] = a.tab[i];
-  __syncthreads();
+  load_tables(tab, a.tab);
   const float *TGX = tab, *TH2X = tab + QLG_NX + 1;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncell) return;
   const int nc = a.ncell;
+  const bool split_pre = a.largest && a.precond;
   const float idf = a.inv_df[c], vpr = a.vproj[c], ekt = a.e_kT[c];
   for (int l = 0; l < a.nline; ++l) {
     // a window or a level outside the arrays skips the line (documented in the header)
@@ -59,6 +106,7 @@ __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
     const int li = a.lev_i[l] - 1, lj = a.lev_j[l] - 1;
     if (li < 0 || li >= a.nk || lj < 0 || lj >= a.nk) continue;
     const float lower_off = a.lower_off[l];
+    const int index = a.split_offset + l + 1;
     const double nsi = a.nstar[(int64_t)li * nc + c], nsj = a.nstar[(int64_t)lj * nc + c];
     float ni = (float)(a.b[(int64_t)li * nc + c] * nsi);
     float nj = (float)(a.b[(int64_t)lj * nc + c] * nsj);
@@ -67,7 +115,7 @@ __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
       const double beta_ref = nsj * a.g[li] / (nsi * a.g[lj]);
       inv_nref = (float)fmin((1.0 - beta_ref) / beta_ref, 1e35);
     }
-    float eta_coef = 0.0f, g_ratio = 0.0f;
+    float eta_coef = 0.0f, g_ratio = 0.0f, abs_coef = 0.0f;
     const float eoff = lower_off != 0 ? (float)exp((double)(-lower_off * ekt)) : 1.0f;
     if (lower_off != 0) {
       ni = ni * eoff;
@@ -79,41 +127,64 @@ __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
       float split_Rij = (float)(nsi * a.pop_ratio[c]);
       split_Rij = (float)((double)split_Rij * a.Bij[l]);
       eta_coef = split_Rij * split_bj;
+      float split_bi = (float)a.b[(int64_t)li * nc + c];
+      if (lower_off != 0) split_bi = split_bi * eoff;
+      abs_coef = split_Rij * split_bi;
       if (nj > 0.0f) g_ratio = (float)((double)(ni / nj) * (a.g[lj] / a.g[li]));
     }
-    // QLG-bot setup for damping a, on Doppler units (qlg_bot_sweep's, HOP/qlg_mod.f90:230-240)
-    const float ad = a.adamp[(int64_t)l * nc + c];
-    const float d = 1.0f + ad * 0.60602209212341318f;
-    const float ih = d / (ad * d + 0.8325546112f);
-    const float r = ad * ih, qq = r * (1.0f - r), scl = 1.7724538509f * ih, qs = qq * scl;
-    const float x = 2.0f * r - 1.0f;
-    const float bL = (0.3183098862f * r) * scl;
-    const float gG = (0.4697186f * (1.0f - r)) * scl + qs * (-0.064924393191013452f + x * (-0.11019326278794868f));
-    const float cZH = qs * (0.38494195930078284f + x * (-0.10567135279610407f));
-    const float cL2 = qs * (0.04054415746208978f + x * (0.16241029253712097f));
+    const Qlg Q(a.adamp[(int64_t)l * nc + c]);
     Kahan sA, sI, sP, sw;
     const int64_t lo = (int64_t)l * a.nf;
-    for (int f = fs; f <= fe; ++f) {
-      const float qf = a.q[lo + f];
-      // qlg_bot_profile (HOP/qlg_mod.f90:664-671): profile per unit frequency
-      const float v = fabsf(qf - vpr) * idf;
-      const float xs = v * ih, u = xs * xs, L = 1.0f / (1.0f + u);
-      const int ig = (int)roundf(fminf(xs * 800.0f, (float)QLG_NX));
-      float p = cL2 + cZH * TH2X[ig];
-      p = bL + L * p;
-      const float K = TGX[ig] * gG + L * p;
-      const float phi = K * 0.5641895835f * idf * (qf + 1.0f);
-      const float wmunu = (float)((double)a.wmu * a.wnu[lo + f] * a.Aij[l]);
-      const float vvv = phi * wmunu;
-      const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
-      const float nb = a.Bnu[o], Iv = a.Inu[o];
-      if (a.own_norm) sw.add(phi * (float)a.wq[lo + f]);
-      sA.add(vvv * (nb * inv_nref));
-      if (a.largest) {
-        sI.add(vvv * (Iv - nb));
-      } else {
-        sI.add(vvv * Iv);
-        sP.add(vvv * phi * a.Psi[o] * a.beta[o]);
+    if (split_pre) {
+      bool in_force = false, raise = false;
+      int pw = 0, pb = 0;
+      if (a.use_flags) {
+        pw = (a.ntrans[l] - 1) / 32;
+        pb = (a.ntrans[l] - 1) % 32;
+        in_force = (a.flags_in[(int64_t)pw * nc + c] >> pb) & 1;
+      }
+      for (int f = fs; f <= fe; ++f) {
+        const float vph = Q.profile(TGX, TH2X, a.q[lo + f], vpr, idf);
+        const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
+        float psi = a.Psi[o];
+        const float bt = a.beta[o];
+        const float eta_u = vph * eta_coef;
+        const float eta_t = eta_u * bt;
+        const float chi_t = vph * abs_coef - eta_t;
+        const float eta_other = bt * other_contribution(eta_u, index, a.so1[o], a.so2[o], a.sid[o]);
+        const float chi_other = other_contribution(vph * abs_coef, index, a.sc1[o], a.sc2[o], a.scid[o]) -
+                                eta_other;
+        float other = eta_other;
+        if (a.scat) other = other + a.chi_scat[o] * a.dJnu[o];
+        if (chi_t < 0.0f || in_force) psi = 0.0f;
+        raise = raise || psi == 0.0f;
+        const bool zero = psi == 0.0f;
+        const float up = zero ? a.Inu[o] : a.Irem[o] + psi * other;
+        const float spont = zero ? 1.0f : (chi_other + a.esc[o] * chi_t) / a.chi[o];
+        const float wmunu = (float)((double)a.wmu * a.wnu[lo + f] * a.Aij[l]);
+        const float vvv = vph * wmunu;
+        const float nb = a.Bnu[o];
+        sA.add(vvv * spont * (nb * inv_nref));  // sum_T
+        sI.add(vvv * (up - nb * spont));        // sum_U
+        if (a.own_norm) sw.add(vph * (float)a.wq[lo + f]);
+      }
+      if (a.use_flags && raise && !in_force) atomicOr(&a.flags_out[(int64_t)pw * nc + c], (int32_t)(1u << pb));
+    } else {
+      for (int f = fs; f <= fe; ++f) {
+        const float qf = a.q[lo + f];
+        const float phi = Q.profile(TGX, TH2X, qf, vpr, idf);
+        const float wmunu = (float)((double)a.wmu * a.wnu[lo + f] * a.Aij[l]);
+        const float vvv = phi * wmunu;
+        const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
+        const float nb = a.Bnu[o], Iv = a.Inu[o];
+        if (a.own_norm) sw.add(phi * (float)a.wq[lo + f]);
+        sA.add(vvv * (nb * inv_nref));
+        if (a.largest) {
+          sI.add(vvv * (Iv - nb));
+        } else {
+          sI.add(vvv * Iv);
+          sP.add(vvv * phi * a.Psi[o] * a.beta[o]);
+        }
       }
     }
     float g_th = sA.s, g_dev = sI.s;
@@ -134,6 +205,140 @@ __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- line opacity
+
+struct OpacArgs {
+  int nline, nf, ncell, nk, lte, split, split_offset;
+  const int32_t *iblue, *ired, *lev_i, *lev_j, *wphi_line;
+  const double *Bij, *wq, *b, *nstar, *pop_ratio;
+  const float *lower_off, *q, *inv_df, *vproj, *e_kT, *adamp;
+  float *xhat, *ehat, *so1, *so2, *sc1, *sc2, *wphi;
+  int32_t *sid, *scid;
+  int64_t sf, sc;
+  float wmu;
+  const float *tab;
+};
+
+// add_largest, SRC/rt_patch_mod.f90:3150-3169
+__device__ __forceinline__ void add_largest(float value, int index, float *largest, float *rest,
+                                            int32_t *largest_index) {
+  const float lg = *largest;
+  if (fabsf(value) > fabsf(lg)) {
+    *rest = *rest + lg;
+    *largest = value;
+    *largest_index = index;
+  } else {
+    *rest = *rest + value;
+  }
+}
+
+__global__ void __launch_bounds__(256) bb_opacity_kernel(const OpacArgs a) {
+  extern __shared__ float tab[];
+  load_tables(tab, a.tab);
+  const float *TGX = tab, *TH2X = tab + QLG_NX + 1;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  const int nc = a.ncell;
+  const float idf = a.inv_df[c], vpr = a.vproj[c], ekt = a.e_kT[c];
+  for (int l = 0; l < a.nline; ++l) {
+    const int fs = max(a.iblue[l] - 1, 0), fe = min(a.ired[l] - 1, a.nf - 1);
+    const int li = a.lev_i[l] - 1, lj = a.lev_j[l] - 1;
+    if (li < 0 || li >= a.nk || lj < 0 || lj >= a.nk) continue;
+    const float lower_off = a.lower_off[l];
+    const int index = a.split_offset + l + 1;
+    const float ni = (float)(a.nstar[(int64_t)li * nc + c] * a.pop_ratio[c]);
+    float bi = 1.0f, bj = 1.0f;
+    if (!a.lte) {
+      bi = (float)a.b[(int64_t)li * nc + c];
+      bj = (float)a.b[(int64_t)lj * nc + c];
+    }
+    if (lower_off != 0) {
+      const float eoff = (float)exp((double)(-lower_off * ekt));
+      bi = bi * eoff;
+      bj = bj * eoff;
+    }
+    const float Rij = (float)((double)ni * a.Bij[l]);
+    const float eta_fac = Rij * bj, chi_fac = Rij * bi;
+    const Qlg Q(a.adamp[(int64_t)l * nc + c]);
+    const int64_t lo = (int64_t)l * a.nf;
+    const bool sum_wphi = a.wphi_line && a.wphi_line[l];
+    double ws = 0.0;
+    for (int f = fs; f <= fe; ++f) {
+      const float vph = Q.profile(TGX, TH2X, a.q[lo + f], vpr, idf);
+      const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
+      a.xhat[o] = a.xhat[o] + vph * chi_fac;
+      a.ehat[o] = a.ehat[o] + vph * eta_fac;
+      if (a.split) {
+        add_largest(vph * eta_fac, index, &a.so1[o], &a.so2[o], &a.sid[o]);
+        add_largest(vph * chi_fac, index, &a.sc1[o], &a.sc2[o], &a.scid[o]);
+      }
+      if (sum_wphi) ws += (double)vph * a.wq[lo + f];  // sum(phi * line%wq) in real(8)
+    }
+    if (sum_wphi) {
+      const int64_t go = (int64_t)l * nc + c;
+      a.wphi[go] = a.wphi[go] + a.wmu * (float)ws;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- bound-free rates
+
+struct BfArgs {
+  int ncont, nline, nf, ncell, nk, precond;
+  double cut_x;
+  const int32_t *iblue, *ired, *lev_i, *lev_j, *active;
+  const double *ev, *wnu, *e_kT, *nstar, *b;
+  const float *alpha, *twohnu3_c2, *Jnu, *beta, *PsiJ;
+  int64_t sf, sc;
+  float *gij, *gji;
+};
+
+__global__ void __launch_bounds__(256) bf_rates_kernel(const BfArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  const int nc = a.ncell;
+  const double ekt = a.e_kT[c];
+  for (int j = 0; j < a.ncont; ++j) {
+    if (a.active && !a.active[j]) continue;
+    const int li = a.lev_i[j] - 1, lj = a.lev_j[j] - 1;
+    if (li < 0 || li >= a.nk || lj < 0 || lj >= a.nk) continue;
+    const double dev = a.ev[lj] - a.ev[li];
+    if (dev * ekt > a.cut_x) continue;  // radiative_cut, SRC/atom_mod.f90:1157-1162
+    const int fs = max(a.iblue[j] - 1, 0), fe = min(a.ired[j] - 1, a.nf - 1);
+    const int64_t lo = (int64_t)j * a.nf;
+    float sum_abs = 0.0f, sum_ems = 0.0f, sum_pre = 0.0f;
+    for (int f = fs; f <= fe; ++f) {
+      const float al = a.alpha[lo + f];
+      const float wmunu = (float)(0.5 * a.wnu[lo + f]);
+      const float vij = a.twohnu3_c2[f] * al * wmunu;
+      const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
+      const float J = a.Jnu[o];
+      const float vji = vij * a.beta[o];
+      const float phi = a.PsiJ[o] * al;
+      sum_abs = sum_abs + vij * J;
+      sum_ems = sum_ems + vji * (J + 1.0f);
+      sum_pre = sum_pre + vji * phi;
+    }
+    float g_ij = sum_abs, g_ji = sum_ems;
+    if (a.precond) {
+      const double nsi = a.nstar[(int64_t)li * nc + c];
+      g_ij = (float)((double)g_ij - (double)sum_pre * nsi * a.b[(int64_t)lj * nc + c]);
+      g_ji = (float)((double)g_ji - (double)sum_pre * nsi * a.b[(int64_t)li * nc + c]);
+    }
+    const int64_t go = (int64_t)(a.nline + j) * nc + c;
+    a.gij[go] = a.gij[go] + g_ij;
+    a.gji[go] = a.gji[go] + g_ji;
+  }
+}
+
+template <typename K, typename A>
+int launch_cells(m3drt_ctx *ctx, K kern, const A &a, int ncell, size_t lds) {
+  if (lds) M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3((ncell + 255) / 256), dim3(256), lds, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
+
 }  // namespace
 
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r) {
@@ -144,19 +349,54 @@ int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r) {
   a.precond = (r->flags & M3DRT_RATES_PRECOND) ? 1 : 0;
   a.own_norm = (r->flags & M3DRT_RATES_OWN_NORM) ? 1 : 0;
   a.largest = (r->flags & M3DRT_RATES_LARGEST) ? 1 : 0;
+  a.scat = (r->flags & M3DRT_RATES_SCAT) ? 1 : 0;
+  a.use_flags = (r->flags & M3DRT_RATES_FLAGS) ? 1 : 0;
+  a.split_offset = r->split_offset;
   a.iblue = r->iblue; a.ired = r->ired; a.lev_i = r->lev_i; a.lev_j = r->lev_j;
   a.Aij = r->Aij; a.Bij = r->Bij; a.wnu = r->wnu; a.wq = r->wq; a.g = r->g; a.b = r->b;
   a.nstar = r->nstar; a.pop_ratio = r->pop_ratio; a.lower_off = r->lower_off; a.q = r->q;
   a.inv_df = r->inv_df; a.vproj = r->vproj; a.e_kT = r->e_kT; a.adamp = r->adamp;
   a.Inu = r->Inu; a.Psi = r->Psi; a.Bnu = r->Bnu; a.beta = r->beta;
+  a.Irem = r->Irem; a.esc = r->escape; a.chi = r->chi;
+  a.so1 = r->split_opac1; a.so2 = r->split_opac2; a.sc1 = r->split_chi1; a.sc2 = r->split_chi2;
+  a.sid = r->split_id; a.scid = r->split_chi_id; a.chi_scat = r->chi_scat; a.dJnu = r->dJnu;
+  a.ntrans = r->ntrans; a.flags_in = r->psi_flags_in; a.flags_out = r->psi_flags_out;
   a.sf = r->sf; a.sc = r->sc; a.wmu = r->wmu; a.wtot = r->wtot;
   a.gij = r->gamma_ij; a.gji = r->gamma_ji;
   a.tab = ctx->qlg_tab;
-  const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
-  M3DRT_HIP(hipFuncSetAttribute((const void *)bb_rates_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(bb_rates_kernel, dim3((r->ncell + 255) / 256), dim3(256), lds, ctx->stream, a);
-  M3DRT_HIP(hipGetLastError());
-  return M3DRT_OK;
+  return launch_cells(ctx, bb_rates_kernel, a, r->ncell, 2 * (QLG_NX + 1) * sizeof(float));
+}
+
+int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r) {
+  int rc = qlg_tables(ctx);
+  if (rc) return rc;
+  OpacArgs a;
+  a.nline = r->nline; a.nf = r->nf; a.ncell = r->ncell; a.nk = r->nk;
+  a.lte = (r->flags & M3DRT_BBOP_LTE) ? 1 : 0;
+  a.split = (r->flags & M3DRT_BBOP_SPLIT) ? 1 : 0;
+  a.split_offset = r->split_offset;
+  a.iblue = r->iblue; a.ired = r->ired; a.lev_i = r->lev_i; a.lev_j = r->lev_j; a.wphi_line = r->wphi_line;
+  a.Bij = r->Bij; a.wq = r->wq; a.b = r->b; a.nstar = r->nstar; a.pop_ratio = r->pop_ratio;
+  a.lower_off = r->lower_off; a.q = r->q; a.inv_df = r->inv_df; a.vproj = r->vproj; a.e_kT = r->e_kT;
+  a.adamp = r->adamp;
+  a.xhat = r->xhat; a.ehat = r->ehat;
+  a.so1 = r->split_opac1; a.so2 = r->split_opac2; a.sc1 = r->split_chi1; a.sc2 = r->split_chi2;
+  a.sid = r->split_id; a.scid = r->split_chi_id; a.wphi = r->wphi;
+  a.sf = r->sf; a.sc = r->sc; a.wmu = r->wmu;
+  a.tab = ctx->qlg_tab;
+  return launch_cells(ctx, bb_opacity_kernel, a, r->ncell, 2 * (QLG_NX + 1) * sizeof(float));
+}
+
+int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
+  BfArgs a;
+  a.ncont = r->ncont; a.nline = r->nline; a.nf = r->nf; a.ncell = r->ncell; a.nk = r->nk;
+  a.precond = r->precond; a.cut_x = r->cut_x;
+  a.iblue = r->iblue; a.ired = r->ired; a.lev_i = r->lev_i; a.lev_j = r->lev_j; a.active = r->active;
+  a.ev = r->ev; a.wnu = r->wnu; a.e_kT = r->e_kT; a.nstar = r->nstar; a.b = r->b;
+  a.alpha = r->alpha; a.twohnu3_c2 = r->twohnu3_c2; a.Jnu = r->Jnu; a.beta = r->beta; a.PsiJ = r->PsiJ;
+  a.sf = r->sf; a.sc = r->sc;
+  a.gij = r->gamma_ij; a.gji = r->gamma_ji;
+  return launch_cells(ctx, bf_rates_kernel, a, r->ncell, 0);
 }
 
 }  // namespace m3drt

@@ -1,0 +1,455 @@
+// Straight-line form of the 3D short-characteristics sweep (short_char2,
+// SRC/rt_solvers_mod.f90:224-507, with integrate_intensity,
+// SRC/rt_patch_mod.f90:3399-3431, fused) for the case the formal solution of an
+// iteration runs: J, or J and PsiJ, as the only outputs, no eta flags, no split
+// rates.  Same mapping and arithmetic as sweep_sc2.hip's general kernel (one
+// workgroup per frequency owning the whole periodic plane, strips of L rows per
+// thread, a sliding 4-row window, PD cells of global loads in flight), with the
+// address work taken out of the cell loop:
+//
+//  * the upwind plane sits in LDS with a periodic halo of 2 cells on every side
+//    ((nx+4) x (ny+4)), so a thread's whole 4-column x (L+3)-row stencil reach is
+//    ONE base address plus row steps, with no wrap per cell;
+//  * I and x = chi/chi500 are interleaved as pairs, so a stencil row is two
+//    64-bit pair reads (ds_read2_b64) instead of six 32-bit ones;
+//  * the cubes are read at a per-thread 32-bit offset that never changes from
+//    plane to plane against a wave-uniform base per (plane, row);
+//  * dt500 and 1/chi500 of a cell come as one pair.
+// The strips are exact (ny = L * strips, L from the instantiated set), so every
+// per-cell branch is resolved at compile time.
+#include "sweep_sc2.hpp"
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+
+namespace m3drt {
+namespace {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
+};
+
+template <typename T>
+struct FastArgs {
+  const T *chi, *S;
+  const T *gchi, *gS, *gI;
+  const Pair<T> *dr;  // per cell (dt500, 1/chi500), planes 0..nz-1, then the ghost plane (0, 1/g500)
+  T *J, *PsiJ, *exitI;
+  int64_t cs_f, cs_z, gs_f;
+  uint32_t cs_x, cs_y, gs_x, gs_y;
+  int nx, ny, nz, dir, xo, yo, PW, nstrips;
+  uint32_t plane_bytes, gplane_bytes;  // byte reach of one (z, f) plane of a cube / the ghost plane
+  T xk0, xk1, xk2, xk3, yk0, yk1, yk2, yk3, hx, hy, mdtau, imax, wj;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldo(const T *base, uint32_t off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
+// Buffer access: a wave-uniform descriptor (base, size in bytes), a per-lane 32-bit
+// offset and a wave-uniform 32-bit offset, so a cell's address costs no vector
+// instruction.  Accesses beyond the size return zero / are dropped.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(Rsrc r, uint32_t v, uint32_t s) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+  else if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, T x) {
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, v, s, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(unsigned int __attribute__((ext_vector_type(2))), x), r, v, s, 0);
+}
+
+// (dt500, 1/chi500) per cell as sc2_prep in sweep_sc2.hip forms them
+// (SRC/rt_solvers_mod.f90:355-370), then (0, 1/g500) for the ghost plane.
+template <typename T>
+__global__ void __launch_bounds__(256) sc2_prep_pairs(const float *__restrict__ chi500,
+                                                      const float *__restrict__ chi500_up,
+                                                      const float *__restrict__ g500, Pair<T> *__restrict__ dr,
+                                                      int nxy, int nz, double dsup) {
+  const int64_t n = (int64_t)nxy * nz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n + nxy;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Pair<T> p;
+    if (i < n) {
+      const double c = (double)chi500[i];
+      const double cu = (double)chi500_up[i];
+      const double r = cu / c;
+      double d;
+      if (fabs(r - 1.0) < 1e-3)
+        d = dsup * c * (1.0 + (r - 1.0) * (0.5 - (r - 1.0) / 12.0));
+      else
+        d = dsup * c * (r - 1.0) / log(r);
+      p.a = (T)d;
+      p.b = (T)(1.0 / c);
+    } else {
+      p.a = (T)0;
+      p.b = (T)(1.0 / (double)g500[i - n]);
+    }
+    dr[i] = p;
+  }
+}
+
+// MODE 2: (I, x) pairs and S in LDS; 1: (I, x) pairs in LDS, upwind S from the
+// cache; 0: I in LDS, upwind x and S from the cache.
+template <typename T, int L, bool PJ, int MODE>
+__global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const FastArgs<T> a) {
+  using namespace dev;
+  constexpr int PD = sizeof(T) == 8 ? 2 : 3;
+  constexpr uint32_t TB = sizeof(T);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nx = a.nx, ny = a.ny, PW = a.PW, nxy = nx * ny;
+  const int np = PW * (ny + 4);
+  Pair<T> *sP = reinterpret_cast<Pair<T> *>(smem_raw);
+  T *sI = reinterpret_cast<T *>(smem_raw);
+  T *sS = reinterpret_cast<T *>(smem_raw + (size_t)np * sizeof(Pair<T>));
+
+  const int f = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int x = tid % nx;
+  const int strip = tid / nx;
+  const int yb = strip * L;
+  const int rb = (yb + a.yo + 2) * PW + (x + a.xo + 2);  // row 0 of the window, padded index
+  const int xc1 = wrap(x + a.xo + 1, nx), xc2 = wrap(x + a.xo + 2, nx);
+  const int g0 = wrap(yb + a.yo, ny);                     // row 0 of the window, unpadded
+  const uint32_t voff = (x * a.cs_x + yb * a.cs_y) * TB;  // first cell of the strip in a cube plane
+  const uint32_t doff = (yb * nx + x) * (uint32_t)sizeof(Pair<T>);
+  const T ohx = (T)1 - a.hx, ohy = (T)1 - a.hy;
+
+  const T *__restrict__ gchi = a.gchi + f * a.gs_f;
+  const T *__restrict__ gS = a.gS + f * a.gs_f;
+  const T *__restrict__ gI = a.gI + f * a.gs_f;
+  const Pair<T> *__restrict__ gdr = a.dr + (int64_t)a.nz * nxy;
+
+  struct In {
+    T chi, S, J, PsJ;
+    Pair<T> dr;
+  };
+  // descriptors of one plane of every cube
+  struct Planes {
+    Rsrc chi, S, J, PsJ, dr;
+  };
+  auto plane_of = [&](int step) { return a.dir > 0 ? step : a.nz - 1 - step; };
+  auto planes = [&](int z) -> Planes {
+    Planes P;
+    const int64_t o = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f;
+    P.chi = rsrc(a.chi + o, a.plane_bytes);
+    P.S = rsrc(a.S + o, a.plane_bytes);
+    P.J = rsrc(a.J + o, a.plane_bytes);
+    P.PsJ = rsrc(PJ ? a.PsiJ + o : a.J + o, a.plane_bytes);
+    P.dr = rsrc(a.dr + (int64_t)z * nxy, (uint32_t)(nxy * sizeof(Pair<T>)));
+    return P;
+  };
+  const uint32_t rowb = a.cs_y * TB, drowb = nx * (uint32_t)sizeof(Pair<T>);
+  auto fetch = [&](const Planes &P, int k) -> In {
+    In v;
+    v.chi = bld<T>(P.chi, voff, k * rowb);
+    v.S = bld<T>(P.S, voff, k * rowb);
+    v.J = bld<T>(P.J, voff, k * rowb);
+    v.PsJ = PJ ? bld<T>(P.PsJ, voff, k * rowb) : (T)0;
+    v.dr = bld<Pair<T>>(P.dr, doff, k * drowb);
+    return v;
+  };
+
+  In pre[PD], ring[PD];
+  {
+    const Planes P0 = planes(plane_of(0));
+#pragma unroll
+    for (int k = 0; k < PD; ++k) pre[k] = fetch(P0, k);
+  }
+  // the upstream plane, halo included
+  for (int c = tid; c < np; c += blockDim.x) {
+    const int pr = c / PW, pc = c - pr * PW;
+    const int cy = wrap(pr - 2, ny), cx = wrap(pc - 2, nx);
+    const uint32_t go = (cx * a.gs_x + cy * a.gs_y) * TB;
+    const T iv = ldo(gI, go);
+    if (MODE >= 1) {
+      Pair<T> p;
+      p.a = iv;
+      p.b = ldo(gchi, go) * gdr[cy * nx + cx].b;
+      sP[c] = p;
+    } else {
+      sI[c] = iv;
+    }
+    if (MODE == 2) sS[c] = ldo(gS, go);
+  }
+  lds_barrier();
+
+  for (int step = 0; step < a.nz; ++step) {
+    const int z = plane_of(step);
+    const Planes P = planes(z);
+    int e0 = rb;  // opaque per plane: cheaper to re-derive the row addresses than to hold them
+    asm volatile("" : "+v"(e0));
+    // the upwind plane in global memory, for the values not held in LDS
+    const bool up_ghost = step == 0;
+    const int zu = z - a.dir;
+    const uint32_t ub = up_ghost ? a.gplane_bytes : a.plane_bytes;
+    const Rsrc up_chi = rsrc(up_ghost ? gchi : a.chi + (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f, ub);
+    const Rsrc up_S = rsrc(up_ghost ? gS : a.S + (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f, ub);
+    const Rsrc up_dr = rsrc(up_ghost ? gdr : a.dr + (int64_t)zu * nxy, (uint32_t)(nxy * sizeof(Pair<T>)));
+    const uint32_t uy = up_ghost ? a.gs_y : a.cs_y;
+    const uint32_t ux = up_ghost ? a.gs_x : a.cs_x;
+    const uint32_t ux1 = xc1 * ux * TB, ux2 = xc2 * ux * TB;
+
+    struct RawRow { T i0, i1, i2, i3, x1, x2, s1, s2; };
+    auto read_row = [&](int r) -> RawRow {
+      RawRow q;
+      const int e = e0 + r * PW;
+      if (MODE >= 1) {
+        const Pair<T> p0 = sP[e], p1 = sP[e + 1], p2 = sP[e + 2], p3 = sP[e + 3];
+        q.i0 = p0.a; q.i1 = p1.a; q.i2 = p2.a; q.i3 = p3.a;
+        q.x1 = p1.b; q.x2 = p2.b;
+      } else {
+        q.i0 = sI[e]; q.i1 = sI[e + 1]; q.i2 = sI[e + 2]; q.i3 = sI[e + 3];
+      }
+      if (MODE == 2) {
+        q.s1 = sS[e + 1];
+        q.s2 = sS[e + 2];
+      } else {
+        int g = g0 + r;
+        g = g >= ny ? g - ny : g;
+        const uint32_t ro = g * uy * TB;
+        q.s1 = bld<T>(up_S, ro + ux1, 0);
+        q.s2 = bld<T>(up_S, ro + ux2, 0);
+        if (MODE == 0) {
+          const uint32_t dro = g * nx * (uint32_t)sizeof(Pair<T>);
+          q.x1 = bld<T>(up_chi, ro + ux1, 0) * bld<Pair<T>>(up_dr, dro + xc1 * (uint32_t)sizeof(Pair<T>), 0).b;
+          q.x2 = bld<T>(up_chi, ro + ux2, 0) * bld<Pair<T>>(up_dr, dro + xc2 * (uint32_t)sizeof(Pair<T>), 0).b;
+        }
+      }
+      return q;
+    };
+    struct Row { T rI, rmin, rX, rS; };
+    auto reduce_row = [&](const RawRow &q) -> Row {
+      Row w;
+      w.rI = a.xk0 * q.i0 + a.xk1 * q.i1 + a.xk2 * q.i2 + a.xk3 * q.i3;
+      w.rmin = fmin(q.i1, q.i2);
+      w.rX = ohx * q.x1 + a.hx * q.x2;
+      w.rS = ohx * q.s1 + a.hx * q.s2;
+      return w;
+    };
+
+    T newI[L], newX[MODE >= 1 ? L : 1], newS[MODE == 2 ? L : 1];
+    Row w0 = reduce_row(read_row(0));
+    Row w1 = reduce_row(read_row(1));
+    Row w2 = reduce_row(read_row(2));
+    RawRow next = read_row(3);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      __builtin_amdgcn_sched_barrier(0);
+      const In in = k < PD ? pre[k] : ring[k % PD];
+      if (k + PD < L) ring[k % PD] = fetch(P, k + PD);
+      const Row w3 = reduce_row(next);
+      if (k + 1 < L) next = read_row(k + 4);
+      // upwind values, SRC/rt_solvers_mod.f90:379-394
+      const T xxup = ohy * w1.rX + a.hy * w2.rX;
+      const T ssup = ohy * w1.rS + a.hy * w2.rS;
+      T Inuup = a.yk0 * w0.rI + a.yk1 * w1.rI + a.yk2 * w2.rI + a.yk3 * w3.rI;
+      Inuup = fmax(Inuup, fmin(w1.rmin, w2.rmin));
+      const T chiv = in.chi, ss = in.S, dt500 = in.dr.a, rc = in.dr.b;
+      const T xx = chiv * rc;
+      T dt = (T)0.5 * (xxup + xx) * dt500;
+      dt = fmax(dt, a.mdtau);
+      T E, W, Ps, Es;
+      lin_weights(dt, E, W, Ps, Es);
+      // the S step of a cell without eta flags (SRC/rt_solvers_mod.f90:419-440, 465-475)
+      const T Ir = Inuup * E + W * ssup;
+      T Inu = Ir + Ps * ss;
+      Inu = fmax(-a.imax, fmin(a.imax, Inu));
+      newI[k] = Inu;
+      if (MODE >= 1) newX[k] = xx;
+      if (MODE == 2) newS[k] = ss;
+      bst<T>(P.J, voff, k * rowb, in.J + a.wj * Inu);
+      if (PJ) {
+        const T sgnchi = copysign(fmax(fabs(chiv), (T)1.17549435e-38f), chiv);
+        const bool off = fmin(dt, chiv) <= (T)0;
+        const T psiout = off ? (T)0 : Ps * rcp(sgnchi);
+        bst<T>(P.PsJ, voff, k * rowb, in.PsJ + a.wj * psiout);
+      }
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (step + 1 < a.nz) {
+      const Planes Pn = planes(plane_of(step + 1));
+#pragma unroll
+      for (int k = 0; k < PD; ++k) pre[k] = fetch(Pn, k);
+    }
+    lds_barrier();  // every read of the upwind plane is done
+    // the new plane, then its halo copies: columns nx-2, nx-1 to the left pad,
+    // 0, 1 to the right; rows ny-2, ny-1 to the top pad, 0, 1 to the bottom.  The
+    // base index is opaque per plane so that the 2L+4 write addresses are formed
+    // here, not held in registers across the whole sweep.
+    int wb = (yb + 2) * PW + x + 2;
+    asm volatile("" : "+v"(wb));
+    const bool xedge = x < 2 || x >= nx - 2;
+    const int dh = x < 2 ? nx : -nx;  // to this column's halo copy
+    auto put = [&](int e, int k) {
+      if (MODE >= 1) {
+        Pair<T> p;
+        p.a = newI[k];
+        p.b = newX[k];
+        sP[e] = p;
+      } else {
+        sI[e] = newI[k];
+      }
+      if (MODE == 2) sS[e] = newS[k];
+    };
+#pragma unroll
+    for (int k = 0; k < L; ++k) put(wb + k * PW, k);
+    if (xedge) {
+#pragma unroll
+      for (int k = 0; k < L; ++k) put(wb + k * PW + dh, k);
+    }
+    if (strip == 0) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        put(wb + (k + ny) * PW, k);
+        if (xedge) put(wb + (k + ny) * PW + dh, k);
+      }
+    }
+    if (strip == a.nstrips - 1) {
+#pragma unroll
+      for (int k = L - 2; k < L; ++k) {
+        put(wb + (k - ny) * PW, k);
+        if (xedge) put(wb + (k - ny) * PW + dh, k);
+      }
+    }
+    if (a.exitI && step == a.nz - 1) {
+      const Rsrc ex = rsrc(a.exitI + f * a.gs_f, a.gplane_bytes);
+      uint32_t eo = (x * a.gs_x + yb * a.gs_y) * TB;
+      asm volatile("" : "+v"(eo));
+#pragma unroll
+      for (int k = 0; k < L; ++k) bst<T>(ex, eo, k * a.gs_y * TB, newI[k]);
+    }
+    lds_barrier();
+  }
+}
+
+template <typename T, int L, bool PJ, int MODE>
+int launch_fast(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, size_t lds) {
+  auto kern = sc2_fast<T, L, PJ, MODE>;
+  M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(nf), dim3(nt), lds, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
+
+template <typename T, int L, bool PJ>
+int fast_mode(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, int mode, size_t np) {
+  if (mode == 2) return launch_fast<T, L, PJ, 2>(ctx, a, nf, nt, np * 3 * sizeof(T));
+  if (mode == 1) return launch_fast<T, L, PJ, 1>(ctx, a, nf, nt, np * 2 * sizeof(T));
+  return -1;
+}
+
+template <typename T, int L>
+int fast_pj(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, int mode, size_t np) {
+  if (a.PsiJ) return fast_mode<T, L, true>(ctx, a, nf, nt, mode, np);
+  return fast_mode<T, L, false>(ctx, a, nf, nt, mode, np);
+}
+
+// strip lengths instantiated (60, 96, 120-cell planes)
+constexpr int kL32[] = {4, 12, 15};
+
+int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int mode, size_t np) {
+  if (L == 4) return fast_pj<float, 4>(ctx, a, nf, nt, mode, np);
+  if (L == 12) return fast_pj<float, 12>(ctx, a, nf, nt, mode, np);
+  if (L == 15) return fast_pj<float, 15>(ctx, a, nf, nt, mode, np);
+  return -1;
+}
+
+template <typename T>
+int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  static_assert(sizeof(T) == 4, "the straight-line sweep is instantiated for FP32");
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const bool get_rates = (d->flags & M3DRT_GET_RATES) != 0;
+  if (b->eta_row || b->I_out || (d->flags & M3DRT_SPLIT_RATES) || !b->J || (get_rates && b->Psi_out))
+    return -1;
+  if (nx < 4 || ny < 4 || getenv("M3DRT_NO_FAST")) return -1;
+  const int tmax = sizeof(T) == 8 ? 512 : 1024;
+  if (nx > tmax) return -1;
+  const int smax = tmax / nx;  // strips that fit the workgroup
+  int L = 0;
+  for (int c : kL32)
+    if (!L && ny % c == 0 && ny / c <= smax && ny / c >= 2) L = c;
+  if (!L) return -1;
+  const int nstrips = ny / L;
+  const int nt = nx * nstrips;
+  const size_t np = (size_t)(nx + 4) * (ny + 4);
+  int mode = -1;
+  if (np * 3 * sizeof(T) <= (size_t)ctx->lds_max) mode = 2;
+  else if (np * 2 * sizeof(T) <= (size_t)ctx->lds_max) mode = 1;
+  else if (sizeof(T) == 8 && np * sizeof(T) <= (size_t)ctx->lds_max) mode = 0;
+  if (mode < 0) return -1;
+  const uint64_t span = (uint64_t)(nx - 1) * d->cs_x + (uint64_t)(ny - 1) * d->cs_y;
+  const uint64_t gspan = (uint64_t)(nx - 1) * d->gs_x + (uint64_t)(ny - 1) * d->gs_y;
+  if ((span + 1) * sizeof(T) >= (1ull << 32) || (gspan + 1) * sizeof(T) >= (1ull << 32)) return -1;
+
+  const Sc2Geom g = sc2_geometry(d);
+  const int64_t nxy = (int64_t)nx * ny;
+  const size_t npairs = (size_t)nxy * (nz + 1);
+  int rc = ensure_scratch(ctx, sizeof(Pair<T>) * npairs);
+  if (rc) return rc;
+  Pair<T> *dr = static_cast<Pair<T> *>(ctx->scratch);
+  {
+    const int blocks = (int)std::min<int64_t>(((int64_t)npairs + 255) / 256, 65536);
+    hipLaunchKernelGGL(sc2_prep_pairs<T>, dim3(blocks), dim3(256), 0, ctx->stream, b->chi500, b->chi500_up,
+                       b->g500, dr, (int)nxy, nz, g.dsup);
+    M3DRT_HIP(hipGetLastError());
+  }
+  FastArgs<T> a;
+  a.chi = static_cast<const T *>(b->chi);
+  a.S = static_cast<const T *>(b->S);
+  a.gchi = static_cast<const T *>(b->ghost_chi);
+  a.gS = static_cast<const T *>(b->ghost_S);
+  a.gI = static_cast<const T *>(b->ghost_I);
+  a.dr = dr;
+  a.J = static_cast<T *>(b->J);
+  a.PsiJ = get_rates ? static_cast<T *>(b->PsiJ) : nullptr;
+  a.exitI = static_cast<T *>(b->exit_I);
+  a.cs_f = d->cs_f; a.cs_z = d->cs_z; a.gs_f = d->gs_f;
+  a.cs_x = (uint32_t)d->cs_x; a.cs_y = (uint32_t)d->cs_y;
+  a.gs_x = (uint32_t)d->gs_x; a.gs_y = (uint32_t)d->gs_y;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.dir = d->dir > 0 ? 1 : -1;
+  a.xo = g.xo; a.yo = g.yo; a.PW = nx + 4; a.nstrips = nstrips;
+  a.plane_bytes = (uint32_t)((span + 1) * sizeof(T));
+  a.gplane_bytes = (uint32_t)((gspan + 1) * sizeof(T));
+  // the stencil in FP32 as the reference casts it (FP64 sweep: in FP64)
+  const bool f64 = sizeof(T) == 8;
+  auto cast = [&](double v) { return f64 ? (T)v : (T)(float)v; };
+  a.xk0 = cast(g.kx[0]); a.xk1 = cast(g.kx[1]); a.xk2 = cast(g.kx[2]); a.xk3 = cast(g.kx[3]);
+  a.yk0 = cast(g.ky[0]); a.yk1 = cast(g.ky[1]); a.yk2 = cast(g.ky[2]); a.yk3 = cast(g.ky[3]);
+  // inner_fraction (SRC/rt_solvers_mod.f90:500-506): the first moment of the kernel
+  a.hx = -a.xk0 + a.xk2 + (T)2 * a.xk3;
+  a.hy = -a.yk0 + a.yk2 + (T)2 * a.yk3;
+  a.mdtau = f64 ? -(T)d->maser_dtau_max : (T)(-(float)d->maser_dtau_max);
+  a.imax = f64 ? (T)d->maser_Inu_max : (T)(float)d->maser_Inu_max;
+  a.wj = (T)d->wj;
+  if (getenv("M3DRT_VERBOSE"))
+    fprintf(stderr, "[m3drt] sc2 fast %s nf=%d plane=%dx%d nz=%d: %d threads, strip %d, LDS mode %d (%zu B)%s\n",
+            f64 ? "f64" : "f32", d->nf, nx, ny, nz, nt, L, mode,
+            np * (mode == 2 ? 3 : mode == 1 ? 2 : 1) * sizeof(T), a.PsiJ ? ", PsiJ" : "");
+  return fast_dispatch(ctx, a, d->nf, nt, L, mode, np);
+}
+
+}  // namespace
+
+// FP32 only: the FP64 sweep holds twice the bytes per register and takes the
+// general kernel's exact-strip form.
+int sweep_sc2_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  if (d->prec == M3DRT_F64) return -1;
+  return run_fast<float>(ctx, d, b);
+}
+
+}  // namespace m3drt

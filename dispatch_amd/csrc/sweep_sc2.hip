@@ -21,6 +21,7 @@
 // fit the LDS next to I and x, is re-read through L1/L2 where this workgroup read
 // it one plane earlier.
 #include "m3drt_internal.hpp"
+#include "sweep_sc2.hpp"
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -108,7 +109,14 @@ __device__ __forceinline__ void stb(T *base, uint32_t off, T v) {
 // plane, so no wavefront waits on HBM when a plane opens.
 // HOLD: keep x and S of the new plane in registers until they go to LDS (FP32);
 // otherwise (FP64) re-read them through the cache after the barrier.
-template <typename T, int LMAX, bool X_LDS, bool S_LDS, bool ETA, bool HOLD>
+// EXACT: every strip holds exactly LMAX rows (ny = LMAX * strips), so the cell loop
+// is straight-line code: no lane-dependent branch joins the registers of the load
+// pipeline, and the loads stay in flight instead of being waited on at every cell.
+// OUT: the outputs, fixed at compile time (OUT_J, OUT_J | OUT_PJ), or OUT_ANY for
+// the general form that tests every output pointer at run time.
+constexpr int OUT_ANY = 0, OUT_J = 1, OUT_PJ = 2;
+
+template <typename T, int LMAX, bool EXACT, int OUT, bool X_LDS, bool S_LDS, bool ETA, bool HOLD>
 __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const Sc2Args<T> a) {
   using namespace dev;
   constexpr int PD = sizeof(T) == 8 ? 1 : 2;
@@ -124,7 +132,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
   const int x = tid % nx;
   const int strip = tid / nx;
   const int ybeg = strip * a.L;
-  const int ylen = max(0, min(a.L, ny - ybeg));
+  const int ylen = EXACT ? LMAX : max(0, min(a.L, ny - ybeg));
 
   // stencil columns: the reference's xup_t(1:4, x) (SRC/rt_solvers_mod.f90:316-319)
   const int xc0 = wrap(x + a.xo, nx), xc1 = wrap(x + a.xo + 1, nx);
@@ -134,7 +142,9 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
   const uint32_t xo1 = xc1 * a.cs_x * TB, xo2 = xc2 * a.cs_x * TB;
   const uint32_t gxo1 = xc1 * a.gs_x * TB, gxo2 = xc2 * a.gs_x * TB;
   const T ohx = (T)1 - a.hx, ohy = (T)1 - a.hy;
-  const bool want_J = a.J != nullptr, want_PJ = a.PsiJ != nullptr;
+  const bool want_J = OUT == OUT_ANY ? a.J != nullptr : (OUT & OUT_J) != 0;
+  const bool want_PJ = OUT == OUT_ANY ? a.PsiJ != nullptr : (OUT & OUT_PJ) != 0;
+  const bool split = OUT == OUT_ANY && a.split;
 
   const T *__restrict__ gchi = a.gchi + f * a.gs_f;
   const T *__restrict__ gS = a.gS + f * a.gs_f;
@@ -152,8 +162,8 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
     P.rc = a.rc + (int64_t)z * nxy;
     P.J = a.J ? a.J + zoff : nullptr;
     P.PJ = a.PsiJ ? a.PsiJ + zoff : nullptr;
-    P.Iout = a.Iout ? a.Iout + zoff : nullptr;
-    P.Psiout = a.Psiout ? a.Psiout + zoff : nullptr;
+    P.Iout = OUT == OUT_ANY && a.Iout ? a.Iout + zoff : nullptr;
+    P.Psiout = OUT == OUT_ANY && a.Psiout ? a.Psiout + zoff : nullptr;
     P.Irem = a.Irem ? a.Irem + zoff : nullptr;
     P.esc = a.esc ? a.esc + zoff : nullptr;
     return P;
@@ -177,7 +187,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
     const Plane<T> P0 = ptrs(0);
 #pragma unroll
     for (int k = 0; k < PD; ++k)
-      if (k < ylen) pre[k] = fetch(P0, ybeg + k);
+      if (EXACT || k < ylen) pre[k] = fetch(P0, ybeg + k);
   }
   for (int c = tid; c < nxy; c += blockDim.x) {
     const int cx = c % nx, cy = c / nx;
@@ -243,14 +253,14 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
     RawRow next = read_row(wrap_near(yb + a.yo + 3, ny));
 #pragma unroll
     for (int k = 0; k < LMAX; ++k) {
-      if (k < ylen) {
+      if (EXACT || k < ylen) {
         __builtin_amdgcn_sched_barrier(0);
         const int y = yb + k;
         const CellIn<T> in = k < PD ? pre[k] : ring[k % PD];
         // keep PD cells of global loads in flight, and one upwind row of LDS reads
-        if (k + PD < ylen) ring[k % PD] = fetch(P, y + PD);
+        if (EXACT ? k + PD < LMAX : k + PD < ylen) ring[k % PD] = fetch(P, y + PD);
         const Row<T> w3 = reduce_row(next);
-        if (k + 1 < ylen) next = read_row(wrap_near(y + a.yo + 4, ny));
+        if (EXACT ? k + 1 < LMAX : k + 1 < ylen) next = read_row(wrap_near(y + a.yo + 4, ny));
         // upwind values, SRC/rt_solvers_mod.f90:379-394
         const T xxup = ohy * w1.rX + a.hy * w2.rX;
         const T ssup = ohy * w1.rS + a.hy * w2.rS;
@@ -271,7 +281,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
           Ir = Inuup * E + W * ssup;
           Inu = Ir + Ps * ss;
           off = fmin(dt, chiv) <= (T)0;
-          if (a.split) off = off || (a.imax - fabs(Inu)) <= (T)0;
+          if (split) off = off || (a.imax - fabs(Inu)) <= (T)0;
           psiout = off ? (T)0 : Ps * rcp(sgnchi);
         } else {
           // flagged pairs take the eta step (SRC/rt_solvers_mod.f90:441-463, 476-490)
@@ -283,7 +293,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
           Ir = special ? Inuup * E + W * eup * rxb : Inuup * E + W * ssup;
           Inu = special ? Ir + Ps * ee * rxb : Ir + Ps * ss;
           off = fmin(fmin(dt, chiv), special ? xxup : (T)1) <= (T)0;
-          if (a.split) {
+          if (split) {
             off = off || (a.imax - fabs(Inu)) <= (T)0;
             Es = special ? Es + (T)0.5 * Ps * (xxup - xx) * rxb : Es;
           }
@@ -299,11 +309,10 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
         if (want_PJ) stb(P.PJ, o, in.PJ + a.wj * psiout);
         if (P.Iout) stb(P.Iout, o, Inu);
         if (P.Psiout) stb(P.Psiout, o, psiout);
-        if (a.split) {
+        if (split) {
           stb(P.Irem, o, off ? Inu : Ir);
           stb(P.esc, o, off ? (T)1 : Es);
         }
-        if (a.exitI && step == a.nz - 1) a.exitI[f * a.gs_f + x * a.gs_x + y * a.gs_y] = Inu;
         w0 = w1;
         w1 = w2;
         w2 = w3;
@@ -315,15 +324,17 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
       const Plane<T> Pn = ptrs(step + 1);
 #pragma unroll
       for (int k = 0; k < PD; ++k)
-        if (k < ylen) pre[k] = fetch(Pn, yb + k);
+        if (EXACT || k < ylen) pre[k] = fetch(Pn, yb + k);
     }
     lds_barrier();  // every read of the upwind plane is done
+    const bool exit_plane = a.exitI && step == a.nz - 1;
 #pragma unroll
     for (int k = 0; k < LMAX; ++k) {
-      if (k < ylen) {
+      if (EXACT || k < ylen) {
         const int y = yb + k;
         const int c = y * nx + x;
         sI[c] = newI[k];
+        if (exit_plane) a.exitI[f * a.gs_f + x * a.gs_x + y * a.gs_y] = newI[k];
         if (HOLD) {
           if (X_LDS) sX[c] = newX[k];
           if (S_LDS) sS[c] = newS[k];
@@ -338,9 +349,9 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
   }
 }
 
-template <typename T, int LMAX, bool X_LDS, bool S_LDS, bool ETA>
+template <typename T, int LMAX, bool EXACT, int OUT, bool X_LDS, bool S_LDS, bool ETA>
 static int launch_sc2(m3drt_ctx *ctx, const Sc2Args<T> &a, int nthreads, size_t lds) {
-  auto kern = sc2_sweep<T, LMAX, X_LDS, S_LDS, ETA, (sizeof(T) == 4)>;
+  auto kern = sc2_sweep<T, LMAX, EXACT, OUT, X_LDS, S_LDS, ETA, (sizeof(T) == 4)>;
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(a.nf), dim3(nthreads), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
@@ -349,8 +360,8 @@ static int launch_sc2(m3drt_ctx *ctx, const Sc2Args<T> &a, int nthreads, size_t 
 
 template <typename T, int LMAX, bool X_LDS, bool S_LDS>
 static int dispatch_eta(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, size_t lds) {
-  if (a.eta_row) return launch_sc2<T, LMAX, X_LDS, S_LDS, true>(ctx, a, nt, lds);
-  return launch_sc2<T, LMAX, X_LDS, S_LDS, false>(ctx, a, nt, lds);
+  if (a.eta_row) return launch_sc2<T, LMAX, false, OUT_ANY, X_LDS, S_LDS, true>(ctx, a, nt, lds);
+  return launch_sc2<T, LMAX, false, OUT_ANY, X_LDS, S_LDS, false>(ctx, a, nt, lds);
 }
 
 template <typename T, int LMAX>
@@ -359,6 +370,38 @@ static int dispatch_lds(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes
   if (nplanes >= 3) return dispatch_eta<T, LMAX, true, true>(ctx, a, nt, 3 * plane);
   if (nplanes == 2) return dispatch_eta<T, LMAX, true, false>(ctx, a, nt, 2 * plane);
   return dispatch_eta<T, LMAX, false, false>(ctx, a, nt, plane);
+}
+
+// The straight-line form: exact strips of L rows, J (and PsiJ) as the only outputs,
+// no eta flags.  L comes from the set instantiated for the plane sizes in use
+// (60, 96, 120 cells square: FP32 4, 12, 15; FP64 12, 24, 30).
+template <typename T, int L, int OUT>
+static int exact_lds(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
+  const size_t plane = (size_t)a.nx * a.ny * sizeof(T);
+  if (nplanes >= 3) return launch_sc2<T, L, true, OUT, true, true, false>(ctx, a, nt, 3 * plane);
+  if (nplanes == 2) return launch_sc2<T, L, true, OUT, true, false, false>(ctx, a, nt, 2 * plane);
+  return launch_sc2<T, L, true, OUT, false, false, false>(ctx, a, nt, plane);
+}
+
+template <typename T, int L>
+static int exact_out(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
+  if (a.PsiJ) return exact_lds<T, L, OUT_J | OUT_PJ>(ctx, a, nt, nplanes);
+  return exact_lds<T, L, OUT_J>(ctx, a, nt, nplanes);
+}
+
+static const int kExactL32[] = {4, 12, 15};
+static const int kExactL64[] = {12, 24, 30};
+
+template <typename T>
+static int exact_dispatch(m3drt_ctx *ctx, const Sc2Args<T> &a, int L, int nt, int nplanes) {
+  if (sizeof(T) == 4) {
+    if (L == 4) return exact_out<T, 4>(ctx, a, nt, nplanes);
+    if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes);
+    return exact_out<T, 15>(ctx, a, nt, nplanes);
+  }
+  if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes);
+  if (L == 24) return exact_out<T, 24>(ctx, a, nt, nplanes);
+  return exact_out<T, 30>(ctx, a, nt, nplanes);
 }
 
 template <typename T>
@@ -374,8 +417,18 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
   const int tmax = sizeof(T) == 8 ? 512 : 1024;
   if (nx > tmax) return set_error(M3DRT_ELIMIT, "m3drt_sc_sweep: nx beyond the workgroup size");
   const int smax = std::max(1, tmax / nx);
-  const int L = (ny + smax - 1) / smax;
+  int L = (ny + smax - 1) / smax;
   if (L > 32) return set_error(M3DRT_ELIMIT, "m3drt_sc_sweep: ny too large for the strip mapping");
+  // the straight-line form when only J (and PsiJ) are asked for and an instantiated
+  // strip length divides ny
+  int Lexact = 0;
+  const bool plain_out = !b->eta_row && !b->I_out && !(d->flags & M3DRT_SPLIT_RATES) && b->J &&
+                         !((d->flags & M3DRT_GET_RATES) && b->Psi_out);
+  if (plain_out && !getenv("M3DRT_NO_EXACT")) {
+    for (int c : (sizeof(T) == 4 ? kExactL32 : kExactL64))
+      if (c >= L && ny % c == 0 && !Lexact) Lexact = c;
+  }
+  if (Lexact) L = Lexact;
   const int nstrips = (ny + L - 1) / L;
   const int nt = nx * nstrips;
 
@@ -456,9 +509,10 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
     a.imax = (T)(float)d->maser_Inu_max;
   }
   if (getenv("M3DRT_VERBOSE"))
-    fprintf(stderr, "[m3drt] sc2 %s nf=%d plane=%dx%d nz=%d: %d threads, strip %d, %d LDS planes of %zu B (lds_max %d)%s\n",
-            sizeof(T) == 8 ? "f64" : "f32", d->nf, nx, ny, nz, nt, L, nplanes, plane, ctx->lds_max,
-            b->eta_row ? ", eta" : "");
+    fprintf(stderr, "[m3drt] sc2 %s nf=%d plane=%dx%d nz=%d: %d threads, strip %d%s, %d LDS planes of %zu B (lds_max %d)%s\n",
+            sizeof(T) == 8 ? "f64" : "f32", d->nf, nx, ny, nz, nt, L, Lexact ? " exact" : "", nplanes, plane,
+            ctx->lds_max, b->eta_row ? ", eta" : "");
+  if (Lexact) return exact_dispatch<T>(ctx, a, L, nt, nplanes);
   if (L <= 4) return dispatch_lds<T, 4>(ctx, a, nt, nplanes);
   if (L <= 8) return dispatch_lds<T, 8>(ctx, a, nt, nplanes);
   if (L <= 16) return dispatch_lds<T, 16>(ctx, a, nt, nplanes);
@@ -466,6 +520,8 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
 }
 
 int sweep_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  const int fast = sweep_sc2_fast(ctx, d, b);
+  if (fast >= 0) return fast;
   if (d->prec == M3DRT_F64) return run_sc2<double>(ctx, d, b);
   return run_sc2<float>(ctx, d, b);
 }

@@ -334,50 +334,154 @@ Context.normalize_rates = _normalize_rates
 # ------------------------------------------------------------------ line rates
 
 M3DRT_RATES_PRECOND, M3DRT_RATES_OWN_NORM, M3DRT_RATES_LARGEST = 1, 2, 4
+M3DRT_RATES_SCAT, M3DRT_RATES_FLAGS = 8, 16
+M3DRT_BBOP_LTE, M3DRT_BBOP_SPLIT = 1, 2
 
 BB_FIELDS = ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij", "lower_off", "q", "wnu", "wq", "g", "b",
              "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp", "Inu", "Psi", "Bnu", "beta")
+SPLIT_FIELDS = ("Irem", "escape", "chi", "split_opac1", "split_opac2", "split_chi1", "split_chi2", "split_id",
+                "split_chi_id", "chi_scat", "dJnu", "ntrans", "psi_flags_in", "psi_flags_out")
 BB_DTYPES = dict(iblue=torch.int32, ired=torch.int32, lev_i=torch.int32, lev_j=torch.int32,
                  Aij=torch.float64, Bij=torch.float64, lower_off=torch.float32, q=torch.float32,
                  wnu=torch.float64, wq=torch.float64, g=torch.float64, b=torch.float64, nstar=torch.float64,
                  pop_ratio=torch.float64, inv_df=torch.float32, vproj=torch.float32, e_kT=torch.float32,
                  adamp=torch.float32, Inu=torch.float32, Psi=torch.float32, Bnu=torch.float32,
-                 beta=torch.float32)
+                 beta=torch.float32, Irem=torch.float32, escape=torch.float32, chi=torch.float32,
+                 split_opac1=torch.float32, split_opac2=torch.float32, split_chi1=torch.float32,
+                 split_chi2=torch.float32, split_id=torch.int32, split_chi_id=torch.int32,
+                 chi_scat=torch.float32, dJnu=torch.float32, ntrans=torch.int32, psi_flags_in=torch.int32,
+                 psi_flags_out=torch.int32, xhat=torch.float32, ehat=torch.float32, wphi=torch.float32,
+                 wphi_line=torch.int32)
 
 
 class BbRatesArgs(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in ("nline", "nf", "ncell", "nk", "flags")]
                 + [("wmu", C.c_float), ("wtot", C.c_float), ("sf", C.c_int64), ("sc", C.c_int64)]
-                + [(n, C.c_void_p) for n in BB_FIELDS + ("gamma_ij", "gamma_ji")])
+                + [(n, C.c_void_p) for n in BB_FIELDS + ("gamma_ij", "gamma_ji")]
+                + [("split_offset", C.c_int32), ("npw", C.c_int32)]
+                + [(n, C.c_void_p) for n in SPLIT_FIELDS])
+
+
+class BbOpacityArgs(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("nline", "nf", "ncell", "nk", "flags", "split_offset")]
+                + [("wmu", C.c_float), ("sf", C.c_int64), ("sc", C.c_int64)]
+                + [(n, C.c_void_p) for n in ("iblue", "ired", "lev_i", "lev_j", "Bij", "lower_off", "q", "wq",
+                                              "wphi_line", "b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT",
+                                              "adamp", "xhat", "ehat", "split_opac1", "split_opac2",
+                                              "split_chi1", "split_chi2", "split_id", "split_chi_id", "wphi")])
+
+
+class BfRatesArgs(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("ncont", "nline", "nf", "ncell", "nk", "precond")]
+                + [("cut_x", C.c_double), ("sf", C.c_int64), ("sc", C.c_int64)]
+                + [(n, C.c_void_p) for n in ("iblue", "ired", "lev_i", "lev_j", "active", "ev", "wnu", "alpha",
+                                              "twohnu3_c2", "e_kT", "nstar", "b", "Jnu", "beta", "PsiJ",
+                                              "gamma_ij", "gamma_ji")])
+
+
+BF_DTYPES = dict(iblue=torch.int32, ired=torch.int32, lev_i=torch.int32, lev_j=torch.int32, active=torch.int32,
+                 ev=torch.float64, wnu=torch.float64, alpha=torch.float32, twohnu3_c2=torch.float32,
+                 e_kT=torch.float64, nstar=torch.float64, b=torch.float64, Jnu=torch.float32, beta=torch.float32,
+                 PsiJ=torch.float32)
+
+
+def _ali_lib():
+    L = lib()
+    if not getattr(L, "_ali_ok", False):
+        L.m3drt_bb_rates.argtypes = [C.c_void_p, C.POINTER(BbRatesArgs)]
+        L.m3drt_bb_opacity.argtypes = [C.c_void_p, C.POINTER(BbOpacityArgs)]
+        L.m3drt_bf_rates.argtypes = [C.c_void_p, C.POINTER(BfRatesArgs)]
+        for n in ("m3drt_bb_rates", "m3drt_bb_opacity", "m3drt_bf_rates"):
+            getattr(L, n).restype = C.c_int
+        L._ali_ok = True
+    return L
+
+
+def _check_dev(d, dtypes, what):
+    for k, t in d.items():
+        if t is not None and (t.dtype != dtypes[k] or not t.is_cuda or not t.is_contiguous()):
+            raise M3drtError(f"{what}: {k} must be a contiguous {dtypes[k]} device tensor")
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _strides(layout, nf, ncell):
+    if layout == "native":
+        return ncell, 1
+    if layout == "reference":
+        return 1, nf
+    raise M3drtError(f"unknown layout {layout!r}")
 
 
 def _bb_rates(self, d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, largest=False,
-              layout="native"):
+              layout="native", split=None, split_offset=0):
     """bound_bound_rates of one direction on the device.  d: dict of device tensors
-    (BB_FIELDS, dtypes BB_DTYPES); spectra Inu, Psi, Bnu, beta are (nf, ncell) in the
-    native layout or (ncell, nf) in the reference layout; gamma_ij, gamma_ji
-    (nline, ncell) float32 are accumulated in place."""
-    L = lib()
-    if not getattr(L, "_bb_ok", False):
-        L.m3drt_bb_rates.argtypes = [C.c_void_p, C.POINTER(BbRatesArgs)]
-        L.m3drt_bb_rates.restype = C.c_int
-        L._bb_ok = True
-    for k, t in d.items():
-        if t is not None and (t.dtype != BB_DTYPES[k] or not t.is_cuda or not t.is_contiguous()):
-            raise M3drtError(f"bb_rates: {k} must be a contiguous {BB_DTYPES[k]} device tensor")
+    (BB_FIELDS, dtypes BB_DTYPES); spectra are (nf, ncell) in the native layout or
+    (ncell, nf) in the reference layout; gamma_ij, gamma_ji (nline, ncell) float32 are
+    accumulated in place.  With largest and precond, split holds SPLIT_FIELDS (psi
+    flags when it holds ntrans, psi_flags_in and psi_flags_out, which is OR-ed into)."""
+    L = _ali_lib()
+    sp = split or {}
+    _check_dev(d, BB_DTYPES, "bb_rates")
+    _check_dev(sp, BB_DTYPES, "bb_rates")
     for t in (gamma_ij, gamma_ji):
         if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
             raise M3drtError("bb_rates: gamma_ij/gamma_ji are contiguous float32 device tensors")
     nline, nf = d["q"].shape
     nk, ncell = d["b"].shape
-    sf, sc = (ncell, 1) if layout == "native" else (1, nf)
+    sf, sc = _strides(layout, nf, ncell)
     flags = ((M3DRT_RATES_PRECOND if precond else 0) | (M3DRT_RATES_OWN_NORM if own_norm else 0)
-             | (M3DRT_RATES_LARGEST if largest else 0))
-    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
-    a = BbRatesArgs(nline, nf, ncell, nk, flags, wmu, wtot, sf, sc, *[p(d.get(k)) for k in BB_FIELDS],
-                    p(gamma_ij), p(gamma_ji))
+             | (M3DRT_RATES_LARGEST if largest else 0) | (M3DRT_RATES_SCAT if sp.get("chi_scat") is not None else 0)
+             | (M3DRT_RATES_FLAGS if sp.get("psi_flags_in") is not None else 0))
+    npw = int(sp["psi_flags_in"].shape[0]) if sp.get("psi_flags_in") is not None else 0
+    a = BbRatesArgs(nline, nf, ncell, nk, flags, wmu, wtot, sf, sc, *[_p(d.get(k)) for k in BB_FIELDS],
+                    _p(gamma_ij), _p(gamma_ji), split_offset, npw, *[_p(sp.get(k)) for k in SPLIT_FIELDS])
     self._bind_stream()
     _check(L.m3drt_bb_rates(self._h, C.byref(a)), "m3drt_bb_rates")
 
 
+def _bb_opacity(self, d, xhat, ehat, wmu=0.0, lte=False, split=None, split_offset=0, wphi_line=None, wphi=None,
+                layout="native"):
+    """add_bound_bound_opacities of one direction on the device: xhat, ehat (and the
+    split tracking arrays, and wphi) are updated in place."""
+    L = _ali_lib()
+    sp = split or {}
+    _check_dev(d, BB_DTYPES, "bb_opacity")
+    _check_dev(dict(sp, xhat=xhat, ehat=ehat, wphi=wphi, wphi_line=wphi_line), BB_DTYPES, "bb_opacity")
+    nline, nf = d["q"].shape
+    nk, ncell = d["nstar"].shape
+    sf, sc = _strides(layout, nf, ncell)
+    flags = (M3DRT_BBOP_LTE if lte else 0) | (M3DRT_BBOP_SPLIT if split is not None else 0)
+    a = BbOpacityArgs(nline, nf, ncell, nk, flags, split_offset, wmu, sf, sc,
+                      *[_p(d.get(k)) for k in ("iblue", "ired", "lev_i", "lev_j", "Bij", "lower_off", "q", "wq")],
+                      _p(wphi_line),
+                      *[_p(d.get(k)) for k in ("b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp")],
+                      _p(xhat), _p(ehat),
+                      *[_p(sp.get(k)) for k in ("split_opac1", "split_opac2", "split_chi1", "split_chi2", "split_id",
+                                                "split_chi_id")],
+                      _p(wphi))
+    self._bind_stream()
+    _check(L.m3drt_bb_opacity(self._h, C.byref(a)), "m3drt_bb_opacity")
+
+
+def _bf_rates(self, d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0, layout="native"):
+    """bound_free_rates on the device; d: BF_DTYPES tensors, spectra (nf, ncell)
+    native or (ncell, nf) reference; gamma arrays (nline + ncont, ncell)."""
+    L = _ali_lib()
+    _check_dev(d, BF_DTYPES, "bf_rates")
+    ncont, nf = d["alpha"].shape
+    nk, ncell = d["nstar"].shape
+    sf, sc = _strides(layout, nf, ncell)
+    a = BfRatesArgs(ncont, nline, nf, ncell, nk, int(precond), cut_x, sf, sc,
+                    *[_p(d.get(k)) for k in ("iblue", "ired", "lev_i", "lev_j", "active", "ev", "wnu", "alpha",
+                                             "twohnu3_c2", "e_kT", "nstar", "b", "Jnu", "beta", "PsiJ")],
+                    _p(gamma_ij), _p(gamma_ji))
+    self._bind_stream()
+    _check(L.m3drt_bf_rates(self._h, C.byref(a)), "m3drt_bf_rates")
+
+
 Context.bb_rates = _bb_rates
+Context.bb_opacity = _bb_opacity
+Context.bf_rates = _bf_rates

@@ -211,9 +211,12 @@ int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, con
 
 /* ------------------------------------------------------------------ line rates */
 
-#define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T ('original' formulation) */
+#define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T */
 #define M3DRT_RATES_OWN_NORM 2  /* the line lies inside the window: own profile normalisation */
-#define M3DRT_RATES_LARGEST 4   /* rate_formulation='largest' (without preconditioning) */
+#define M3DRT_RATES_LARGEST 4   /* rate_formulation='largest' (split rates; with PRECOND the
+                                   tracked-contribution integrands) */
+#define M3DRT_RATES_SCAT 8      /* line scattering: other += chi_scat * dJnu */
+#define M3DRT_RATES_FLAGS 16    /* psi_flags: read in-force bits, raise new ones */
 
 /* One direction's line-rate integrals.  (nline, nf) line arrays; (f, cell) fields at
  * f*sf + cell*sc; (nk, ncell) and (nline, ncell) per-cell arrays cell-fastest. */
@@ -232,13 +235,74 @@ typedef struct {
   const float *adamp;                       /* (nline, ncell) damping in Doppler units */
   const float *Inu, *Psi, *Bnu, *beta;      /* this direction's intensity and Psi; B_nu, beta */
   float *gamma_ij, *gamma_ji;               /* (nline, ncell) th and dev, accumulated */
+  /* LARGEST | PRECOND: the solver's split parts and the tracked contributions */
+  int32_t split_offset, npw;                /* index base of this atom's lines; psi_flags words */
+  const float *Irem, *escape, *chi;         /* (f, cell) Irem, escape, total chi (work slot 1) */
+  const float *split_opac1, *split_opac2;   /* (f, cell) largest eta term, the rest */
+  const float *split_chi1, *split_chi2;     /* (f, cell) largest chi term, the rest */
+  const int32_t *split_id, *split_chi_id;   /* (f, cell) index of the largest terms */
+  const float *chi_scat, *dJnu;             /* (f, cell), M3DRT_RATES_SCAT */
+  const int32_t *ntrans;                    /* (nline) transition numbers, M3DRT_RATES_FLAGS */
+  const int32_t *psi_flags_in;              /* (npw, ncell) words: bits in force */
+  int32_t *psi_flags_out;                   /* (npw, ncell) words: bits raised (OR-ed in) */
 } m3drt_bb_rates_args;
 
 /* bound_bound_rates for one direction (SRC/rt_patch_mod.f90:2852-3142), FP32 with
- * Kahan-compensated frequency sums.  A line whose window is clipped to [1, nf]; a line
- * whose level index lies outside [1, nk] is skipped.  M3DRT_RATES_LARGEST with
- * M3DRT_RATES_PRECOND (the tracked split contributions) returns M3DRT_EINVAL. */
+ * Kahan-compensated frequency sums: 'original' (with or without preconditioning),
+ * 'largest' without preconditioning, and 'largest' with it (Rybicki & Hummer
+ * integrands from Irem, escape and the tracked contributions, psi_flags).  A line's
+ * window is clipped to [1, nf]; a line whose level index lies outside [1, nk] is
+ * skipped. */
 int m3drt_bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *args);
+
+/* ------------------------------------------------------------------ line opacity of a model atom */
+
+#define M3DRT_BBOP_LTE 1    /* b = 1 (the LTE companion set) */
+#define M3DRT_BBOP_SPLIT 2  /* track the largest contribution per point (add_largest) */
+
+typedef struct {
+  int32_t nline, nf, ncell, nk, flags, split_offset;
+  float wmu;                                /* direction weight for wphi */
+  int64_t sf, sc;
+  const int32_t *iblue, *ired, *lev_i, *lev_j;
+  const double *Bij;                        /* (nline) */
+  const float *lower_off, *q;               /* (nline), (nline, nf) */
+  const double *wq;                         /* (nline, nf) */
+  const int32_t *wphi_line;                 /* (nline) 1: add this direction's wphi share; or NULL */
+  const double *b, *nstar, *pop_ratio;      /* (nk, ncell) x2, (ncell) */
+  const float *inv_df, *vproj, *e_kT;       /* (ncell) */
+  const float *adamp;                       /* (nline, ncell) */
+  float *xhat, *ehat;                       /* (f, cell) accumulated */
+  float *split_opac1, *split_opac2, *split_chi1, *split_chi2;  /* (f, cell), SPLIT */
+  int32_t *split_id, *split_chi_id;         /* (f, cell), SPLIT */
+  float *wphi;                              /* (nline, ncell) += wmu * sum(phi wq) */
+} m3drt_bb_opacity_args;
+
+/* add_bound_bound_opacities (SRC/rt_patch_mod.f90:2689-2831) for one direction: the
+ * model atom's (non-hydrogen) lines into xhat/ehat, add_largest tracking (3150-3169)
+ * and the direction's wphi share.  Lines visit each point in list order. */
+int m3drt_bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *args);
+
+/* ------------------------------------------------------------------ bound-free rates */
+
+typedef struct {
+  int32_t ncont, nline, nf, ncell, nk, precond;
+  double cut_x;                             /* radiative_cut threshold (dev * e/kT) */
+  int64_t sf, sc;
+  const int32_t *iblue, *ired, *lev_i, *lev_j, *active;  /* (ncont), 1-based */
+  const double *ev;                         /* (nk) level energies [eV] */
+  const double *wnu;                        /* (ncont, nf) frequency weights */
+  const float *alpha;                       /* (ncont, nf) cross sections */
+  const float *twohnu3_c2;                  /* (nf) */
+  const double *e_kT;                       /* (ncell) */
+  const double *nstar, *b;                  /* (nk, ncell) */
+  const float *Jnu, *beta, *PsiJ;           /* (f, cell) */
+  float *gamma_ij, *gamma_ji;               /* (nline + ncont, ncell); continua from row nline */
+} m3drt_bf_rates_args;
+
+/* bound_free_rates (SRC/rt_patch_mod.f90:3193-3259): photoionisation and
+ * recombination sums from J, with the PsiJ preconditioning. */
+int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *args);
 
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */

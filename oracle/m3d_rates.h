@@ -24,9 +24,58 @@ typedef struct {
   const float *adamp;                           /* (nline, ncell) */
   const float *Inu, *Psi, *Bnu, *beta;          /* (f, cell) at f*sf + cell*sc */
   size_t sf, sc;
+  /* 'largest' with preconditioning (the tracked split contributions) */
+  int split_offset, use_flags, do_scat, npw;
+  const float *Irem, *escape, *chi;             /* (f, cell): the solver's split parts, total chi */
+  const float *split_opac1, *split_opac2, *split_chi1, *split_chi2;  /* (f, cell) */
+  const int32_t *split_id, *split_chi_id;       /* (f, cell) */
+  const float *chi_scat, *dJnu;                 /* (f, cell), do_scat */
+  const int32_t *ntrans;                        /* (nline) transition numbers (psi_flags bits) */
+  const int32_t *psi_flags_in;                  /* (npw, ncell) words, cells fastest */
+  int32_t *psi_flags_out;                       /* (npw, ncell) raised bits */
 } orc_bb_args;
 
 void orc_bb_rates(const orc_bb_args *a, float *gamma_ij, float *gamma_ji);
+
+/* add_bound_bound_opacities (SRC/rt_patch_mod.f90:2689-2831) with add_largest
+ * (3150-3169) and the direction's wphi share */
+typedef struct {
+  int nline, nf, ncell, nk;
+  int lte, split, split_offset;
+  const int32_t *iblue, *ired, *lev_i, *lev_j;
+  const double *Bij;
+  const float *lower_off, *q;                   /* (nline), (nline, nf) */
+  const double *wq;                             /* (nline, nf) */
+  const int32_t *wphi_line;                     /* (nline) 1 = sum wphi for the line; NULL = none */
+  const double *b, *nstar, *pop_ratio;          /* (nk, ncell) x2, (ncell) */
+  const float *inv_df, *vproj, *e_kT, *adamp;   /* (ncell) x3, (nline, ncell) */
+  float wmu;
+  size_t sf, sc;
+  float *xhat, *ehat;                           /* (f, cell) */
+  float *split_opac1, *split_opac2, *split_chi1, *split_chi2;
+  int32_t *split_id, *split_chi_id;
+  float *wphi;                                  /* (nline, ncell), += wmu * sum phi wq */
+} orc_bbop_args;
+
+void orc_bb_opacity(const orc_bbop_args *a);
+
+/* bound_free_rates (SRC/rt_patch_mod.f90:3193-3259) */
+typedef struct {
+  int ncont, nline, nf, ncell, nk, precond;
+  double cut_x;
+  const int32_t *iblue, *ired, *lev_i, *lev_j, *active;  /* (ncont), 1-based */
+  const double *ev;                             /* (nk) */
+  const double *wnu;                            /* (ncont, nf) */
+  const float *alpha;                           /* (ncont, nf) */
+  const float *twohnu3_c2;                      /* (nf) */
+  const double *e_kT;                           /* (ncell) */
+  const double *nstar, *b;                      /* (nk, ncell) */
+  const float *Jnu, *beta, *PsiJ;               /* (f, cell) */
+  size_t sf, sc;
+} orc_bf_args;
+
+/* gamma arrays (nline + ncont, ncell); the continua at rows nline.. */
+void orc_bf_rates(const orc_bf_args *a, float *gamma_ij, float *gamma_ji);
 
 #ifdef __cplusplus
 }

@@ -463,8 +463,17 @@ class BbArgs(C.Structure):
                 + [("wmu", C.c_float), ("wtot", C.c_float)]
                 + [(n, C.c_void_p) for n in ("g", "b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp",
                                               "Inu", "Psi", "Bnu", "beta")]
-                + [("sf", C.c_size_t), ("sc", C.c_size_t)])
+                + [("sf", C.c_size_t), ("sc", C.c_size_t)]
+                + [(n, C.c_int) for n in ("split_offset", "use_flags", "do_scat", "npw")]
+                + [(n, C.c_void_p) for n in ("Irem", "escape", "chi", "split_opac1", "split_opac2", "split_chi1",
+                                              "split_chi2", "split_id", "split_chi_id", "chi_scat", "dJnu",
+                                              "ntrans", "psi_flags_in", "psi_flags_out")])
 
+
+SPLIT_FIELDS = dict(Irem=np.float32, escape=np.float32, chi=np.float32, split_opac1=np.float32,
+                    split_opac2=np.float32, split_chi1=np.float32, split_chi2=np.float32, split_id=np.int32,
+                    split_chi_id=np.int32, chi_scat=np.float32, dJnu=np.float32, ntrans=np.int32,
+                    psi_flags_in=np.int32)
 
 RATE_FIELDS = dict(  # name -> dtype of the line-rate inputs (one direction)
     iblue=np.int32, ired=np.int32, lev_i=np.int32, lev_j=np.int32, Aij=np.float64, Bij=np.float64,
@@ -473,10 +482,15 @@ RATE_FIELDS = dict(  # name -> dtype of the line-rate inputs (one direction)
     adamp=np.float32, Inu=np.float32, Psi=np.float32, Bnu=np.float32, beta=np.float32)
 
 
-def bb_rates(d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, largest=False, kahan=False):
+def bb_rates(d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, largest=False, kahan=False,
+             split=None, split_offset=0, psi_flags=None):
     """bound_bound_rates for one direction (SRC/rt_patch_mod.f90:2852-3142).  d holds
-    the arrays of RATE_FIELDS; spectra (Inu, Psi, Bnu, beta) are (nf, ncell).
-    Returns the accumulated (gamma_ij, gamma_ji), each (nline, ncell) float32."""
+    the arrays of RATE_FIELDS; spectra (Inu, Psi, Bnu, beta) are (nf, ncell).  With
+    largest and precond, split holds the SPLIT_FIELDS arrays (the solver's Irem and
+    escape, total chi, the tracked contributions; chi_scat and dJnu for line
+    scattering; ntrans and psi_flags_in (npw, ncell) for the psi flags).
+    Returns the accumulated (gamma_ij, gamma_ji), each (nline, ncell) float32, and
+    with psi flags the raised words (npw, ncell)."""
     L = lib()
     if not getattr(L, "_bb_ok", False):
         L.orc_bb_rates.argtypes = [C.POINTER(BbArgs), C.c_void_p, C.c_void_p]
@@ -485,14 +499,104 @@ def bb_rates(d, wmu, wtot, gamma_ij, gamma_ji, precond=False, own_norm=False, la
     keep = _Keep()
     nline, nf = d["q"].shape
     nk, ncell = d["b"].shape
+    sp = split or {}
+    use_flags = "psi_flags_in" in sp
+    npw = sp["psi_flags_in"].shape[0] if use_flags else 0
+    flags_out = np.zeros((max(npw, 1), ncell), np.int32)
     a = BbArgs(nline, nf, ncell, nk, int(precond), int(own_norm), int(largest), int(kahan),
                *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij",
                                                            "lower_off", "q", "wnu", "wq")],
                wmu, wtot,
                *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("g", "b", "nstar", "pop_ratio", "inv_df", "vproj",
                                                            "e_kT", "adamp", "Inu", "Psi", "Bnu", "beta")],
-               ncell, 1)
+               ncell, 1, split_offset, int(use_flags), int("chi_scat" in sp), npw,
+               *[keep.p(sp.get(k), SPLIT_FIELDS[k]) for k in ("Irem", "escape", "chi", "split_opac1", "split_opac2",
+                                                             "split_chi1", "split_chi2", "split_id", "split_chi_id",
+                                                             "chi_scat", "dJnu", "ntrans", "psi_flags_in")],
+               flags_out.ctypes.data_as(C.c_void_p))
     gij = np.array(gamma_ij, np.float32, copy=True)
     gji = np.array(gamma_ji, np.float32, copy=True)
     L.orc_bb_rates(C.byref(a), gij.ctypes.data_as(C.c_void_p), gji.ctypes.data_as(C.c_void_p))
+    if use_flags:
+        return gij, gji, flags_out
+    return gij, gji
+
+
+class BbopArgs(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("nline", "nf", "ncell", "nk", "lte", "split", "split_offset")]
+                + [(n, C.c_void_p) for n in ("iblue", "ired", "lev_i", "lev_j", "Bij", "lower_off", "q", "wq",
+                                              "wphi_line", "b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT",
+                                              "adamp")]
+                + [("wmu", C.c_float), ("sf", C.c_size_t), ("sc", C.c_size_t)]
+                + [(n, C.c_void_p) for n in ("xhat", "ehat", "split_opac1", "split_opac2", "split_chi1",
+                                              "split_chi2", "split_id", "split_chi_id", "wphi")])
+
+
+def bb_opacity(d, xhat, ehat, wmu=0.0, lte=False, split=None, split_offset=0, wphi_line=None, wphi=None):
+    """add_bound_bound_opacities (SRC/rt_patch_mod.f90:2689-2831) over (nf, ncell)
+    spectra xhat, ehat; split: dict of the tracked contributions (split_opac1/2,
+    split_id, split_chi1/2, split_chi_id), updated like xhat and ehat; wphi (nline,
+    ncell) gains wmu * sum(phi wq) for the lines wphi_line marks.  Returns copies."""
+    L = lib()
+    if not getattr(L, "_bbop_ok", False):
+        L.orc_bb_opacity.argtypes = [C.POINTER(BbopArgs)]
+        L.orc_bb_opacity.restype = None
+        L._bbop_ok = True
+    keep = _Keep()
+    nline, nf = d["q"].shape
+    nk, ncell = d["b"].shape
+    out = dict(xhat=np.array(xhat, np.float32, copy=True), ehat=np.array(ehat, np.float32, copy=True))
+    if split is not None:
+        for k in ("split_opac1", "split_opac2", "split_chi1", "split_chi2"):
+            out[k] = np.array(split[k], np.float32, copy=True)
+        for k in ("split_id", "split_chi_id"):
+            out[k] = np.array(split[k], np.int32, copy=True)
+    if wphi is not None:
+        out["wphi"] = np.array(wphi, np.float32, copy=True)
+    ptr = lambda k: out[k].ctypes.data_as(C.c_void_p) if k in out else None
+    a = BbopArgs(nline, nf, ncell, nk, int(lte), int(split is not None), split_offset,
+                 *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("iblue", "ired", "lev_i", "lev_j", "Bij", "lower_off",
+                                                             "q", "wq")],
+                 keep.p(wphi_line, np.int32),
+                 *[keep.p(d.get(k), RATE_FIELDS[k]) for k in ("b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT",
+                                                             "adamp")],
+                 wmu, ncell, 1,
+                 *[ptr(k) for k in ("xhat", "ehat", "split_opac1", "split_opac2", "split_chi1", "split_chi2",
+                                    "split_id", "split_chi_id", "wphi")])
+    L.orc_bb_opacity(C.byref(a))
+    return out
+
+
+class BfArgs(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("ncont", "nline", "nf", "ncell", "nk", "precond")]
+                + [("cut_x", C.c_double)]
+                + [(n, C.c_void_p) for n in ("iblue", "ired", "lev_i", "lev_j", "active", "ev", "wnu", "alpha",
+                                              "twohnu3_c2", "e_kT", "nstar", "b", "Jnu", "beta", "PsiJ")]
+                + [("sf", C.c_size_t), ("sc", C.c_size_t)])
+
+
+BF_FIELDS = dict(iblue=np.int32, ired=np.int32, lev_i=np.int32, lev_j=np.int32, active=np.int32, ev=np.float64,
+                 wnu=np.float64, alpha=np.float32, twohnu3_c2=np.float32, e_kT=np.float64, nstar=np.float64,
+                 b=np.float64, Jnu=np.float32, beta=np.float32, PsiJ=np.float32)
+
+
+def bf_rates(d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0):
+    """bound_free_rates (SRC/rt_patch_mod.f90:3193-3259): d holds BF_FIELDS, spectra
+    (nf, ncell); gamma arrays (nline + ncont, ncell), the continua from row nline."""
+    L = lib()
+    if not getattr(L, "_bf_ok", False):
+        L.orc_bf_rates.argtypes = [C.POINTER(BfArgs), C.c_void_p, C.c_void_p]
+        L.orc_bf_rates.restype = None
+        L._bf_ok = True
+    keep = _Keep()
+    ncont, nf = d["alpha"].shape
+    nk, ncell = d["nstar"].shape
+    a = BfArgs(ncont, nline, nf, ncell, nk, int(precond), cut_x,
+               *[keep.p(d.get(k), BF_FIELDS[k]) for k in ("iblue", "ired", "lev_i", "lev_j", "active", "ev", "wnu",
+                                                         "alpha", "twohnu3_c2", "e_kT", "nstar", "b", "Jnu", "beta",
+                                                         "PsiJ")],
+               ncell, 1)
+    gij = np.array(gamma_ij, np.float32, copy=True)
+    gji = np.array(gamma_ji, np.float32, copy=True)
+    L.orc_bf_rates(C.byref(a), gij.ctypes.data_as(C.c_void_p), gji.ctypes.data_as(C.c_void_p))
     return gij, gji

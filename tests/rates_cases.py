@@ -50,7 +50,7 @@ def make_rates_case(ncell=257, nf=240, nline=6, seed=0, flat=False):
     iblue = np.zeros(nline, np.int32)
     ired = np.zeros(nline, np.int32)
     for l in range(nline):
-        width = 1 if l == 1 else int(rng.integers(20, 91))
+        width = 1 if l == 1 else int(rng.integers(min(20, nf), min(90, nf) + 1))
         fs = int(rng.integers(0, nf - width + 1))
         iblue[l], ired[l] = fs + 1, fs + width
         qq = np.linspace(-2.5e-5, 2.5e-5, width) if width > 1 else np.zeros(1)
@@ -99,3 +99,69 @@ def level_products(d):
     ni = d["b"][li] * d["nstar"][li] * eoff
     nj = d["b"][lj] * d["nstar"][lj] * eoff
     return ni, nj
+
+
+def make_equal_pair_case(ncell=48, nf=160, nline=5, seed=0):
+    """Lines that all share one energy gap (2 eV) and one statistical-weight ratio,
+    so beta_ref is the same for every line of a cell, and beta(f) = beta_ref,
+    Bnu = beta/(1-beta) across the grid.  Overlapping ragged windows, departures
+    b of either sign of 1, some lines with a lower-level offset."""
+    rng = np.random.default_rng(seed)
+    npair = 4
+    nk = 2 * npair
+    g = np.tile([2.0, 6.0], npair)
+    ev = np.repeat(np.linspace(0.0, 1.5, npair), 2) + np.tile([0.0, 2.0], npair)
+    T = rng.uniform(4500.0, 8000.0, ncell)
+    e_kT = synth.EV_K / T
+    totn = 10 ** rng.uniform(8.0, 10.0, ncell)
+    w = g[:, None] * np.exp(-ev[:, None] * e_kT[None, :])
+    nstar = w * totn / w.sum(0)
+    b = rng.uniform(0.4, 2.5, nstar.shape)
+    pick = rng.integers(0, npair, nline)
+    lev_i = (2 * pick + 1).astype(np.int32)
+    lev_j = (2 * pick + 2).astype(np.int32)
+    beta_ref = np.exp(-2.0 * e_kT)
+    d = dict(lev_i=lev_i, lev_j=lev_j, g=g, b=b, nstar=nstar, e_kT=e_kT.astype(np.float32))
+    d["Aij"] = 10 ** rng.uniform(6, 8, nline)
+    nu0 = 2.0 * 2.417989242e14
+    q = np.zeros((nline, nf), np.float32)
+    wq = np.zeros((nline, nf))
+    wnu = np.zeros((nline, nf))
+    iblue = np.zeros(nline, np.int32)
+    ired = np.zeros(nline, np.int32)
+    for l in range(nline):
+        width = int(rng.integers(min(30, nf), min(120, nf) + 1))
+        fs = int(rng.integers(0, nf - width + 1))
+        iblue[l], ired[l] = fs + 1, fs + width
+        qq = np.linspace(-2.5e-5, 2.5e-5, width) + rng.uniform(-5e-6, 5e-6)
+        q[l, fs:fs + width] = qq
+        wq[l, fs:fs + width] = trapezoid(qq.astype(np.float32).astype(np.float64))
+        wnu[l, fs:fs + width] = nu0 * wq[l, fs:fs + width]
+    d.update(iblue=iblue, ired=ired, q=q, wq=wq, wnu=wnu)
+    d["lower_off"] = np.where(rng.uniform(size=nline) < 0.4, 0.0, rng.uniform(0.01, 0.3, nline)).astype(np.float32)
+    d["inv_df"] = (1.0 / rng.uniform(2e-6, 5e-6, ncell)).astype(np.float32)
+    d["vproj"] = rng.uniform(-5e-6, 5e-6, ncell).astype(np.float32)
+    d["adamp"] = (10 ** rng.uniform(-3, -0.5, (nline, ncell))).astype(np.float32)
+    d["pop_ratio"] = rng.uniform(0.5, 2.0, ncell) * 1e-2
+    phi_peak = 0.5641895835 * float(np.median(d["inv_df"]))
+    nsi_med = np.median(nstar[lev_i - 1], axis=1)
+    d["Bij"] = 0.3 / (nsi_med * 1e-2 * phi_peak)
+    beta = np.broadcast_to(beta_ref[None, :], (nf, ncell)).astype(np.float32)
+    d["beta"] = np.ascontiguousarray(beta)
+    d["Bnu"] = np.ascontiguousarray((beta.astype(np.float64) / (1.0 - beta)).astype(np.float32))
+    d["Inu"] = np.ascontiguousarray((d["Bnu"] * rng.uniform(0.3, 2.0, (nf, ncell))).astype(np.float32))
+    d["Psi"] = np.zeros((nf, ncell), np.float32)
+    return d
+
+
+def split_solver_outputs(d, opac, Psi):
+    """The solver's split outputs consistent with the tracked opacities: total chi =
+    xhat - beta ehat, eta = beta ehat, Irem = Inu - Psi eta, escape = 1 - Psi chi."""
+    beta = d["beta"].astype(np.float32)
+    chi = (opac["xhat"] - opac["ehat"] * beta).astype(np.float32)
+    eta = (opac["ehat"] * beta).astype(np.float32)
+    Psi = Psi.astype(np.float32)
+    return dict(Irem=(d["Inu"] - Psi * eta).astype(np.float32), escape=(1.0 - Psi * chi).astype(np.float32),
+                chi=chi, split_opac1=opac["split_opac1"], split_opac2=opac["split_opac2"],
+                split_chi1=opac["split_chi1"], split_chi2=opac["split_chi2"], split_id=opac["split_id"],
+                split_chi_id=opac["split_chi_id"])

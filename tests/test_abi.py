@@ -39,29 +39,36 @@ def test_header_symbols_are_exported(lib):
         getattr(lib, n)
 
 
+MIRRORS = {  # ctypes mirror in dispatch_amd/m3drt.py -> header typedef
+    "SweepDesc": "m3drt_sweep_desc", "SweepBufs": "m3drt_sweep_bufs", "Linelist": "m3drt_linelist",
+    "OpacityArgs": "m3drt_opacity_args", "RateSolveArgs": "m3drt_rate_solve_args",
+    "RateSolveResult": "m3drt_rate_solve_result", "BbRatesArgs": "m3drt_bb_rates_args",
+    "BbOpacityArgs": "m3drt_bb_opacity_args", "BfRatesArgs": "m3drt_bf_rates_args",
+}
+
+
 def test_struct_layout_matches_header():
-    # the C compiler's offsets of the descriptor fields vs the ctypes mirror
+    """Every field of every ctypes mirror sits at the C compiler's offset (a
+    misnamed field fails to compile), and the sizes agree."""
     from dispatch_amd import m3drt
-    src = r'''
-#include <stdio.h>
-#include <stddef.h>
-#include "m3drt.h"
-int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(m3drt_sweep_desc), offsetof(m3drt_sweep_desc, ds),
-         offsetof(m3drt_sweep_desc, wj), offsetof(m3drt_sweep_desc, cs_f),
-         offsetof(m3drt_sweep_desc, eta_ld), sizeof(m3drt_sweep_bufs));
-  return 0;
-}'''
+    lines, want = [], []
+    for py, c in MIRRORS.items():
+        S = getattr(m3drt, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        want.append(C.sizeof(S))
+        for name, _ in S._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {name}));')
+            want.append(getattr(S, name).offset)
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"m3drt.h\"\nint main(void) {\n" + \
+          "\n".join(lines) + "\n  return 0;\n}\n"
     import tempfile
     with tempfile.TemporaryDirectory() as d:
-        c = os.path.join(d, "t.c")
-        open(c, "w").write(src)
+        cfile = os.path.join(d, "t.c")
+        open(cfile, "w").write(src)
         exe = os.path.join(d, "t")
-        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
-    D = m3drt.SweepDesc
-    assert vals == [C.sizeof(D), D.ds.offset, D.wj.offset, D.cs_f.offset, D.eta_ld.offset,
-                    C.sizeof(m3drt.SweepBufs)]
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), cfile, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    assert got == want
 
 
 def test_errors_are_status_codes(lib):

@@ -81,7 +81,103 @@ def test_bb_rates_two_directions_and_empty(ctx):
 def test_bb_rates_rejects_bad_arguments(ctx):
     d = to_dev(make_rates_case(ncell=64, nf=60, nline=3, seed=13), "native")
     g = torch.zeros((3, 64), dtype=torch.float32, device="cuda")
-    with pytest.raises(m3drt.M3drtError, match="preconditioned 'largest'"):
+    with pytest.raises(m3drt.M3drtError, match="'largest' with preconditioning needs"):
         ctx.bb_rates(d, WMU, 1.0, g, g, precond=True, largest=True)
     with pytest.raises(m3drt.M3drtError, match="own normalisation needs wq"):
         ctx.bb_rates(dict(d, wq=None), WMU, 1.0, g, g, own_norm=True)
+
+
+def dev_dict(d, dtypes=None):
+    dtypes = dtypes or m3drt.BB_DTYPES
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(dtypes[k]).cuda() for k, v in d.items()}
+
+
+def zeros_split(nf, ncell):
+    z = np.zeros((nf, ncell), np.float32)
+    zi = np.zeros((nf, ncell), np.int32)
+    return dict(split_opac1=z, split_opac2=z, split_chi1=z, split_chi2=z, split_id=zi, split_chi_id=zi)
+
+
+@pytest.mark.parametrize("lte", [False, True])
+def test_bb_opacity_matches_oracle(ctx, lte):
+    """add_bound_bound_opacities with the split tracking and wphi: the profile
+    arithmetic and the order of the additions are the oracle's, so xhat, ehat, the
+    tracked terms and their indices agree to the last bit except where the FP64
+    wphi sum rounds differently (2e-7)."""
+    from rates_cases import make_rates_case
+    d = make_rates_case(ncell=300, nf=200, nline=6, seed=31)
+    nline, nf = d["q"].shape
+    ncell = d["b"].shape[1]
+    rng = np.random.default_rng(3)
+    x0 = rng.uniform(0, 1e-3, (nf, ncell)).astype(np.float32)
+    e0 = rng.uniform(0, 1e-3, (nf, ncell)).astype(np.float32)
+    sp0 = zeros_split(nf, ncell)
+    wl = (np.arange(nline) % 2).astype(np.int32)
+    w0 = np.full((nline, ncell), 0.25, np.float32)
+    ref = O.bb_opacity(d, x0, e0, wmu=0.3, lte=lte, split=sp0, split_offset=5, wphi_line=wl, wphi=w0)
+    g = dev_dict(dict(xhat=x0, ehat=e0, wphi=w0, wphi_line=wl, **sp0))
+    ctx.bb_opacity(dev_dict(d), g["xhat"], g["ehat"], wmu=0.3, lte=lte,
+                   split={k: g[k] for k in sp0}, split_offset=5, wphi_line=g["wphi_line"], wphi=g["wphi"])
+    torch.cuda.synchronize()
+    for k in ("xhat", "ehat", "split_opac1", "split_opac2", "split_chi1", "split_chi2", "split_id", "split_chi_id"):
+        np.testing.assert_array_equal(g[k].cpu().numpy(), ref[k], err_msg=k)
+    np.testing.assert_allclose(g["wphi"].cpu().numpy(), ref["wphi"], rtol=2e-7)
+
+
+@pytest.mark.parametrize("seed,scat,flags", [(0, False, False), (1, True, True)])
+def test_bb_rates_split_preconditioned_matches_oracle(ctx, seed, scat, flags):
+    from rates_cases import make_equal_pair_case, split_solver_outputs
+    d = make_equal_pair_case(seed=seed, ncell=400)
+    nline, nf = d["q"].shape
+    ncell = d["b"].shape[1]
+    z = np.zeros((nf, ncell), np.float32)
+    opac = O.bb_opacity(d, z, z, split=zeros_split(nf, ncell), split_offset=3)
+    chi = opac["xhat"] - opac["ehat"] * d["beta"]
+    rng = np.random.default_rng(seed + 7)
+    Psi = np.where(chi > 0, rng.uniform(0.05, 0.8, chi.shape) / np.where(chi > 0, chi, 1.0), 0.0).astype(np.float32)
+    Psi[:, ::17] = 0.0
+    d["Psi"] = Psi
+    sp = split_solver_outputs(d, opac, Psi)
+    if scat:
+        sp["chi_scat"] = rng.uniform(0, 1e-3, (nf, ncell)).astype(np.float32)
+        sp["dJnu"] = rng.uniform(-0.1, 0.1, (nf, ncell)).astype(np.float32)
+    if flags:
+        sp["ntrans"] = (np.arange(nline, dtype=np.int32) * 13 + 2)
+        npw = (int(sp["ntrans"].max()) - 1) // 32 + 1
+        sp["psi_flags_in"] = rng.integers(-2**31, 2**31 - 1, (npw, ncell), dtype=np.int64).astype(np.int32)
+    g0 = rng.uniform(-1, 1, (nline, ncell)).astype(np.float32) * 1e10
+    ref = O.bb_rates(d, WMU, 0.8, g0, g0, precond=True, largest=True, own_norm=True, kahan=True, split=sp,
+                     split_offset=3)
+    gsp = dev_dict(sp)
+    if flags:
+        gsp["psi_flags_out"] = torch.zeros_like(gsp["psi_flags_in"])
+    gij = torch.from_numpy(g0.copy()).cuda()
+    gji = torch.from_numpy(g0.copy()).cuda()
+    ctx.bb_rates(dev_dict(d), WMU, 0.8, gij, gji, precond=True, largest=True, own_norm=True, split=gsp,
+                 split_offset=3)
+    torch.cuda.synchronize()
+    check(gij.cpu().numpy(), ref[0], "split gamma_ij")
+    check(gji.cpu().numpy(), ref[1], "split gamma_ji")
+    if flags:
+        np.testing.assert_array_equal(gsp["psi_flags_out"].cpu().numpy(), ref[2])
+
+
+def test_bf_rates_match_oracle(ctx):
+    from test_oracle_ali import bf_case
+    d, nline = bf_case(33, ncell=500)
+    ncont, nf = d["alpha"].shape
+    ncell = d["nstar"].shape[1]
+    rng = np.random.default_rng(4)
+    d["Jnu"] = (d["Jnu"] * rng.uniform(0.5, 1.5, d["Jnu"].shape)).astype(np.float32)
+    d["PsiJ"] = rng.uniform(0, 1e-3, (nf, ncell)).astype(np.float32)
+    d["active"] = np.array([1, 0, 1], np.int32)
+    g0 = np.full((nline + ncont, ncell), 0.5, np.float32)
+    for precond in (False, True):
+        ref = O.bf_rates(d, nline, g0, g0, precond=precond, cut_x=15.0)
+        gij = torch.from_numpy(g0.copy()).cuda()
+        gji = torch.from_numpy(g0.copy()).cuda()
+        ctx.bf_rates(dev_dict(d, m3drt.BF_DTYPES), nline, gij, gji, precond=precond, cut_x=15.0)
+        torch.cuda.synchronize()
+        # the same FP32 sums in the same order: bit for bit
+        np.testing.assert_array_equal(gij.cpu().numpy(), ref[0])
+        np.testing.assert_array_equal(gji.cpu().numpy(), ref[1])

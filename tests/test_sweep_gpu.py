@@ -158,3 +158,32 @@ def test_sweep_rejects_bad_arguments(ctx):
     with pytest.raises(m3drt.M3drtError, match="upwind"):
         ctx.sc_sweep(c, c, p, p, p, dev(case["chi500"]), None, dev(case["g500"]),
                      dir=1, ds=case["ds"], vec=case["vec"])
+
+
+@pytest.mark.parametrize("nx,ny,fp64,psij", [(60, 60, False, False), (60, 60, False, True), (120, 120, False, False),
+                                              (96, 96, False, True), (60, 60, True, False), (120, 120, True, True)])
+def test_sweep_exact_strips(ctx, nx, ny, fp64, psij):
+    """The straight-line kernel form (only J, or J and PsiJ, asked for; the strip
+    length divides ny): J, PsiJ and the exit plane against the oracle."""
+    case = make_case(nf=3, nx=nx, ny=ny, nz=4, seed=nx + ny + int(fp64), vec=(0.61481017, 0.0, 0.78867513))
+    print(f"exact strips {nx}x{ny} fp64={fp64} psij={psij}", flush=True)
+    I, psi, _, _ = run_oracle(case, fp64=fp64)
+    dt = torch.float64 if fp64 else torch.float32
+    gz = case["gz"]
+    cub = lambda a: dev(to_native(interior(a)), dt)
+    pla = lambda a: dev(plane_to_native(a[gz]), dt)
+    c, s = cub(case["chi"]), cub(case["S"])
+    J = torch.full_like(c, 0.125)
+    PsiJ = torch.full_like(c, 0.5) if psij else None
+    exitI = torch.zeros_like(pla(case["chi"]))
+    ctx.sc_sweep(c, s, pla(case["chi"]), pla(case["S"]), pla(case["I"]), dev(case["chi500"]),
+                 dev(case["chi500_up"]), dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"],
+                 wj=0.25, J=J, PsiJ=PsiJ, exit_I=exitI, get_rates=psij)
+    torch.cuda.synchronize()
+    nat = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+    tol = 1e-11 if fp64 else RTOL32
+    check32(nat(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J", rtol=tol)
+    if psij:
+        check32(nat(PsiJ), 0.5 + 0.25 * psi.astype(np.float64), "PsiJ", rtol=tol)
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(np.moveaxis(exitI.double().cpu().numpy(), 0, 2), I[last], "exit plane", rtol=tol)
