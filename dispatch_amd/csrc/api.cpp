@@ -156,14 +156,24 @@ int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
     return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: empty grid");
   if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must be +1 or -1");
   if (d->prec != M3DRT_F32 && d->prec != M3DRT_F64) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: bad prec");
-  if (d->solver != M3DRT_LINEAR2)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: solver must be M3DRT_LINEAR2");
+  if (d->solver != M3DRT_LINEAR2 && d->solver != M3DRT_QUADRATIC && d->solver != M3DRT_LINEAR)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: solver must be M3DRT_LINEAR2, M3DRT_QUADRATIC or M3DRT_LINEAR");
   if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0)) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: ds must be > 0");
   if (d->vec[2] == 0.0) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: horizontal ray (vec(3) = 0)");
   if ((d->vec[2] > 0) != (d->dir > 0))
     return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must follow the sign of vec(3)");
-  if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I || !b->chi500 || !b->g500)
+  if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I)
     return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing input array");
+  if (d->solver != M3DRT_LINEAR2) {
+    if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
+      return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: strides must be >= 1 (see m3drt_native_strides)");
+    if ((d->flags & M3DRT_SPLIT_RATES) && (!b->Irem_out || !b->escape_out))
+      return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
+    M3DRT_HIP(hipSetDevice(ctx->device));
+    if (d->solver == M3DRT_QUADRATIC) return m3drt::sweep_lcq(ctx, d, b);
+    return m3drt::sweep_sc1(ctx, d, b);
+  }
+  if (!b->chi500 || !b->g500) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing chi500 / g500");
   if (d->nx * (int64_t)d->ny > 1 && !b->chi500_up)
     return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: short_char2: chi500 at the upwind points is missing");
   if (b->eta_row && (!b->eta_mask || d->eta_ld < d->nf))
@@ -311,6 +321,43 @@ int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
   if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_bf_rates: strides must be >= 1");
   M3DRT_HIP(hipSetDevice(ctx->device));
   return m3drt::bf_rates(ctx, r);
+}
+
+int m3drt_work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_work_to_transfer: NULL argument");
+  if (r->n < 0) return set_error(M3DRT_EINVAL, "m3drt_work_to_transfer: n < 0");
+  if (r->n == 0) return M3DRT_OK;
+  if (!r->work1 || !r->work2 || !r->beta) return set_error(M3DRT_EINVAL, "m3drt_work_to_transfer: missing array");
+  if ((r->flags & M3DRT_UNITS_SPLIT) && (!r->split_opac1 || !r->split_opac2))
+    return set_error(M3DRT_EINVAL, "m3drt_work_to_transfer: the split form needs split_opac1/2");
+  if ((r->flags & M3DRT_UNITS_SCAT) && (!r->chi_scat || !r->dJnu))
+    return set_error(M3DRT_EINVAL, "m3drt_work_to_transfer: line scattering needs chi_scat and dJnu");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::work_to_transfer(ctx, r);
+}
+
+int m3drt_convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *chi_cont, float *xhat, float *ehat,
+                                const float *beta) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_convert_to_work_units: NULL context");
+  if (n < 0) return set_error(M3DRT_EINVAL, "m3drt_convert_to_work_units: n < 0");
+  if (n == 0) return M3DRT_OK;
+  if (!chi_cont || !xhat || !ehat || !beta)
+    return set_error(M3DRT_EINVAL, "m3drt_convert_to_work_units: missing array");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::convert_to_work_units(ctx, n, chi_cont, xhat, ehat, beta);
+}
+
+int m3drt_update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_update_eta_mask: NULL argument");
+  if (r->nf < 1 || r->nx < 1 || r->ny < 1 || r->nz < 1 || r->rx < 0 || r->ry < 0)
+    return set_error(M3DRT_EINVAL, "m3drt_update_eta_mask: bad sizes");
+  if (r->dir != 1 && r->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_update_eta_mask: dir must be +1 or -1");
+  if (!r->chi || !r->S || !r->beta || !r->src || !r->mask)
+    return set_error(M3DRT_EINVAL, "m3drt_update_eta_mask: missing array");
+  if ((r->flags & M3DRT_ETA_GHOST) && (!r->ghost_chi || !r->ghost_S || !r->ghost_src))
+    return set_error(M3DRT_EINVAL, "m3drt_update_eta_mask: the ghost plane needs ghost_chi, ghost_S, ghost_src");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::update_eta_mask(ctx, r);
 }
 
 }  // extern "C"

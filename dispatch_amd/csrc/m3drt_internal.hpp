@@ -35,6 +35,12 @@ int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opac
 int rate_solve(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res);
 int normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi);
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);
+int sweep_lcq(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
+int sweep_sc1(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
+int update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r);
+int work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r);
+int convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *cont, float *xhat, float *ehat,
+                          const float *beta);
 int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r);
 int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r);
 

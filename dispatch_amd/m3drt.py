@@ -24,7 +24,8 @@ LIB_PATH = os.path.join(_HERE, "libm3drt.so")
 
 M3DRT_OK = 0
 M3DRT_F32, M3DRT_F64 = 0, 1
-M3DRT_LINEAR2 = 0
+M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR = 0, 1, 2
+SOLVERS = dict(linear2=M3DRT_LINEAR2, quadratic=M3DRT_QUADRATIC, linear=M3DRT_LINEAR)
 M3DRT_SPLIT_RATES, M3DRT_GET_RATES = 1, 2
 
 
@@ -115,9 +116,11 @@ class Context:
     def sc_sweep(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, *, dir, ds, vec,
                  layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None, Irem_out=None,
                  escape_out=None, exit_I=None, eta_row=None, eta_mask=None, split_rates=False,
-                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30):
+                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2"):
         """One directional formal solution (short_char2 in 3D, long_char2 when
-        nx = ny = 1) with J += wj*I and PsiJ += wj*Psi fused."""
+        nx = ny = 1; solver="quadratic": long_char_quadratic along every column;
+        solver="linear": short_char, the Multi3D-comparison preset)
+        with J += wj*I and PsiJ += wj*Psi fused."""
         if chi.dtype not in (torch.float32, torch.float64):
             raise M3drtError("chi must be float32 or float64")
         nz = chi.shape[0]
@@ -141,15 +144,17 @@ class Context:
         for t in (chi500, chi500_up):
             if t is not None and (t.dtype != torch.float32 or t.numel() != nx * ny * nz):
                 raise M3drtError("chi500 arrays are float32 of nx*ny*nz cells")
-        if g500.dtype != torch.float32 or g500.numel() != nx * ny:
+        if g500 is not None and (g500.dtype != torch.float32 or g500.numel() != nx * ny):
             raise M3drtError("g500 is float32 of nx*ny cells")
+        if solver not in SOLVERS:
+            raise M3drtError(f"unknown solver {solver!r}")
         if get_rates is None:
             get_rates = PsiJ is not None or Psi_out is not None
         d = SweepDesc()
         d.nf, d.nx, d.ny, d.nz = nf, nx, ny, nz
         d.dir = int(dir)
         d.prec = M3DRT_F64 if chi.dtype == torch.float64 else M3DRT_F32
-        d.solver = M3DRT_LINEAR2
+        d.solver = SOLVERS[solver]
         d.flags = (M3DRT_SPLIT_RATES if split_rates else 0) | (M3DRT_GET_RATES if get_rates else 0)
         d.ds = (C.c_double * 3)(*[float(v) for v in ds])
         d.vec = (C.c_double * 3)(*[float(v) for v in vec])
@@ -485,3 +490,112 @@ def _bf_rates(self, d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0, lay
 Context.bb_rates = _bb_rates
 Context.bb_opacity = _bb_opacity
 Context.bf_rates = _bf_rates
+
+
+# ------------------------------------------------------------------ working units
+
+M3DRT_UNITS_SPLIT, M3DRT_UNITS_SCAT, M3DRT_UNITS_SUPPRESS_MASERS = 1, 2, 4
+
+
+class UnitsArgs(C.Structure):
+    _fields_ = [("n", C.c_int64), ("flags", C.c_int32), ("maser_chi_floor", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("work1", "work2", "beta", "split_opac1", "split_opac2", "chi_scat", "dJnu",
+                                          "n_floored")]
+
+
+def _units_lib():
+    L = lib()
+    if not getattr(L, "_units_ok", False):
+        L.m3drt_work_to_transfer.argtypes = [C.c_void_p, C.POINTER(UnitsArgs)]
+        L.m3drt_work_to_transfer.restype = C.c_int
+        L.m3drt_convert_to_work_units.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 4
+        L.m3drt_convert_to_work_units.restype = C.c_int
+        L._units_ok = True
+    return L
+
+
+def _f32_like(ref, *ts):
+    for t in ts:
+        if t is not None and (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()
+                              or t.numel() != ref.numel()):
+            raise M3drtError("working-unit arrays are contiguous float32 device tensors of one size")
+
+
+def _work_to_transfer(self, work1, work2, beta, split_opac1=None, split_opac2=None, chi_scat=None, dJnu=None,
+                      suppress_masers=False, maser_chi_floor=1e-4, n_floored=None):
+    """work_to_transfer_units in place: work1 xhat -> chi, work2 ehat -> S.
+    n_floored: optional int64 device tensor of one element, incremented."""
+    L = _units_lib()
+    _f32_like(work1, work2, beta, split_opac1, split_opac2, chi_scat, dJnu)
+    if n_floored is not None and (n_floored.dtype != torch.int64 or not n_floored.is_cuda):
+        raise M3drtError("n_floored is an int64 device tensor")
+    flags = ((M3DRT_UNITS_SPLIT if split_opac1 is not None else 0) | (M3DRT_UNITS_SCAT if chi_scat is not None else 0)
+             | (M3DRT_UNITS_SUPPRESS_MASERS if suppress_masers else 0))
+    a = UnitsArgs(work1.numel(), flags, maser_chi_floor, *[_p(t) for t in (work1, work2, beta, split_opac1,
+                                                                           split_opac2, chi_scat, dJnu, n_floored)])
+    self._bind_stream()
+    _check(L.m3drt_work_to_transfer(self._h, C.byref(a)), "m3drt_work_to_transfer")
+
+
+def _convert_to_work_units(self, chi_cont, xhat, ehat, beta):
+    """convert_to_work_units in place: xhat = chi_cont + ehat, ehat /= beta."""
+    L = _units_lib()
+    _f32_like(chi_cont, xhat, ehat, beta)
+    self._bind_stream()
+    _check(L.m3drt_convert_to_work_units(self._h, chi_cont.numel(), _p(chi_cont), _p(xhat), _p(ehat), _p(beta)),
+           "m3drt_convert_to_work_units")
+
+
+Context.work_to_transfer = _work_to_transfer
+Context.convert_to_work_units = _convert_to_work_units
+
+
+# ------------------------------------------------------------------ eta-step flags
+
+M3DRT_ETA_GHOST = 1
+
+
+class EtaArgs(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nf", "nx", "ny", "nz", "dir", "flags", "rx", "ry")] + \
+               [(n, C.c_void_p) for n in ("chi", "S", "beta", "ghost_chi", "ghost_S", "src", "mask", "ghost_src")]
+
+
+def eta_spread(rays, ds, ds_upstream=None, columns=False):
+    """eta_rx, eta_ry of update_eta_mask (SRC/rt_patch_mod.f90:3690-3706): the
+    horizontal reach of the upwind point over the slab's directions, plus one."""
+    if columns:
+        return 0, 0
+    dz = max(ds[2], ds_upstream[2]) if ds_upstream is not None else ds[2]
+    sx = max(abs(v[0] / v[2]) for v in rays)
+    sy = max(abs(v[1] / v[2]) for v in rays)
+    return int(np.floor(sx * dz / ds[0])) + 1, int(np.floor(sy * dz / ds[1])) + 1
+
+
+def _update_eta_mask(self, chi, S, beta, src, mask, *, dir, rx, ry, ghost_chi=None, ghost_S=None,
+                     ghost_src=None):
+    """update_eta_mask on dense flags in the native layout: chi, S, beta float32
+    (nz, nf, ny, nx); src, mask uint8 of that shape; the ghost plane (nf, ny, nx)."""
+    L = lib()
+    if not getattr(L, "_eta_ok", False):
+        L.m3drt_update_eta_mask.argtypes = [C.c_void_p, C.POINTER(EtaArgs)]
+        L.m3drt_update_eta_mask.restype = C.c_int
+        L._eta_ok = True
+    nz, nf, ny, nx = chi.shape
+    for t in (chi, S, beta):
+        if t.dtype != torch.float32 or tuple(t.shape) != tuple(chi.shape) or not t.is_cuda or not t.is_contiguous():
+            raise M3drtError("eta mask: chi, S, beta are float32 (nz, nf, ny, nx) device tensors")
+    for t in (src, mask):
+        if t.dtype != torch.uint8 or tuple(t.shape) != tuple(chi.shape) or not t.is_cuda or not t.is_contiguous():
+            raise M3drtError("eta mask: src, mask are uint8 (nz, nf, ny, nx) device tensors")
+    ghost = ghost_chi is not None
+    if ghost:
+        for t in (ghost_chi, ghost_S, ghost_src):
+            if t is None or t.numel() != nf * ny * nx or not t.is_cuda or not t.is_contiguous():
+                raise M3drtError("eta mask: the ghost plane is (nf, ny, nx) chi, S (float32) and src (uint8)")
+    a = EtaArgs(nf, nx, ny, nz, int(dir), M3DRT_ETA_GHOST if ghost else 0, rx, ry,
+                *[_p(t) for t in (chi, S, beta, ghost_chi, ghost_S, src, mask, ghost_src)])
+    self._bind_stream()
+    _check(L.m3drt_update_eta_mask(self._h, C.byref(a)), "m3drt_update_eta_mask")
+
+
+Context.update_eta_mask = _update_eta_mask

@@ -37,6 +37,10 @@ extern "C" {
 
 /* formal solvers (rt_iteration_params solver=..., SRC/rt_params_mod.f90) */
 #define M3DRT_LINEAR2 0   /* short_char2 in 3D, long_char2 in 1D (the default) */
+#define M3DRT_QUADRATIC 1 /* long_char_quadratic along every column (the output pass, after
+                             rotation for inclined rays; SRC/rt_solvers_mod.f90:891-1020), FP32 */
+#define M3DRT_LINEAR 2    /* short_char, the 'linear' Multi3D-comparison preset: chi, S, I all
+                             from the 4x4 stencil (SRC/rt_solvers_mod.f90:19-189), FP32 */
 
 /* sweep flags */
 #define M3DRT_SPLIT_RATES 1  /* fill Irem and escape ('largest' rate formulation) */
@@ -67,7 +71,7 @@ typedef struct {
   int32_t nf, nx, ny, nz;  /* frequencies, plane size, planes swept */
   int32_t dir;             /* +1: z increasing (up), -1: down (rt_patch_t%dir) */
   int32_t prec;            /* M3DRT_F32 or M3DRT_F64 for every float cube */
-  int32_t solver;          /* M3DRT_LINEAR2 */
+  int32_t solver;          /* M3DRT_LINEAR2, M3DRT_QUADRATIC or M3DRT_LINEAR */
   int32_t flags;           /* M3DRT_SPLIT_RATES | M3DRT_GET_RATES */
   double ds[3];            /* cell size in cm (self%ds * scaling%l) */
   double vec[3];           /* ray direction (rt_iter%vec(:,i_mu)) */
@@ -303,6 +307,53 @@ typedef struct {
 /* bound_free_rates (SRC/rt_patch_mod.f90:3193-3259): photoionisation and
  * recombination sums from J, with the PsiJ preconditioning. */
 int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *args);
+
+/* ------------------------------------------------------------------ working units */
+
+#define M3DRT_UNITS_SPLIT 1            /* eta from the tracked parts split_opac1/2 */
+#define M3DRT_UNITS_SCAT 2             /* eta += chi_scat * dJnu (line scattering) */
+#define M3DRT_UNITS_SUPPRESS_MASERS 4  /* chi = max(chi, floor) instead of the signed floor */
+
+/* n (f, cell) points of one window, every array in the same layout. */
+typedef struct {
+  int64_t n;
+  int32_t flags;
+  double maser_chi_floor;              /* |chi| >= maser_chi_floor * |xhat| (as real(4)) */
+  float *work1, *work2;                /* in: xhat, ehat; out: chi, S = eta/chi */
+  const float *beta;                   /* stimulated-emission factor */
+  const float *split_opac1, *split_opac2;  /* M3DRT_UNITS_SPLIT */
+  const float *chi_scat, *dJnu;        /* M3DRT_UNITS_SCAT */
+  int64_t *n_floored;                  /* device counter or NULL: += points whose chi was floored */
+} m3drt_units_args;
+
+/* work_to_transfer_units (SRC/rt_patch_mod.f90:3571-3651), in place. */
+int m3drt_work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *args);
+
+/* convert_to_work_units (SRC/rt_patch_mod.f90:3518-3536): xhat = chi_cont + ehat,
+ * then ehat = ehat / beta, over n points. */
+int m3drt_convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *chi_cont, float *xhat, float *ehat,
+                                const float *beta);
+
+/* ------------------------------------------------------------------ eta-step flags */
+
+#define M3DRT_ETA_GHOST 1  /* the upstream ghost plane fires too (not the top of the domain) */
+
+/* Dense flags, native layout: cubes (z, f, y, x) x fastest, the ghost plane (f, y, x). */
+typedef struct {
+  int32_t nf, nx, ny, nz, dir, flags;
+  int32_t rx, ry;                       /* eta_rx, eta_ry: the spread of a flag (0, 0 for columns) */
+  const float *chi, *S, *beta;          /* the transfer-unit chi, S and beta of the window */
+  const float *ghost_chi, *ghost_S;     /* upstream ghost plane (M3DRT_ETA_GHOST) */
+  uint8_t *src;                         /* (z, f, y, x) pairs that fired (grows) */
+  uint8_t *mask;                        /* (z, f, y, x) pairs that take the eta step (grows) */
+  uint8_t *ghost_src;                   /* (f, y, x) ghost pairs that fired (grows) */
+} m3drt_eta_args;
+
+/* update_eta_mask (SRC/rt_patch_mod.f90:3671-3802) with dense flags in place of the
+ * reference's rows: the same flag sets.  The sweep takes them as eta_row[cell] =
+ * cell + 1 (one row per cell) and eta_mask = mask viewed per z plane, or any
+ * compaction of them.  Clearing the three arrays is the reference's eta reset. */
+int m3drt_update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *args);
 
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */

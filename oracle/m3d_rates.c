@@ -13,6 +13,8 @@
  *  orc_bb_opacity  add_bound_bound_opacities (:2689-2831) with add_largest (:3150-3169)
  *                  and the direction's share of wphi.
  *  orc_bf_rates    bound_free_rates (:3193-3259).
+ *  orc_work_to_transfer, orc_convert_to_work_units: the (xhat, ehat) <-> (chi, S)
+ *                  conversions (:3518-3536, 3571-3651).
  *
  * Pinning (tests/test_oracle_rates.py): closed form of the normalised thermal
  * integral, detailed balance, the preconditioning invariant of the net rate, the
@@ -20,6 +22,7 @@
  * every (f, cell) sum and the 'largest' split path reducing to the unsplit one
  * where Psi = 0; bf rates by their LTE detailed balance.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include "m3d_qlg.h"
@@ -254,4 +257,45 @@ void orc_bf_rates(const orc_bf_args *a, float *gamma_ij, float *gamma_ji)
     }
   }
   free(wa);
+}
+
+long orc_work_to_transfer(size_t n, float *work1, float *work2, const float *beta, const float *so1,
+                          const float *so2, const float *chi_scat, const float *dJnu, int split,
+                          int scat, int suppress_masers, double maser_chi_floor)
+{
+  long nfl = 0;
+  const float fl = (float)maser_chi_floor;
+  for (size_t i = 0; i < n; i++) {
+    float eta, chi;
+    if (split) {
+      const float eta_rest = beta[i] * so2[i];
+      eta = beta[i] * so1[i] + eta_rest;
+      chi = work1[i] - eta;
+      if (scat) eta = beta[i] * so1[i] + (eta_rest + chi_scat[i] * dJnu[i]);
+    } else {
+      eta = work2[i] * beta[i];
+      chi = work1[i] - eta;
+      if (scat) eta = eta + chi_scat[i] * dJnu[i];
+    }
+    const float xhat = work1[i];
+    float floor = fl * fabsf(xhat);
+    if (floor < FLT_MIN) floor = FLT_MIN;       /* max(., tiny(1.0)) */
+    if (suppress_masers) {
+      if (chi < floor) chi = floor;
+    } else if (fabsf(chi) < floor) {
+      chi = copysignf(floor, chi);
+      nfl++;
+    }
+    work1[i] = chi;
+    work2[i] = eta / chi;
+  }
+  return nfl;
+}
+
+void orc_convert_to_work_units(size_t n, const float *chi_cont, float *xhat, float *ehat, const float *beta)
+{
+  for (size_t i = 0; i < n; i++) {
+    xhat[i] = chi_cont[i] + ehat[i];
+    ehat[i] = ehat[i] / beta[i];
+  }
 }

@@ -77,6 +77,16 @@ typedef struct {
 /* gamma arrays (nline + ncont, ncell); the continua at rows nline.. */
 void orc_bf_rates(const orc_bf_args *a, float *gamma_ij, float *gamma_ji);
 
+/* work_to_transfer_units (SRC/rt_patch_mod.f90:3571-3651) over n (f, cell) points of
+ * any common layout: work1 = xhat -> chi, work2 = ehat -> S.  split: eta from the
+ * tracked parts; scat: chi_scat, dJnu; returns the number of floored points. */
+long orc_work_to_transfer(size_t n, float *work1, float *work2, const float *beta, const float *so1,
+                          const float *so2, const float *chi_scat, const float *dJnu, int split,
+                          int scat, int suppress_masers, double maser_chi_floor);
+
+/* convert_to_work_units (:3518-3536): xhat = chi_cont + ehat, then ehat /= beta */
+void orc_convert_to_work_units(size_t n, const float *chi_cont, float *xhat, float *ehat, const float *beta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -600,3 +600,66 @@ def bf_rates(d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0):
     gji = np.array(gamma_ji, np.float32, copy=True)
     L.orc_bf_rates(C.byref(a), gij.ctypes.data_as(C.c_void_p), gji.ctypes.data_as(C.c_void_p))
     return gij, gji
+
+
+def work_to_transfer(xhat, ehat, beta, split_opac1=None, split_opac2=None, chi_scat=None, dJnu=None,
+                     suppress_masers=False, maser_chi_floor=1e-4):
+    """work_to_transfer_units (SRC/rt_patch_mod.f90:3571-3651) on flat (f, cell)
+    arrays; returns (chi, S, n_floored)."""
+    L = lib()
+    if not getattr(L, "_units_ok", False):
+        vp = C.c_void_p
+        L.orc_work_to_transfer.argtypes = [C.c_size_t, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                           C.c_double]
+        L.orc_work_to_transfer.restype = C.c_long
+        L.orc_convert_to_work_units.argtypes = [C.c_size_t, vp, vp, vp, vp]
+        L.orc_convert_to_work_units.restype = None
+        L._units_ok = True
+    keep = _Keep()
+    w1 = np.array(xhat, np.float32, copy=True)
+    w2 = np.array(ehat, np.float32, copy=True)
+    split = split_opac1 is not None
+    scat = chi_scat is not None
+    nfl = L.orc_work_to_transfer(w1.size, w1.ctypes.data_as(C.c_void_p), w2.ctypes.data_as(C.c_void_p),
+                                 keep.p(beta, np.float32), keep.p(split_opac1, np.float32),
+                                 keep.p(split_opac2, np.float32), keep.p(chi_scat, np.float32),
+                                 keep.p(dJnu, np.float32), int(split), int(scat), int(suppress_masers),
+                                 maser_chi_floor)
+    return w1, w2, int(nfl)
+
+
+def convert_to_work_units(chi_cont, ehat, beta):
+    """convert_to_work_units (SRC/rt_patch_mod.f90:3518-3536): returns (xhat, ehat)."""
+    work_to_transfer(np.zeros(1, np.float32), np.zeros(1, np.float32), np.ones(1, np.float32))  # declares
+    L = lib()
+    keep = _Keep()
+    x = np.zeros(np.shape(chi_cont), np.float32)
+    e = np.array(ehat, np.float32, copy=True)
+    L.orc_convert_to_work_units(x.size, keep.p(chi_cont, np.float32), x.ctypes.data_as(C.c_void_p),
+                                e.ctypes.data_as(C.c_void_p), keep.p(beta, np.float32))
+    return x, e
+
+
+class EtaArgs(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nf", "nx", "ny", "nz", "dir", "rx", "ry", "ghost")] + \
+               [(n, C.c_void_p) for n in ("chi", "S", "beta", "gchi", "gS", "src", "mask", "gsrc")]
+
+
+def update_eta_mask(chi, S, beta, gchi, gS, src, mask, gsrc, dir, rx, ry, ghost=True):
+    """update_eta_mask (SRC/rt_patch_mod.f90:3671-3802) on dense flags: cubes
+    (nz, nf, ny, nx), ghost plane (nf, ny, nx); returns updated (src, mask, gsrc)."""
+    L = lib()
+    if not getattr(L, "_eta_ok", False):
+        L.orc_update_eta_mask.argtypes = [C.POINTER(EtaArgs)]
+        L.orc_update_eta_mask.restype = None
+        L._eta_ok = True
+    keep = _Keep()
+    nz, nf, ny, nx = np.shape(chi)
+    src = np.array(src, np.uint8, copy=True)
+    mask = np.array(mask, np.uint8, copy=True)
+    gsrc = np.array(gsrc, np.uint8, copy=True)
+    a = EtaArgs(nf, nx, ny, nz, dir, rx, ry, int(ghost), keep.p(chi, np.float32), keep.p(S, np.float32),
+                keep.p(beta, np.float32), keep.p(gchi, np.float32), keep.p(gS, np.float32),
+                src.ctypes.data_as(C.c_void_p), mask.ctypes.data_as(C.c_void_p), gsrc.ctypes.data_as(C.c_void_p))
+    L.orc_update_eta_mask(C.byref(a))
+    return src, mask, gsrc

@@ -187,3 +187,70 @@ def test_sweep_exact_strips(ctx, nx, ny, fp64, psij):
         check32(nat(PsiJ), 0.5 + 0.25 * psi.astype(np.float64), "PsiJ", rtol=tol)
     last = case["nz"] if case["dir"] > 0 else 1
     check32(np.moveaxis(exitI.double().cpu().numpy(), 0, 2), I[last], "exit plane", rtol=tol)
+
+
+@pytest.mark.parametrize("layout", ["native", "reference"])
+@pytest.mark.parametrize("vec,nz", [((0.0, 0.0, 0.78867513), 30), ((0.0, 0.0, -0.21132487), 30), ((0.0, 0.0, 1.0), 2)])
+def test_sweep_quadratic_columns(ctx, layout, vec, nz):
+    """long_char_quadratic along every column (M3DRT_QUADRATIC): I, Psi (= Lambda*
+    / chi), J and the exit plane against the oracle's quad_step restatement."""
+    from oracle import oracle as O
+    case = make_case(nf=24, nx=5, ny=4, nz=nz, seed=40 + nz, vec=vec)
+    print(f"quadratic {vec} nz={nz} {layout}", flush=True)
+    I, psi = O.long_char_quadratic(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"])
+    gz = case["gz"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(interior(a)))
+        pla = lambda a: dev(plane_to_native(a[gz]))
+        back = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pback = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        cub = lambda a: dev(interior(a))
+        pla = lambda a: dev(a[gz])
+        back = pback = lambda t: t.double().cpu().numpy()
+    c = cub(case["chi"])
+    J = torch.full_like(c, 0.125)
+    Iout, Psi = torch.zeros_like(c), torch.zeros_like(c)
+    exitI = torch.zeros_like(pla(case["chi"]))
+    ctx.sc_sweep(c, cub(case["S"]), pla(case["chi"]), pla(case["S"]), pla(case["I"]), None, None, None,
+                 dir=case["dir"], ds=case["ds"], vec=case["vec"], layout=layout, wj=0.25, J=J, I_out=Iout,
+                 Psi_out=Psi, exit_I=exitI, solver="quadratic")
+    torch.cuda.synchronize()
+    check32(back(Iout), interior(I), "I quadratic")
+    check32(back(Psi), psi, "Lambda*/chi quadratic")
+    check32(back(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J quadratic")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(pback(exitI), I[last], "exit plane quadratic")
+
+
+@pytest.mark.parametrize("kw", [dict(seed=50, vec=(0.3, -0.4, 0.866)), dict(seed=51, vec=(-0.5, 0.2, -0.84)),
+                                dict(seed=52, vec=(0.9, 0.2, 0.39), nx=17, ny=13, nz=7, split=True)])
+def test_sweep_short_char_linear(ctx, kw):
+    """short_char (M3DRT_LINEAR): chi, S, I from the 4x4 stencil with the inner-min
+    clamp, against the oracle's restatement; split parts when asked."""
+    from oracle import oracle as O
+    kw = dict(kw)
+    split = kw.pop("split", False)
+    case = make_case(nf=kw.pop("nf", 12), nx=kw.pop("nx", 12), ny=kw.pop("ny", 10), nz=kw.pop("nz", 9), **kw)
+    print(f"short_char {kw} split={split}", flush=True)
+    I, psi, irem, esc = O.short_char(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"],
+                                     split_rates=split)
+    gz = case["gz"]
+    cub = lambda a: dev(to_native(interior(a)))
+    pla = lambda a: dev(plane_to_native(a[gz]))
+    back = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+    c = cub(case["chi"])
+    J = torch.full_like(c, 0.125)
+    Iout, Psi = torch.zeros_like(c), torch.zeros_like(c)
+    Irem = torch.zeros_like(c) if split else None
+    Esc = torch.zeros_like(c) if split else None
+    ctx.sc_sweep(c, cub(case["S"]), pla(case["chi"]), pla(case["S"]), pla(case["I"]), None, None, None,
+                 dir=case["dir"], ds=case["ds"], vec=case["vec"], wj=0.25, J=J, I_out=Iout, Psi_out=Psi,
+                 Irem_out=Irem, escape_out=Esc, split_rates=split, solver="linear")
+    torch.cuda.synchronize()
+    check32(back(Iout), interior(I), "I short_char")
+    check32(back(Psi), psi, "Psi short_char")
+    check32(back(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J short_char")
+    if split:
+        check32(back(Irem), irem, "Irem short_char")
+        check32(back(Esc), esc, "escape short_char")
