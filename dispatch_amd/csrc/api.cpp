@@ -336,6 +336,20 @@ int m3drt_work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r) {
   return m3drt::work_to_transfer(ctx, r);
 }
 
+int m3drt_background_emission(m3drt_ctx *ctx, const m3drt_background_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_background_emission: NULL argument");
+  if (r->n < 0) return set_error(M3DRT_EINVAL, "m3drt_background_emission: n < 0");
+  if (r->n == 0) return M3DRT_OK;
+  if (!r->J || !r->B || !r->chi_cont || !r->sigma || !r->eta)
+    return set_error(M3DRT_EINVAL, "m3drt_background_emission: missing array");
+  if ((r->flags & M3DRT_BG_SCAT_ALI) && (!r->PsiJ || !r->Jali_old))
+    return set_error(M3DRT_EINVAL, "m3drt_background_emission: the ALI step needs PsiJ and Jali_old");
+  if ((r->flags & M3DRT_BG_SCAT_ALI) && !(r->scat_op_max < 1.0))
+    return set_error(M3DRT_EINVAL, "m3drt_background_emission: scat_op_max must be < 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::background_emission(ctx, r);
+}
+
 int m3drt_convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *chi_cont, float *xhat, float *ehat,
                                 const float *beta) {
   if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_convert_to_work_units: NULL context");

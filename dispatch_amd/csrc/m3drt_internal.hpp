@@ -39,6 +39,7 @@ int sweep_lcq(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs 
 int sweep_sc1(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r);
 int work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r);
+int background_emission(m3drt_ctx *ctx, const m3drt_background_args *r);
 int convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *cont, float *xhat, float *ehat,
                           const float *beta);
 int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r);

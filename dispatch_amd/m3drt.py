@@ -599,3 +599,32 @@ def _update_eta_mask(self, chi, S, beta, src, mask, *, dir, rx, ry, ghost_chi=No
 
 
 Context.update_eta_mask = _update_eta_mask
+
+
+M3DRT_BG_SCAT_ALI, M3DRT_BG_J_TO_B = 1, 2
+
+
+class BackgroundArgs(C.Structure):
+    _fields_ = [("n", C.c_int64), ("flags", C.c_int32), ("wscat", C.c_float), ("scat_op_max", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("J", "B", "chi_cont", "sigma", "PsiJ", "Jali_old", "Jali", "eta", "dJnu",
+                                          "n_neg", "diag")]
+
+
+def _background_emission(self, J, B, chi_cont, sigma, eta, *, PsiJ=None, Jali_old=None, Jali=None, dJnu=None,
+                         j_to_b=False, wscat=1.0, scat_op_max=0.9, n_neg=None, diag=None):
+    """lte_background_emission in place over a window's flat float32 arrays (J in:
+    J_FS, out: J - B); n_neg int64[1] and diag float64[3] device counters, optional."""
+    L = lib()
+    if not getattr(L, "_bg_ok", False):
+        L.m3drt_background_emission.argtypes = [C.c_void_p, C.POINTER(BackgroundArgs)]
+        L.m3drt_background_emission.restype = C.c_int
+        L._bg_ok = True
+    _f32_like(J, B, chi_cont, sigma, eta, PsiJ, Jali_old, Jali, dJnu)
+    flags = (M3DRT_BG_SCAT_ALI if PsiJ is not None else 0) | (M3DRT_BG_J_TO_B if j_to_b else 0)
+    a = BackgroundArgs(J.numel(), flags, wscat, scat_op_max,
+                       *[_p(t) for t in (J, B, chi_cont, sigma, PsiJ, Jali_old, Jali, eta, dJnu, n_neg, diag)])
+    self._bind_stream()
+    _check(L.m3drt_background_emission(self._h, C.byref(a)), "m3drt_background_emission")
+
+
+Context.background_emission = _background_emission

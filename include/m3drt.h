@@ -329,6 +329,27 @@ typedef struct {
 /* work_to_transfer_units (SRC/rt_patch_mod.f90:3571-3651), in place. */
 int m3drt_work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *args);
 
+#define M3DRT_BG_SCAT_ALI 1  /* the continuum-scattering ALI step (scattering_preconditioning) */
+#define M3DRT_BG_J_TO_B 2    /* initial_Jnu = Planck on the first iteration: J = B */
+
+typedef struct {
+  int64_t n;
+  int32_t flags;
+  float wscat;                         /* 1, or 0 to drop the scattering term */
+  double scat_op_max;                  /* the clamp of op = sigma * PsiJ */
+  float *J;                            /* in: J_FS; out: J - B (J_ali - B with the ALI step) */
+  const float *B, *chi_cont, *sigma;
+  const float *PsiJ, *Jali_old;        /* M3DRT_BG_SCAT_ALI */
+  float *Jali;                         /* M3DRT_BG_SCAT_ALI: the new J_ali, or NULL */
+  float *eta;                          /* out: chi_cont B + wscat sigma (J - B) */
+  float *dJnu;                         /* out: wscat (J - B), or NULL */
+  int64_t *n_neg;                      /* device counter or NULL: ALI values that went negative */
+  double *diag;                        /* device [3] or NULL: max op_raw, op, J_ali/J_FS (max-ed in) */
+} m3drt_background_args;
+
+/* lte_background_emission (SRC/rt_patch_mod.f90:2127-2328), elementwise over a window. */
+int m3drt_background_emission(m3drt_ctx *ctx, const m3drt_background_args *args);
+
 /* convert_to_work_units (SRC/rt_patch_mod.f90:3518-3536): xhat = chi_cont + ehat,
  * then ehat = ehat / beta, over n points. */
 int m3drt_convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *chi_cont, float *xhat, float *ehat,

@@ -299,3 +299,42 @@ void orc_convert_to_work_units(size_t n, const float *chi_cont, float *xhat, flo
     ehat[i] = ehat[i] / beta[i];
   }
 }
+
+long orc_background_emission(size_t n, float *J, const float *B, const float *chi_cont, const float *sigma,
+                             const float *PsiJ, const float *Jali_old, float *Jali, float *eta, float *dJnu,
+                             int scat_ali, int j_to_b, float wscat, double scat_op_max, double *diag)
+{
+  long nneg = 0;
+  double op_raw = 0, op_max = 0, amp_max = 0;
+  if (scat_ali)
+    for (size_t i = 0; i < n; i++) {
+      double op = (double)sigma[i] * (double)PsiJ[i];
+      if (op > op_raw) op_raw = op;
+      op = op < scat_op_max ? op : scat_op_max;
+      op = op > 0.0 ? op : 0.0;
+      if (op > op_max) op_max = op;
+      const double Jfs = J[i];
+      double Jn = (Jfs - op * (double)Jali_old[i]) / (1.0 - op);
+      if (Jn < 0.0) {
+        nneg++;
+        Jn = Jfs;
+      }
+      if (Jfs > 0.0 && Jn / Jfs > amp_max) amp_max = Jn / Jfs;
+      J[i] = (float)Jn;
+      if (Jali) Jali[i] = J[i];
+    }
+  for (size_t i = 0; i < n; i++) {
+    float j = J[i];
+    if (j_to_b) j = B[i];
+    j = j - B[i];
+    J[i] = j;
+    eta[i] = chi_cont[i] * B[i] + wscat * sigma[i] * j;
+    if (dJnu) dJnu[i] = wscat * j;
+  }
+  if (diag) {
+    diag[0] = op_raw;
+    diag[1] = op_max;
+    diag[2] = amp_max;
+  }
+  return nneg;
+}

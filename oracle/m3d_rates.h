@@ -84,6 +84,15 @@ long orc_work_to_transfer(size_t n, float *work1, float *work2, const float *bet
                           const float *so2, const float *chi_scat, const float *dJnu, int split,
                           int scat, int suppress_masers, double maser_chi_floor);
 
+/* lte_background_emission (SRC/rt_patch_mod.f90:2127-2328), elementwise over n points:
+ * with scat_ali the continuum-scattering ALI step J = (J_FS - op J_ali_old)/(1 - op),
+ * op = clamp(sigma PsiJ, 0, scat_op_max), in FP64 (negative -> J_FS), J_ali = J; with
+ * j_to_b J = B first; then J -= B, eta = chi_cont B + wscat sigma J, dJnu = wscat J.
+ * Returns the count of negative ALI values; diag[3] = op_raw max, op max, amp max. */
+long orc_background_emission(size_t n, float *J, const float *B, const float *chi_cont, const float *sigma,
+                             const float *PsiJ, const float *Jali_old, float *Jali, float *eta, float *dJnu,
+                             int scat_ali, int j_to_b, float wscat, double scat_op_max, double *diag);
+
 /* convert_to_work_units (:3518-3536): xhat = chi_cont + ehat, then ehat /= beta */
 void orc_convert_to_work_units(size_t n, const float *chi_cont, float *xhat, float *ehat, const float *beta);
 

@@ -663,3 +663,56 @@ def update_eta_mask(chi, S, beta, gchi, gS, src, mask, gsrc, dir, rx, ry, ghost=
                 src.ctypes.data_as(C.c_void_p), mask.ctypes.data_as(C.c_void_p), gsrc.ctypes.data_as(C.c_void_p))
     L.orc_update_eta_mask(C.byref(a))
     return src, mask, gsrc
+
+
+def background_emission(J, B, chi_cont, sigma, PsiJ=None, Jali_old=None, j_to_b=False, wscat=1.0,
+                        scat_op_max=0.9):
+    """lte_background_emission (SRC/rt_patch_mod.f90:2127-2328) on flat arrays.
+    Returns dict(J, Jali, eta, dJnu, n_neg, op_raw, op_max, amp_max)."""
+    L = lib()
+    if not getattr(L, "_bge_ok", False):
+        vp = C.c_void_p
+        L.orc_background_emission.argtypes = [C.c_size_t] + [vp] * 9 + [C.c_int, C.c_int, C.c_float, C.c_double, vp]
+        L.orc_background_emission.restype = C.c_long
+        L._bge_ok = True
+    keep = _Keep()
+    Jo = np.array(J, np.float32, copy=True)
+    n = Jo.size
+    ali = PsiJ is not None
+    Jali = np.zeros(n, np.float32)
+    eta = np.zeros(n, np.float32)
+    dJ = np.zeros(n, np.float32)
+    diag = np.zeros(3)
+    nneg = L.orc_background_emission(n, Jo.ctypes.data_as(C.c_void_p), keep.p(B, np.float32),
+                                     keep.p(chi_cont, np.float32), keep.p(sigma, np.float32),
+                                     keep.p(PsiJ, np.float32), keep.p(Jali_old, np.float32),
+                                     Jali.ctypes.data_as(C.c_void_p), eta.ctypes.data_as(C.c_void_p),
+                                     dJ.ctypes.data_as(C.c_void_p), int(ali), int(j_to_b), wscat, scat_op_max,
+                                     diag.ctypes.data_as(C.c_void_p))
+    return dict(J=Jo, Jali=Jali, eta=eta, dJnu=dJ, n_neg=int(nneg), op_raw=diag[0], op_max=diag[1], amp_max=diag[2])
+
+
+def short_char_hermite(chi, S, I, beta, pe4, temp, dpe, dlnt, hknu, dir, ds, vec, split_rates=False,
+                       get_rates=True, maser_dtau_max=2.0, maser_Inu_max=1e30):
+    """short_char_hermite (SRC/rt_solvers_mod.f90:1917-2248): work arrays (nz+2, ny, nx,
+    nf) with ghosts; beta (nz, ny, nx, nf); pe4, temp (nz, ny, nx); dpe, dlnt tuples
+    of three (nz, ny, nx) gradients; hknu (nf).  Returns (I, Psi, Irem, escape)."""
+    L = lib()
+    if not getattr(L, "_herm_ok", False):
+        L.orc_short_char_hermite_f32.argtypes = [C.POINTER(ScParams)] + [C.c_void_p] * 16
+        L.orc_short_char_hermite_f32.restype = None
+        L._herm_ok = True
+    keep = _Keep()
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    irem = np.zeros_like(psi) if split_rates else None
+    esc = np.zeros_like(psi) if split_rates else None
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, split_rates, get_rates)
+    f32 = lambda a: keep.p(a, np.float32)
+    L.orc_short_char_hermite_f32(C.byref(p), _p(chi), _p(S), _p(I), f32(beta), f32(pe4), f32(temp),
+                                 f32(dpe[0]), f32(dpe[1]), f32(dpe[2]), f32(dlnt[0]), f32(dlnt[1]), f32(dlnt[2]),
+                                 f32(hknu), _p(psi), _p(irem), _p(esc))
+    return I, psi, irem, esc

@@ -46,3 +46,31 @@ def test_units_match_oracle(ctx, mode):
     assert int(cnt.item()) == nfl
     if mode != "suppress":
         assert nfl > 0
+
+
+@pytest.mark.parametrize("ali", [False, True])
+def test_background_emission_matches_oracle(ctx, ali):
+    from test_oracle_background import fields as bfields
+    rng, B, chi, sigma = bfields(n=200003, seed=9)
+    Jfs = (B * rng.uniform(0.2, 2.0, B.size)).astype(np.float32)
+    kw, gkw = {}, {}
+    if ali:
+        PsiJ = (rng.uniform(0.0, 1.5, B.size) / sigma).astype(np.float32)
+        Jold = (B * rng.uniform(0.5, 3.0, B.size)).astype(np.float32)
+        kw = dict(PsiJ=PsiJ, Jali_old=Jold, scat_op_max=0.8)
+        gkw = dict(PsiJ=d(PsiJ), Jali_old=d(Jold), scat_op_max=0.8)
+    ref = O.background_emission(Jfs, B, chi, sigma, wscat=0.75, **kw)
+    gJ, geta, gdJ, gJali = d(Jfs), torch.zeros_like(d(B)), torch.zeros_like(d(B)), torch.zeros_like(d(B))
+    nneg = torch.zeros(1, dtype=torch.int64, device="cuda")
+    diag = torch.zeros(3, dtype=torch.float64, device="cuda")
+    ctx.background_emission(gJ, d(B), d(chi), d(sigma), geta, dJnu=gdJ, Jali=gJali if ali else None, wscat=0.75,
+                            n_neg=nneg, diag=diag, **gkw)
+    torch.cuda.synchronize()
+    # the FP64 ALI step rounds once to FP32: bit for bit; the FP32 source terms too
+    np.testing.assert_array_equal(gJ.cpu().numpy(), ref["J"])
+    np.testing.assert_array_equal(geta.cpu().numpy(), ref["eta"])
+    np.testing.assert_array_equal(gdJ.cpu().numpy(), ref["dJnu"])
+    if ali:
+        np.testing.assert_array_equal(gJali.cpu().numpy(), ref["Jali"])
+        assert int(nneg.item()) == ref["n_neg"] > 0
+        np.testing.assert_allclose(diag.cpu().numpy(), [ref["op_raw"], ref["op_max"], ref["amp_max"]], rtol=1e-15)
