@@ -108,6 +108,24 @@ class Solver:
         del self.chi, self.S, self.J, self.ghost
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sc2_fast_f32_box120x120x230_nf1024.md")
+
+
+def measured_traffic(box):
+    """HBM bytes per sweep launch measured by the FETCH_SIZE / WRITE_SIZE counter
+    passes (calibrated on a plain copy) recorded in profiles/, for the workload they
+    were taken on; None for any other box."""
+    if (box.nx, box.ny, box.nz, box.nf) != (120, 120, 230, 1024) or not os.path.exists(TRAFFIC_PROFILE):
+        return None, None
+    val = kern = None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            val = float(line.split(":", 1)[1])
+        if line.startswith("kernel:"):
+            kern = line.split(":", 1)[1].strip()
+    return val, kern
+
+
 def timed(solver, steps, warmup):
     torch = solver.torch
     for _ in range(warmup):
@@ -219,11 +237,14 @@ def main():
     avg = sum(ms32) / len(ms32)
     per_launch_bytes = BYTES_PER_UPDATE["f32"] * ncell * box.nf
     achieved = per_launch_bytes / (avg * 1e-3) / 1e9
+    traffic, tkern = measured_traffic(box)
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                       "kernel": "sc2_sweep<float> (+ its per-direction dt500 prep)",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "kernel": "sc2_fast<float> (+ its per-direction dt500 prep)",
                        "avg_launch_ms": avg, "launches": len(ms32),
                        "algorithmic_bytes_per_launch": per_launch_bytes}
+    if traffic is not None:
+        res["roofline"]["traffic_source"] = f"profiles/{os.path.basename(TRAFFIC_PROFILE)} ({tkern})"
 
     if not args.no_fp64:
         em32 = s32.emergent.clone()
