@@ -188,6 +188,27 @@ int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
   return m3drt::sweep_sc2(ctx, d, b);
 }
 
+int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
+                           const m3drt_hermite_fields *h) {
+  if (!ctx || !d || !b || !h) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: NULL argument");
+  if (d->nf < 1 || d->nx < 1 || d->ny < 1 || d->nz < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: empty grid");
+  if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: dir must be +1 or -1");
+  if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0) || d->vec[2] == 0.0 || (d->vec[2] > 0) != (d->dir > 0))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: bad geometry (ds > 0, dir following vec(3))");
+  if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: missing input array");
+  if (!h->beta || !h->pe4 || !h->temp || !h->dpe_dx || !h->dpe_dy || !h->dpe_dz || !h->dlnt_dx || !h->dlnt_dy ||
+      !h->dlnt_dz || !h->hknu)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: missing composition field");
+  if ((d->flags & M3DRT_SPLIT_RATES) && (!b->Irem_out || !b->escape_out))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
+  if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep_hermite: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::sweep_sch(ctx, d, b, h);
+}
+
 int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
   if (!ctx || !ll || !o) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NULL argument");
   if (ll->nline < 0 || o->ncell < 0 || o->nf < 0)

@@ -100,30 +100,42 @@ __global__ void __launch_bounds__(256) sc2_prep_pairs(const float *__restrict__ 
   }
 }
 
+template <typename T, int N>
+struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) Arr {
+  T v[N];
+};
+
 // MODE 2: (I, x) pairs and S in LDS; 1: (I, x) pairs in LDS, upwind S from the
-// cache; 0: I in LDS, upwind x and S from the cache.
-template <typename T, int L, bool PJ, int MODE>
-__global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const FastArgs<T> a) {
+// cache.  CPL: adjacent columns per lane (1, or 2 with the native layout: the
+// cube rows are then read and written 8 bytes per lane and the (dt500, 1/chi500)
+// pairs 16, so a pair of cells costs the vector-memory instructions of one).
+template <int L, bool PJ, int MODE, int CPL>
+__global__ void __launch_bounds__(1024) sc2_fast(const FastArgs<float> a) {
   using namespace dev;
-  constexpr int PD = sizeof(T) == 8 ? 2 : 3;
+  using T = float;
+  using V = Arr<T, CPL>;
+  using DV = Arr<Pair<T>, CPL>;
+  constexpr int PD = CPL == 1 ? 3 : 1;  // two cells' loads per step in flight with CPL = 2
   constexpr uint32_t TB = sizeof(T);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, PW = a.PW, nxy = nx * ny;
   const int np = PW * (ny + 4);
   Pair<T> *sP = reinterpret_cast<Pair<T> *>(smem_raw);
-  T *sI = reinterpret_cast<T *>(smem_raw);
   T *sS = reinterpret_cast<T *>(smem_raw + (size_t)np * sizeof(Pair<T>));
 
   const int f = blockIdx.x;
   const int tid = threadIdx.x;
-  const int x = tid % nx;
-  const int strip = tid / nx;
+  const int nxp = nx / CPL;
+  const int x0 = (tid % nxp) * CPL;
+  const int strip = tid / nxp;
   const int yb = strip * L;
-  const int rb = (yb + a.yo + 2) * PW + (x + a.xo + 2);  // row 0 of the window, padded index
-  const int xc1 = wrap(x + a.xo + 1, nx), xc2 = wrap(x + a.xo + 2, nx);
+  const int rb = (yb + a.yo + 2) * PW + (x0 + a.xo + 2);  // row 0 of the window, padded index
+  int xc[CPL + 1];                                        // upwind columns of S (wrapped)
+#pragma unroll
+  for (int c = 0; c <= CPL; ++c) xc[c] = wrap(x0 + a.xo + 1 + c, nx);
   const int g0 = wrap(yb + a.yo, ny);                     // row 0 of the window, unpadded
-  const uint32_t voff = (x * a.cs_x + yb * a.cs_y) * TB;  // first cell of the strip in a cube plane
-  const uint32_t doff = (yb * nx + x) * (uint32_t)sizeof(Pair<T>);
+  const uint32_t voff = (x0 * a.cs_x + yb * a.cs_y) * TB; // first cell of the strip in a cube plane
+  const uint32_t doff = (yb * nx + x0) * (uint32_t)sizeof(Pair<T>);
   const T ohx = (T)1 - a.hx, ohy = (T)1 - a.hy;
 
   const T *__restrict__ gchi = a.gchi + f * a.gs_f;
@@ -132,8 +144,8 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
   const Pair<T> *__restrict__ gdr = a.dr + (int64_t)a.nz * nxy;
 
   struct In {
-    T chi, S, J, PsJ;
-    Pair<T> dr;
+    V chi, S, J, PsJ;
+    DV dr;
   };
   // descriptors of one plane of every cube
   struct Planes {
@@ -153,11 +165,11 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
   const uint32_t rowb = a.cs_y * TB, drowb = nx * (uint32_t)sizeof(Pair<T>);
   auto fetch = [&](const Planes &P, int k) -> In {
     In v;
-    v.chi = bld<T>(P.chi, voff, k * rowb);
-    v.S = bld<T>(P.S, voff, k * rowb);
-    v.J = bld<T>(P.J, voff, k * rowb);
-    v.PsJ = PJ ? bld<T>(P.PsJ, voff, k * rowb) : (T)0;
-    v.dr = bld<Pair<T>>(P.dr, doff, k * drowb);
+    v.chi = bld<V>(P.chi, voff, k * rowb);
+    v.S = bld<V>(P.S, voff, k * rowb);
+    v.J = bld<V>(P.J, voff, k * rowb);
+    if (PJ) v.PsJ = bld<V>(P.PsJ, voff, k * rowb);
+    v.dr = bld<DV>(P.dr, doff, k * drowb);
     return v;
   };
 
@@ -172,15 +184,10 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
     const int pr = c / PW, pc = c - pr * PW;
     const int cy = wrap(pr - 2, ny), cx = wrap(pc - 2, nx);
     const uint32_t go = (cx * a.gs_x + cy * a.gs_y) * TB;
-    const T iv = ldo(gI, go);
-    if (MODE >= 1) {
-      Pair<T> p;
-      p.a = iv;
-      p.b = ldo(gchi, go) * gdr[cy * nx + cx].b;
-      sP[c] = p;
-    } else {
-      sI[c] = iv;
-    }
+    Pair<T> p;
+    p.a = ldo(gI, go);
+    p.b = ldo(gchi, go) * gdr[cy * nx + cx].b;
+    sP[c] = p;
     if (MODE == 2) sS[c] = ldo(gS, go);
   }
   lds_barrier();
@@ -190,92 +197,94 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
     const Planes P = planes(z);
     int e0 = rb;  // opaque per plane: cheaper to re-derive the row addresses than to hold them
     asm volatile("" : "+v"(e0));
-    // the upwind plane in global memory, for the values not held in LDS
+    // the upwind plane in global memory, for S when it is not held in LDS
     const bool up_ghost = step == 0;
     const int zu = z - a.dir;
     const uint32_t ub = up_ghost ? a.gplane_bytes : a.plane_bytes;
-    const Rsrc up_chi = rsrc(up_ghost ? gchi : a.chi + (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f, ub);
     const Rsrc up_S = rsrc(up_ghost ? gS : a.S + (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f, ub);
-    const Rsrc up_dr = rsrc(up_ghost ? gdr : a.dr + (int64_t)zu * nxy, (uint32_t)(nxy * sizeof(Pair<T>)));
     const uint32_t uy = up_ghost ? a.gs_y : a.cs_y;
     const uint32_t ux = up_ghost ? a.gs_x : a.cs_x;
-    const uint32_t ux1 = xc1 * ux * TB, ux2 = xc2 * ux * TB;
 
-    struct RawRow { T i0, i1, i2, i3, x1, x2, s1, s2; };
+    // one upwind row for the CPL cells: I at CPL+3 columns, x and S at CPL+1
+    struct RawRow { T i[CPL + 3], x[CPL + 1], s[CPL + 1]; };
     auto read_row = [&](int r) -> RawRow {
       RawRow q;
       const int e = e0 + r * PW;
-      if (MODE >= 1) {
-        const Pair<T> p0 = sP[e], p1 = sP[e + 1], p2 = sP[e + 2], p3 = sP[e + 3];
-        q.i0 = p0.a; q.i1 = p1.a; q.i2 = p2.a; q.i3 = p3.a;
-        q.x1 = p1.b; q.x2 = p2.b;
-      } else {
-        q.i0 = sI[e]; q.i1 = sI[e + 1]; q.i2 = sI[e + 2]; q.i3 = sI[e + 3];
+#pragma unroll
+      for (int j = 0; j < CPL + 3; ++j) {
+        const Pair<T> p = sP[e + j];
+        q.i[j] = p.a;
+        if (j >= 1 && j <= CPL + 1) q.x[j - 1] = p.b;
       }
       if (MODE == 2) {
-        q.s1 = sS[e + 1];
-        q.s2 = sS[e + 2];
+#pragma unroll
+        for (int j = 0; j <= CPL; ++j) q.s[j] = sS[e + 1 + j];
       } else {
         int g = g0 + r;
         g = g >= ny ? g - ny : g;
         const uint32_t ro = g * uy * TB;
-        q.s1 = bld<T>(up_S, ro + ux1, 0);
-        q.s2 = bld<T>(up_S, ro + ux2, 0);
-        if (MODE == 0) {
-          const uint32_t dro = g * nx * (uint32_t)sizeof(Pair<T>);
-          q.x1 = bld<T>(up_chi, ro + ux1, 0) * bld<Pair<T>>(up_dr, dro + xc1 * (uint32_t)sizeof(Pair<T>), 0).b;
-          q.x2 = bld<T>(up_chi, ro + ux2, 0) * bld<Pair<T>>(up_dr, dro + xc2 * (uint32_t)sizeof(Pair<T>), 0).b;
-        }
+#pragma unroll
+        for (int j = 0; j <= CPL; ++j) q.s[j] = bld<T>(up_S, ro + xc[j] * ux * TB, 0);
       }
       return q;
     };
     struct Row { T rI, rmin, rX, rS; };
-    auto reduce_row = [&](const RawRow &q) -> Row {
-      Row w;
-      w.rI = a.xk0 * q.i0 + a.xk1 * q.i1 + a.xk2 * q.i2 + a.xk3 * q.i3;
-      w.rmin = fmin(q.i1, q.i2);
-      w.rX = ohx * q.x1 + a.hx * q.x2;
-      w.rS = ohx * q.s1 + a.hx * q.s2;
+    struct Rows { Row c[CPL]; };
+    auto reduce_row = [&](const RawRow &q) -> Rows {
+      Rows w;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        w.c[c].rI = a.xk0 * q.i[c] + a.xk1 * q.i[c + 1] + a.xk2 * q.i[c + 2] + a.xk3 * q.i[c + 3];
+        w.c[c].rmin = fmin(q.i[c + 1], q.i[c + 2]);
+        w.c[c].rX = ohx * q.x[c] + a.hx * q.x[c + 1];
+        w.c[c].rS = ohx * q.s[c] + a.hx * q.s[c + 1];
+      }
       return w;
     };
 
-    T newI[L], newX[MODE >= 1 ? L : 1], newS[MODE == 2 ? L : 1];
-    Row w0 = reduce_row(read_row(0));
-    Row w1 = reduce_row(read_row(1));
-    Row w2 = reduce_row(read_row(2));
+    T newI[CPL][L], newX[CPL][L], newS[CPL][MODE == 2 ? L : 1];
+    Rows w0 = reduce_row(read_row(0));
+    Rows w1 = reduce_row(read_row(1));
+    Rows w2 = reduce_row(read_row(2));
     RawRow next = read_row(3);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       __builtin_amdgcn_sched_barrier(0);
       const In in = k < PD ? pre[k] : ring[k % PD];
       if (k + PD < L) ring[k % PD] = fetch(P, k + PD);
-      const Row w3 = reduce_row(next);
+      const Rows w3 = reduce_row(next);
       if (k + 1 < L) next = read_row(k + 4);
-      // upwind values, SRC/rt_solvers_mod.f90:379-394
-      const T xxup = ohy * w1.rX + a.hy * w2.rX;
-      const T ssup = ohy * w1.rS + a.hy * w2.rS;
-      T Inuup = a.yk0 * w0.rI + a.yk1 * w1.rI + a.yk2 * w2.rI + a.yk3 * w3.rI;
-      Inuup = fmax(Inuup, fmin(w1.rmin, w2.rmin));
-      const T chiv = in.chi, ss = in.S, dt500 = in.dr.a, rc = in.dr.b;
-      const T xx = chiv * rc;
-      T dt = (T)0.5 * (xxup + xx) * dt500;
-      dt = fmax(dt, a.mdtau);
-      T E, W, Ps, Es;
-      lin_weights(dt, E, W, Ps, Es);
-      // the S step of a cell without eta flags (SRC/rt_solvers_mod.f90:419-440, 465-475)
-      const T Ir = Inuup * E + W * ssup;
-      T Inu = Ir + Ps * ss;
-      Inu = fmax(-a.imax, fmin(a.imax, Inu));
-      newI[k] = Inu;
-      if (MODE >= 1) newX[k] = xx;
-      if (MODE == 2) newS[k] = ss;
-      bst<T>(P.J, voff, k * rowb, in.J + a.wj * Inu);
-      if (PJ) {
-        const T sgnchi = copysign(fmax(fabs(chiv), (T)1.17549435e-38f), chiv);
-        const bool off = fmin(dt, chiv) <= (T)0;
-        const T psiout = off ? (T)0 : Ps * rcp(sgnchi);
-        bst<T>(P.PsJ, voff, k * rowb, in.PsJ + a.wj * psiout);
+      V Jn, PJn;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        // upwind values, SRC/rt_solvers_mod.f90:379-394
+        const T xxup = ohy * w1.c[c].rX + a.hy * w2.c[c].rX;
+        const T ssup = ohy * w1.c[c].rS + a.hy * w2.c[c].rS;
+        T Inuup = a.yk0 * w0.c[c].rI + a.yk1 * w1.c[c].rI + a.yk2 * w2.c[c].rI + a.yk3 * w3.c[c].rI;
+        Inuup = fmax(Inuup, fmin(w1.c[c].rmin, w2.c[c].rmin));
+        const T chiv = in.chi.v[c], ss = in.S.v[c], dt500 = in.dr.v[c].a, rc = in.dr.v[c].b;
+        const T xx = chiv * rc;
+        T dt = (T)0.5 * (xxup + xx) * dt500;
+        dt = fmax(dt, a.mdtau);
+        T E, W, Ps, Es;
+        lin_weights(dt, E, W, Ps, Es);
+        // the S step of a cell without eta flags (SRC/rt_solvers_mod.f90:419-440, 465-475)
+        const T Ir = Inuup * E + W * ssup;
+        T Inu = Ir + Ps * ss;
+        Inu = fmax(-a.imax, fmin(a.imax, Inu));
+        newI[c][k] = Inu;
+        if (MODE >= 1) newX[c][k] = xx;
+        if (MODE == 2) newS[c][k] = ss;
+        Jn.v[c] = in.J.v[c] + a.wj * Inu;
+        if (PJ) {
+          const T sgnchi = copysign(fmax(fabs(chiv), (T)1.17549435e-38f), chiv);
+          const bool off = fmin(dt, chiv) <= (T)0;
+          const T psiout = off ? (T)0 : Ps * rcp(sgnchi);
+          PJn.v[c] = in.PsJ.v[c] + a.wj * psiout;
+        }
       }
+      bst<V>(P.J, voff, k * rowb, Jn);
+      if (PJ) bst<V>(P.PsJ, voff, k * rowb, PJn);
       w0 = w1;
       w1 = w2;
       w2 = w3;
@@ -289,22 +298,26 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
     lds_barrier();  // every read of the upwind plane is done
     // the new plane, then its halo copies: columns nx-2, nx-1 to the left pad,
     // 0, 1 to the right; rows ny-2, ny-1 to the top pad, 0, 1 to the bottom.  The
-    // base index is opaque per plane so that the 2L+4 write addresses are formed
-    // here, not held in registers across the whole sweep.
-    int wb = (yb + 2) * PW + x + 2;
+    // base index is opaque per plane so that the write addresses are formed here,
+    // not held in registers across the whole sweep.
+    int wb = (yb + 2) * PW + x0 + 2;
     asm volatile("" : "+v"(wb));
-    const bool xedge = x < 2 || x >= nx - 2;
-    const int dh = x < 2 ? nx : -nx;  // to this column's halo copy
+    const bool xedge = x0 < 2 || x0 + CPL - 1 >= nx - 2;
+    const int dh = x0 < 2 ? nx : -nx;  // to this group's halo copy
     auto put = [&](int e, int k) {
-      if (MODE >= 1) {
-        Pair<T> p;
-        p.a = newI[k];
-        p.b = newX[k];
-        sP[e] = p;
-      } else {
-        sI[e] = newI[k];
+      Arr<Pair<T>, CPL> pp;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        pp.v[c].a = newI[c][k];
+        pp.v[c].b = newX[c][k];
       }
-      if (MODE == 2) sS[e] = newS[k];
+      *reinterpret_cast<Arr<Pair<T>, CPL> *>(sP + e) = pp;
+      if (MODE == 2) {
+        V sv;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) sv.v[c] = newS[c][k];
+        *reinterpret_cast<V *>(sS + e) = sv;
+      }
     };
 #pragma unroll
     for (int k = 0; k < L; ++k) put(wb + k * PW, k);
@@ -328,44 +341,62 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_fast(const Fa
     }
     if (a.exitI && step == a.nz - 1) {
       const Rsrc ex = rsrc(a.exitI + f * a.gs_f, a.gplane_bytes);
-      uint32_t eo = (x * a.gs_x + yb * a.gs_y) * TB;
+      uint32_t eo = (x0 * a.gs_x + yb * a.gs_y) * TB;
       asm volatile("" : "+v"(eo));
 #pragma unroll
-      for (int k = 0; k < L; ++k) bst<T>(ex, eo, k * a.gs_y * TB, newI[k]);
+      for (int k = 0; k < L; ++k) {
+        if (CPL == 1) {
+          bst<T>(ex, eo, k * a.gs_y * TB, newI[0][k]);
+        } else {
+          V ev;
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) ev.v[c] = newI[c][k];
+          bst<V>(ex, eo, k * a.gs_y * TB, ev);
+        }
+      }
     }
     lds_barrier();
   }
 }
 
-template <typename T, int L, bool PJ, int MODE>
-int launch_fast(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, size_t lds) {
-  auto kern = sc2_fast<T, L, PJ, MODE>;
+template <int L, bool PJ, int MODE, int CPL>
+int launch_fast(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, size_t lds) {
+  auto kern = sc2_fast<L, PJ, MODE, CPL>;
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(nf), dim3(nt), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
   return M3DRT_OK;
 }
 
-template <typename T, int L, bool PJ>
-int fast_mode(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, int mode, size_t np) {
-  if (mode == 2) return launch_fast<T, L, PJ, 2>(ctx, a, nf, nt, np * 3 * sizeof(T));
-  if (mode == 1) return launch_fast<T, L, PJ, 1>(ctx, a, nf, nt, np * 2 * sizeof(T));
-  return -1;
+template <int L, int CPL>
+int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int mode, size_t np) {
+  const size_t l2 = np * 2 * sizeof(float), l3 = np * 3 * sizeof(float);
+  if (a.PsiJ) {
+    if constexpr (CPL == 1) {
+      if (mode == 2) return launch_fast<L, true, 2, CPL>(ctx, a, nf, nt, l3);
+      return launch_fast<L, true, 1, CPL>(ctx, a, nf, nt, l2);
+    }
+    return -1;
+  }
+  if (mode == 2) return launch_fast<L, false, 2, CPL>(ctx, a, nf, nt, l3);
+  return launch_fast<L, false, 1, CPL>(ctx, a, nf, nt, l2);
 }
 
-template <typename T, int L>
-int fast_pj(m3drt_ctx *ctx, const FastArgs<T> &a, int nf, int nt, int mode, size_t np) {
-  if (a.PsiJ) return fast_mode<T, L, true>(ctx, a, nf, nt, mode, np);
-  return fast_mode<T, L, false>(ctx, a, nf, nt, mode, np);
-}
+// strip lengths instantiated: one lane per column (60, 96, 120-cell planes), two
+// adjacent columns per lane (native layout, even nx)
+constexpr int kL1[] = {4, 12, 15};
+constexpr int kL2[] = {4, 6, 8};
 
-// strip lengths instantiated (60, 96, 120-cell planes)
-constexpr int kL32[] = {4, 12, 15};
-
-int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int mode, size_t np) {
-  if (L == 4) return fast_pj<float, 4>(ctx, a, nf, nt, mode, np);
-  if (L == 12) return fast_pj<float, 12>(ctx, a, nf, nt, mode, np);
-  if (L == 15) return fast_pj<float, 15>(ctx, a, nf, nt, mode, np);
+int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int cpl, int mode, size_t np) {
+  if (cpl == 2) {
+    if (L == 4) return fast_variant<4, 2>(ctx, a, nf, nt, mode, np);
+    if (L == 6) return fast_variant<6, 2>(ctx, a, nf, nt, mode, np);
+    if (L == 8) return fast_variant<8, 2>(ctx, a, nf, nt, mode, np);
+    return -1;
+  }
+  if (L == 4) return fast_variant<4, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 12) return fast_variant<12, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 15) return fast_variant<15, 1>(ctx, a, nf, nt, mode, np);
   return -1;
 }
 
@@ -376,21 +407,27 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   const bool get_rates = (d->flags & M3DRT_GET_RATES) != 0;
   if (b->eta_row || b->I_out || (d->flags & M3DRT_SPLIT_RATES) || !b->J || (get_rates && b->Psi_out))
     return -1;
-  if (nx < 4 || ny < 4 || getenv("M3DRT_NO_FAST")) return -1;
-  const int tmax = sizeof(T) == 8 ? 512 : 1024;
-  if (nx > tmax) return -1;
-  const int smax = tmax / nx;  // strips that fit the workgroup
+  if (nx < 4 || ny < 4 || nx > 1024 || getenv("M3DRT_NO_FAST")) return -1;
+  // two columns per lane where the rows are contiguous and nx is even (J only: with
+  // PsiJ the pair form needs more registers than a lane has)
+  const char *cenv = getenv("M3DRT_FAST_CPL");
+  int cpl = (d->cs_x == 1 && d->gs_x == 1 && nx % 2 == 0 && !(get_rates && b->PsiJ)) ? 2 : 1;
+  if (cenv && atoi(cenv) == 1) cpl = 1;
   int L = 0;
-  for (int c : kL32)
-    if (!L && ny % c == 0 && ny / c <= smax && ny / c >= 2) L = c;
+  for (int pass = 0; pass < 2 && !L; ++pass) {
+    const int nxp = nx / cpl;
+    const int smax = 1024 / nxp;  // strips that fit the workgroup
+    for (int c : (cpl == 2 ? kL2 : kL1))
+      if (!L && ny % c == 0 && ny / c <= smax && ny / c >= 2) L = c;
+    if (!L && cpl == 2) cpl = 1;
+  }
   if (!L) return -1;
   const int nstrips = ny / L;
-  const int nt = nx * nstrips;
+  const int nt = (nx / cpl) * nstrips;
   const size_t np = (size_t)(nx + 4) * (ny + 4);
   int mode = -1;
   if (np * 3 * sizeof(T) <= (size_t)ctx->lds_max) mode = 2;
   else if (np * 2 * sizeof(T) <= (size_t)ctx->lds_max) mode = 1;
-  else if (sizeof(T) == 8 && np * sizeof(T) <= (size_t)ctx->lds_max) mode = 0;
   if (mode < 0) return -1;
   const uint64_t span = (uint64_t)(nx - 1) * d->cs_x + (uint64_t)(ny - 1) * d->cs_y;
   const uint64_t gspan = (uint64_t)(nx - 1) * d->gs_x + (uint64_t)(ny - 1) * d->gs_y;
@@ -437,10 +474,10 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   a.imax = f64 ? (T)d->maser_Inu_max : (T)(float)d->maser_Inu_max;
   a.wj = (T)d->wj;
   if (getenv("M3DRT_VERBOSE"))
-    fprintf(stderr, "[m3drt] sc2 fast %s nf=%d plane=%dx%d nz=%d: %d threads, strip %d, LDS mode %d (%zu B)%s\n",
-            f64 ? "f64" : "f32", d->nf, nx, ny, nz, nt, L, mode,
-            np * (mode == 2 ? 3 : mode == 1 ? 2 : 1) * sizeof(T), a.PsiJ ? ", PsiJ" : "");
-  return fast_dispatch(ctx, a, d->nf, nt, L, mode, np);
+    fprintf(stderr, "[m3drt] sc2 fast %s nf=%d plane=%dx%d nz=%d: %d threads, %d column(s) per lane, strip %d, "
+            "LDS mode %d (%zu B)%s\n", f64 ? "f64" : "f32", d->nf, nx, ny, nz, nt, cpl, L, mode,
+            np * (mode == 2 ? 3 : 2) * sizeof(T), a.PsiJ ? ", PsiJ" : "");
+  return fast_dispatch(ctx, a, d->nf, nt, L, cpl, mode, np);
 }
 
 }  // namespace

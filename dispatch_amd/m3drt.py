@@ -5,9 +5,11 @@ through the Fortran module (dispatch_amd/fortran/m3drt_mod.f90): ctypes over the
 C ABI, with torch tensors providing device memory and the current HIP stream.
 There is no CPU fallback: a missing library raises, and so does a failing call.
 
-Layouts (element strides are passed to the kernels, both layouts work):
+Layouts (element strides are passed to the kernels, every layout works):
   native     tensor shape (nz, nf, ny, nx)   x fastest, then y, f, z  (fast path)
   reference  tensor shape (nz, ny, nx, nf)   work(f,x,y,z), f fastest
+  fmajor     tensor shape (nf, nz, ny, nx)   a frequency's cube contiguous: the (f, cell)
+                                             spectra of the opacity and rate kernels
 Per-cell arrays (chi500 ...) are (nz, ny, nx); planes (ghost, exit) are
 (nf, ny, nx) native or (ny, nx, nf) reference.
 """
@@ -47,6 +49,11 @@ class SweepBufs(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "chi", "S", "ghost_chi", "ghost_S", "ghost_I", "chi500", "chi500_up", "g500",
         "eta_row", "eta_mask", "J", "PsiJ", "I_out", "Psi_out", "Irem_out", "escape_out", "exit_I")]
+
+
+class HermiteFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("beta", "pe4", "temp", "dpe_dx", "dpe_dy", "dpe_dz", "dlnt_dx", "dlnt_dy",
+                                          "dlnt_dz", "hknu")]
 
 
 _lib = None
@@ -116,10 +123,11 @@ class Context:
     def sc_sweep(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, *, dir, ds, vec,
                  layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None, Irem_out=None,
                  escape_out=None, exit_I=None, eta_row=None, eta_mask=None, split_rates=False,
-                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2"):
+                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2", hermite=None):
         """One directional formal solution (short_char2 in 3D, long_char2 when
         nx = ny = 1; solver="quadratic": long_char_quadratic along every column;
-        solver="linear": short_char, the Multi3D-comparison preset)
+        solver="linear": short_char, the Multi3D-comparison preset; hermite=dict of
+        the composition fields of m3drt_hermite_fields: short_char_hermite)
         with J += wj*I and PsiJ += wj*Psi fused."""
         if chi.dtype not in (torch.float32, torch.float64):
             raise M3drtError("chi must be float32 or float64")
@@ -128,6 +136,8 @@ class Context:
             _, nf, ny, nx = chi.shape
         elif layout == "reference":
             _, ny, nx, nf = chi.shape
+        elif layout == "fmajor":
+            nf, nz, ny, nx = chi.shape
         else:
             raise M3drtError(f"unknown layout {layout!r}")
         cubes = [t for t in (chi, S, J, PsiJ, I_out, Psi_out, Irem_out, escape_out) if t is not None]
@@ -167,13 +177,34 @@ class Context:
             d.eta_nrow = eta_mask.shape[0]
         if eta_row is not None and (eta_row.dtype != torch.int32 or eta_row.numel() != nx * ny * nz):
             raise M3drtError("eta_row is int32 of nx*ny*nz cells")
-        strides = lib().m3drt_native_strides if layout == "native" else lib().m3drt_reference_strides
-        _check(strides(C.byref(d)), "strides")
+        if layout == "fmajor":
+            # (nf, nz, ny, nx): each frequency's cube contiguous, cells flat as in the
+            # (f, cell) spectra of the opacity and rate kernels; planes as native
+            _check(lib().m3drt_native_strides(C.byref(d)), "strides")
+            d.cs_f, d.cs_z = nz * ny * nx, ny * nx
+        else:
+            strides = lib().m3drt_native_strides if layout == "native" else lib().m3drt_reference_strides
+            _check(strides(C.byref(d)), "strides")
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         b = SweepBufs(p(chi), p(S), p(ghost_chi), p(ghost_S), p(ghost_I), p(chi500), p(chi500_up), p(g500),
                       p(eta_row), p(eta_mask), p(J), p(PsiJ), p(I_out), p(Psi_out), p(Irem_out),
                       p(escape_out), p(exit_I))
         self._bind_stream()
+        if hermite is not None:
+            L = lib()
+            if not getattr(L, "_herm_ok", False):
+                L.m3drt_sc_sweep_hermite.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(SweepBufs),
+                                                     C.POINTER(HermiteFields)]
+                L.m3drt_sc_sweep_hermite.restype = C.c_int
+                L._herm_ok = True
+            for k, t in hermite.items():
+                if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+                    raise M3drtError(f"hermite field {k} must be a contiguous float32 device tensor")
+            if tuple(hermite["beta"].shape) != tuple(chi.shape):
+                raise M3drtError("hermite beta is a cube shaped like chi")
+            h = HermiteFields(*[p(hermite[k]) for k, _ in HermiteFields._fields_])
+            _check(L.m3drt_sc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h)), "m3drt_sc_sweep_hermite")
+            return
         _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")
 
 

@@ -213,6 +213,24 @@ int m3drt_rate_solve_f64(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, m3dr
  * n*_i/n*_j.  Uses trans_*, nline, nrad, ncell, nstar, gamma_ij/ji of args. */
 int m3drt_normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *args, const float *wphi);
 
+/* ------------------------------------------------------------------ quadratic 3D sweep */
+
+/* The composition fields short_char_hermite reads besides chi, S and I. */
+typedef struct {
+  const float *beta;                        /* cube (the sweep's layout): stimulated-emission factor */
+  const float *pe4, *temp;                  /* (nz, ny, nx) electron pressure, temperature */
+  const float *dpe_dx, *dpe_dy, *dpe_dz;    /* (nz, ny, nx) gradients of Pe [per cm] */
+  const float *dlnt_dx, *dlnt_dy, *dlnt_dz; /* (nz, ny, nx) gradients of ln T [per cm] */
+  const float *hknu;                        /* (nf) h nu / k */
+} m3drt_hermite_fields;
+
+/* short_char_hermite (SRC/rt_solvers_mod.f90:1917-2248) + integrate_intensity: the 3D
+ * short-characteristics sweep with quadratic (parabolic-moment) weights and the
+ * product-rule slopes of chi (through Pe) and S (through B).  FP32 (FP64 step length
+ * and weights), as the reference.  desc->solver is ignored. */
+int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs,
+                           const m3drt_hermite_fields *fields);
+
 /* ------------------------------------------------------------------ line rates */
 
 #define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T */

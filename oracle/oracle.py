@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -46,7 +47,9 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        # make rebuilds the library whenever a source is newer than it, so a
+        # stale build is never the checker
+        if not os.path.exists(_LIB) or shutil.which("make"):
             build()
         _lib = C.CDLL(_LIB)
         _declare(_lib)

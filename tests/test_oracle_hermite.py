@@ -58,8 +58,11 @@ def test_constant_source_fixed():
     np.testing.assert_allclose(I[1:-1], np.broadcast_to(Bv, I[1:-1].shape), rtol=3e-6)
 
 
-def test_first_plane_is_short_chars_step():
-    case = make_case(nf=5, nx=10, ny=9, nz=6, seed=5, vec=(-0.3, 0.7, -0.64))
+@pytest.mark.parametrize("vec", [(-0.3, 0.7, -0.64), (0.3, -0.4, 0.866), (-0.61, 0.2, 0.5), (0.45, 0.35, -0.82)])
+def test_first_plane_is_short_chars_step(vec):
+    """Both sweep directions and all four signs of the upwind offsets: the stencil
+    tables of the quadratic solver are short_char's."""
+    case = make_case(nf=5, nx=10, ny=9, nz=6, seed=5, vec=vec)
     h = hermite_inputs(case, seed=1)
     I, psi, _, _ = run(case, h)
     Is, psis, _, _ = O.short_char(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"])

@@ -160,33 +160,47 @@ def test_sweep_rejects_bad_arguments(ctx):
                      dir=1, ds=case["ds"], vec=case["vec"])
 
 
-@pytest.mark.parametrize("nx,ny,fp64,psij", [(60, 60, False, False), (60, 60, False, True), (120, 120, False, False),
-                                              (96, 96, False, True), (60, 60, True, False), (120, 120, True, True)])
-def test_sweep_exact_strips(ctx, nx, ny, fp64, psij):
-    """The straight-line kernel form (only J, or J and PsiJ, asked for; the strip
-    length divides ny): J, PsiJ and the exit plane against the oracle."""
+@pytest.mark.parametrize("nx,ny,fp64,psij,layout", [(60, 60, False, False, "native"), (60, 60, False, True, "native"),
+                                                     (120, 120, False, False, "native"),
+                                                     (96, 96, False, False, "native"), (96, 96, False, True, "native"),
+                                                     (60, 60, False, False, "reference"),
+                                                     (120, 120, False, False, "reference"),
+                                                     (60, 60, True, False, "native"), (120, 120, True, True, "native")])
+def test_sweep_exact_strips(ctx, nx, ny, fp64, psij, layout):
+    """The straight-line kernel forms (only J, or J and PsiJ, asked for; the strip
+    length divides ny): two columns per lane (native layout, J only), one column
+    per lane (PsiJ, or the reference layout), the general kernel's exact strips
+    (FP64): J, PsiJ and the exit plane against the oracle."""
     case = make_case(nf=3, nx=nx, ny=ny, nz=4, seed=nx + ny + int(fp64), vec=(0.61481017, 0.0, 0.78867513))
-    print(f"exact strips {nx}x{ny} fp64={fp64} psij={psij}", flush=True)
+    print(f"exact strips {nx}x{ny} fp64={fp64} psij={psij} {layout}", flush=True)
     I, psi, _, _ = run_oracle(case, fp64=fp64)
     dt = torch.float64 if fp64 else torch.float32
     gz = case["gz"]
-    cub = lambda a: dev(to_native(interior(a)), dt)
-    pla = lambda a: dev(plane_to_native(a[gz]), dt)
+    if layout == "native":
+        cub = lambda a: dev(to_native(interior(a)), dt)
+        pla = lambda a: dev(plane_to_native(a[gz]), dt)
+    else:
+        cub = lambda a: dev(interior(a), dt)
+        pla = lambda a: dev(a[gz], dt)
     c, s = cub(case["chi"]), cub(case["S"])
     J = torch.full_like(c, 0.125)
     PsiJ = torch.full_like(c, 0.5) if psij else None
     exitI = torch.zeros_like(pla(case["chi"]))
     ctx.sc_sweep(c, s, pla(case["chi"]), pla(case["S"]), pla(case["I"]), dev(case["chi500"]),
                  dev(case["chi500_up"]), dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"],
-                 wj=0.25, J=J, PsiJ=PsiJ, exit_I=exitI, get_rates=psij)
+                 wj=0.25, J=J, PsiJ=PsiJ, exit_I=exitI, get_rates=psij, layout=layout)
     torch.cuda.synchronize()
-    nat = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+    if layout == "native":
+        nat = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pnat = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        nat = pnat = lambda t: t.double().cpu().numpy()
     tol = 1e-11 if fp64 else RTOL32
     check32(nat(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J", rtol=tol)
     if psij:
         check32(nat(PsiJ), 0.5 + 0.25 * psi.astype(np.float64), "PsiJ", rtol=tol)
     last = case["nz"] if case["dir"] > 0 else 1
-    check32(np.moveaxis(exitI.double().cpu().numpy(), 0, 2), I[last], "exit plane", rtol=tol)
+    check32(pnat(exitI), I[last], "exit plane", rtol=tol)
 
 
 @pytest.mark.parametrize("layout", ["native", "reference"])
@@ -254,3 +268,42 @@ def test_sweep_short_char_linear(ctx, kw):
     if split:
         check32(back(Irem), irem, "Irem short_char")
         check32(back(Esc), esc, "escape short_char")
+
+
+@pytest.mark.parametrize("kw", [dict(seed=60, vec=(0.3, -0.4, 0.866)), dict(seed=61, vec=(-0.61, 0.2, -0.5)),
+                                dict(seed=62, vec=(0.9, 0.2, 0.39), nx=17, ny=13, nz=7, split=True)])
+def test_sweep_hermite(ctx, kw):
+    """short_char_hermite (m3drt_sc_sweep_hermite) against the oracle's restatement:
+    the quadratic step from the second plane on, the linear step on the first."""
+    from oracle import oracle as O
+    from hermite_cases import hermite_inputs
+    kw = dict(kw)
+    split = kw.pop("split", False)
+    case = make_case(nf=kw.pop("nf", 10), nx=kw.pop("nx", 12), ny=kw.pop("ny", 10), nz=kw.pop("nz", 9), **kw)
+    h = hermite_inputs(case, seed=case["nz"])
+    print(f"hermite {kw} split={split}", flush=True)
+    I, psi, irem, esc = O.short_char_hermite(case["chi"], case["S"], case["I"], h["beta"], h["pe4"], h["temp"],
+                                             h["dpe"], h["dlnt"], h["hknu"], case["dir"], case["ds"], case["vec"],
+                                             split_rates=split)
+    gz = case["gz"]
+    cub = lambda a: dev(to_native(interior(a)))
+    pla = lambda a: dev(plane_to_native(a[gz]))
+    back = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+    c = cub(case["chi"])
+    J = torch.full_like(c, 0.125)
+    Iout, Psi = torch.zeros_like(c), torch.zeros_like(c)
+    Irem = torch.zeros_like(c) if split else None
+    Esc = torch.zeros_like(c) if split else None
+    fields = dict(beta=dev(to_native(h["beta"])), pe4=dev(h["pe4"]), temp=dev(h["temp"]),
+                  dpe_dx=dev(h["dpe"][0]), dpe_dy=dev(h["dpe"][1]), dpe_dz=dev(h["dpe"][2]),
+                  dlnt_dx=dev(h["dlnt"][0]), dlnt_dy=dev(h["dlnt"][1]), dlnt_dz=dev(h["dlnt"][2]), hknu=dev(h["hknu"]))
+    ctx.sc_sweep(c, cub(case["S"]), pla(case["chi"]), pla(case["S"]), pla(case["I"]), None, None, None,
+                 dir=case["dir"], ds=case["ds"], vec=case["vec"], wj=0.25, J=J, I_out=Iout, Psi_out=Psi,
+                 Irem_out=Irem, escape_out=Esc, split_rates=split, get_rates=True, hermite=fields)
+    torch.cuda.synchronize()
+    check32(back(Iout), interior(I), "I hermite")
+    check32(back(Psi), psi, "Psi hermite")
+    check32(back(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J hermite")
+    if split:
+        check32(back(Irem), irem, "Irem hermite")
+        check32(back(Esc), esc, "escape hermite")
