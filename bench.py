@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--calib-copy", action="store_true",
                    help="run a plain 4 GiB device copy first (HBM-counter calibration)")
+    p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--adjoint-fd", action="store_true",
+                   help="check the adjoint's abundance gradients against FP64 central differences")
     p.add_argument("--nf", type=int, default=1024)
     p.add_argument("--nz", type=int, default=230)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -196,6 +199,57 @@ def cpu_baseline(box, chi32, S32, rays, weights, cu, seconds):
                        f"FP32 + integrate_intensity, OpenMP over {4 * ncores} frequency windows")
 
 
+def adjoint_leg(ctx, args):
+    """Adjoint / forward time on the abundance-fit workload (96 x 96 x 160, 24 rays,
+    2048 frequencies, 12 lines of 4 elements): the forward is the formal solution as
+    configured (24 sweeps, J and the emergent intensities); the adjoint is the
+    gradient of the flux chi^2 with respect to chi and S (the 12 upward rays, each
+    recomputed by the forward sweep and swept back), then to the 4 log abundances."""
+    import torch
+    from dispatch_amd.fit import FitProblem
+    t0 = time.time()
+    torch.cuda.reset_peak_memory_stats()
+    p = FitProblem(ctx)
+    p.make_target()
+    log(f"[bench] fit problem ready in {time.time() - t0:.1f}s")
+    d0 = [0.0, 0.0, 0.0, 0.0]
+    chi, S = p.cubes(d0)
+    fwd = []
+    for i in range(1 + max(1, args.steps // 2)):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        with torch.no_grad():
+            p.forward_full(chi, S)
+        torch.cuda.synchronize()
+        if i:
+            fwd.append(time.perf_counter() - a)
+    del chi, S
+    tm = {}
+    for i in range(2):
+        loss, g, _ = p.loss_and_grad(d0, timing=tm)
+    t_fwd = min(fwd)
+    out = {"workload": "abundance_fit", "box": [p.box.nx, p.box.ny, p.box.nz], "rays": len(p.vec),
+           "frequencies": p.box.nf, "lines": 12, "elements": 4,
+           "forward_ms": t_fwd * 1e3, "adjoint_sweeps_ms": tm["adjoint_s"] * 1e3,
+           "abundance_chain_ms": tm["chain_s"] * 1e3,
+           "adjoint_over_forward": tm["adjoint_s"] / t_fwd,
+           "gradient_over_forward": (tm["adjoint_s"] + tm["chain_s"]) / t_fwd,
+           "peak_hbm_GB": torch.cuda.max_memory_allocated() / 1e9,
+           "loss": loss, "grad_log_abundance": [float(v) for v in g]}
+    log(f"[bench] adjoint {tm['adjoint_s'] * 1e3:.1f} ms vs forward {t_fwd * 1e3:.1f} ms")
+    if args.adjoint_fd:
+        fd = []
+        for e in range(4):
+            dp, dm = list(d0), list(d0)
+            dp[e] += 1e-3
+            dm[e] -= 1e-3
+            fd.append((float(p.loss_of_flux(p.flux_fp64(dp))) - float(p.loss_of_flux(p.flux_fp64(dm)))) / 2e-3)
+        out["fd_fp64_h1e-3"] = fd
+        out["fd_max_rel_err"] = max(abs(float(a) - b) / abs(b) for a, b in zip(g, fd))
+    del p
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -267,6 +321,12 @@ def main():
         s64.free()
         torch.cuda.empty_cache()
         chi_src = None
+    if not args.no_adjoint:
+        if args.no_fp64:
+            s32.free()
+            torch.cuda.empty_cache()
+        res["adjoint"] = adjoint_leg(ctx, args)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         chi32, S32 = synth.build_cubes(box, torch.float32)
         res["cpu_baseline"] = cpu_baseline(box, chi32, S32, rays, weights, cu, args.cpu_seconds)

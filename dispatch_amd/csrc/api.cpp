@@ -209,6 +209,26 @@ int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3dr
   return m3drt::sweep_sch(ctx, d, b, h);
 }
 
+int m3drt_sc_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint_bufs *b) {
+  if (!ctx || !d || !b) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: NULL argument");
+  if (d->nf < 1 || d->nx < 1 || d->ny < 1 || d->nz < 1) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: empty grid");
+  if (d->nx == 1 && d->ny == 1) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: 3D planes only (nx*ny > 1)");
+  if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: dir must be +1 or -1");
+  if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0) || d->vec[2] == 0.0 || (d->vec[2] > 0) != (d->dir > 0))
+    return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: bad geometry (ds > 0, dir following vec(3))");
+  if (d->prec != M3DRT_F32 || d->solver != M3DRT_LINEAR2)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: FP32 short_char2 only");
+  if (d->flags & M3DRT_SPLIT_RATES) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: no split rates");
+  if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I || !b->chi500 || !b->chi500_up || !b->g500 ||
+      !b->I)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: missing input array");
+  if (!b->chibar || !b->Sbar) return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: missing chibar or Sbar");
+  if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_sc_adjoint: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::sweep_adjoint(ctx, d, b);
+}
+
 int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
   if (!ctx || !ll || !o) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NULL argument");
   if (ll->nline < 0 || o->ncell < 0 || o->nf < 0)

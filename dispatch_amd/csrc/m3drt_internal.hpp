@@ -38,6 +38,7 @@ int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);
 int sweep_lcq(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_sc1(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_sch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const m3drt_hermite_fields *h);
+int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint_bufs *b);
 int update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r);
 int work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r);
 int background_emission(m3drt_ctx *ctx, const m3drt_background_args *r);

@@ -79,6 +79,22 @@ __global__ void __launch_bounds__(256) sc2_prep(const float *__restrict__ chi500
   }
 }
 
+int sc2_prep_f32(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const float *chi500, const float *chi500_up,
+                 const float *g500, float **dt500, float **rc) {
+  const int64_t nxy = (int64_t)d->nx * d->ny;
+  const size_t nprep = (size_t)nxy * (d->nz + 1);
+  const int st = ensure_scratch(ctx, sizeof(float) * 2 * nprep);
+  if (st) return st;
+  *dt500 = static_cast<float *>(ctx->scratch);
+  *rc = *dt500 + nprep;
+  const double dsup = d->ds[2] / fabs(d->vec[2]);
+  const int blocks = (int)std::min<int64_t>(((int64_t)nprep + 255) / 256, 65536);
+  hipLaunchKernelGGL(sc2_prep<float>, dim3(blocks), dim3(256), 0, ctx->stream, chi500, chi500_up, g500, *dt500,
+                     *rc, (int)nxy, d->nz, dsup);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
+
 // Global inputs of one cell: opacity, source function, the cell's dt500 and
 // 1/chi500, and the J / PsiJ it accumulates into.
 template <typename T>

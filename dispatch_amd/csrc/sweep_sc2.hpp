@@ -34,6 +34,11 @@ inline Sc2Geom sc2_geometry(const m3drt_sweep_desc *d) {
   return g;
 }
 
+// dt500 of every cell's step and 1/chi500 (FP32, then the ghost plane's 1/g500) in
+// the context's scratch: the per-direction preparation of the short_char2 sweeps.
+int sc2_prep_f32(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const float *chi500, const float *chi500_up,
+                 const float *g500, float **dt500, float **rc);
+
 // The straight-line sweep for J (and PsiJ) only; returns -1 when the case is not
 // one it takes (the caller then runs the general kernel), else a status.
 int sweep_sc2_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);

@@ -51,6 +51,12 @@ class SweepBufs(C.Structure):
         "eta_row", "eta_mask", "J", "PsiJ", "I_out", "Psi_out", "Irem_out", "escape_out", "exit_I")]
 
 
+class AdjointBufs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("chi", "S", "ghost_chi", "ghost_S", "ghost_I", "chi500", "chi500_up",
+                                          "g500", "I", "Jbar", "Ibar", "exit_bar", "chibar", "Sbar",
+                                          "ghost_chibar", "ghost_Sbar", "ghost_Ibar")]
+
+
 class HermiteFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("beta", "pe4", "temp", "dpe_dx", "dpe_dy", "dpe_dz", "dlnt_dx", "dlnt_dy",
                                           "dlnt_dz", "hknu")]
@@ -94,6 +100,56 @@ def _check(status: int, what: str):
         raise M3drtError(f"{what} failed ({status}): {last_error()}")
 
 
+def _sweep_desc(chi, layout, cubes, planes, chi500, chi500_up, g500, dir, ds, vec, wj, maser_dtau_max,
+                maser_Inu_max):
+    """The descriptor of one direction's sweep over the cubes shaped like chi in the
+    given layout, after checking every array's device, shape and dtype."""
+    if chi.dtype not in (torch.float32, torch.float64):
+        raise M3drtError("chi must be float32 or float64")
+    nz = chi.shape[0]
+    if layout == "native":
+        _, nf, ny, nx = chi.shape
+    elif layout == "reference":
+        _, ny, nx, nf = chi.shape
+    elif layout == "fmajor":
+        nf, nz, ny, nx = chi.shape
+    else:
+        raise M3drtError(f"unknown layout {layout!r}")
+    cubes = [t for t in cubes if t is not None]
+    planes = [t for t in planes if t is not None]
+    for t in cubes + planes + [chi500, chi500_up, g500]:
+        if t is not None and (not t.is_cuda or not t.is_contiguous()):
+            raise M3drtError("every array must be a contiguous device tensor")
+    for t in cubes:
+        if t.dtype != chi.dtype or tuple(t.shape) != tuple(chi.shape):
+            raise M3drtError("cubes must share chi's dtype and shape")
+    for t in planes:
+        if t.dtype != chi.dtype or t.numel() != nf * nx * ny:
+            raise M3drtError("planes must be (nf, ny, nx) / (ny, nx, nf) of chi's dtype")
+    for t in (chi500, chi500_up):
+        if t is not None and (t.dtype != torch.float32 or t.numel() != nx * ny * nz):
+            raise M3drtError("chi500 arrays are float32 of nx*ny*nz cells")
+    if g500 is not None and (g500.dtype != torch.float32 or g500.numel() != nx * ny):
+        raise M3drtError("g500 is float32 of nx*ny cells")
+    d = SweepDesc()
+    d.nf, d.nx, d.ny, d.nz = nf, nx, ny, nz
+    d.dir = int(dir)
+    d.prec = M3DRT_F64 if chi.dtype == torch.float64 else M3DRT_F32
+    d.ds = (C.c_double * 3)(*[float(v) for v in ds])
+    d.vec = (C.c_double * 3)(*[float(v) for v in vec])
+    d.maser_dtau_max, d.maser_Inu_max, d.wj = maser_dtau_max, maser_Inu_max, float(wj)
+    d.eta_ld = 0
+    if layout == "fmajor":
+        # (nf, nz, ny, nx): each frequency's cube contiguous, cells flat as in the
+        # (f, cell) spectra of the opacity and rate kernels; planes as native
+        _check(lib().m3drt_native_strides(C.byref(d)), "strides")
+        d.cs_f, d.cs_z = nz * ny * nx, ny * nx
+    else:
+        strides = lib().m3drt_native_strides if layout == "native" else lib().m3drt_reference_strides
+        _check(strides(C.byref(d)), "strides")
+    return d, (nf, nz, ny, nx)
+
+
 class Context:
     """One m3drt context on a device; calls run on torch's current HIP stream."""
 
@@ -129,47 +185,15 @@ class Context:
         solver="linear": short_char, the Multi3D-comparison preset; hermite=dict of
         the composition fields of m3drt_hermite_fields: short_char_hermite)
         with J += wj*I and PsiJ += wj*Psi fused."""
-        if chi.dtype not in (torch.float32, torch.float64):
-            raise M3drtError("chi must be float32 or float64")
-        nz = chi.shape[0]
-        if layout == "native":
-            _, nf, ny, nx = chi.shape
-        elif layout == "reference":
-            _, ny, nx, nf = chi.shape
-        elif layout == "fmajor":
-            nf, nz, ny, nx = chi.shape
-        else:
-            raise M3drtError(f"unknown layout {layout!r}")
-        cubes = [t for t in (chi, S, J, PsiJ, I_out, Psi_out, Irem_out, escape_out) if t is not None]
-        planes = [t for t in (ghost_chi, ghost_S, ghost_I, exit_I) if t is not None]
-        for t in cubes + planes + [chi500, chi500_up, g500]:
-            if t is not None and (not t.is_cuda or not t.is_contiguous()):
-                raise M3drtError("every array must be a contiguous device tensor")
-        for t in cubes:
-            if t.dtype != chi.dtype or tuple(t.shape) != tuple(chi.shape):
-                raise M3drtError("cubes must share chi's dtype and shape")
-        for t in planes:
-            if t.dtype != chi.dtype or t.numel() != nf * nx * ny:
-                raise M3drtError("planes must be (nf, ny, nx) / (ny, nx, nf) of chi's dtype")
-        for t in (chi500, chi500_up):
-            if t is not None and (t.dtype != torch.float32 or t.numel() != nx * ny * nz):
-                raise M3drtError("chi500 arrays are float32 of nx*ny*nz cells")
-        if g500 is not None and (g500.dtype != torch.float32 or g500.numel() != nx * ny):
-            raise M3drtError("g500 is float32 of nx*ny cells")
         if solver not in SOLVERS:
             raise M3drtError(f"unknown solver {solver!r}")
         if get_rates is None:
             get_rates = PsiJ is not None or Psi_out is not None
-        d = SweepDesc()
-        d.nf, d.nx, d.ny, d.nz = nf, nx, ny, nz
-        d.dir = int(dir)
-        d.prec = M3DRT_F64 if chi.dtype == torch.float64 else M3DRT_F32
+        d, (nf, nz, ny, nx) = _sweep_desc(chi, layout, [chi, S, J, PsiJ, I_out, Psi_out, Irem_out, escape_out],
+                                          [ghost_chi, ghost_S, ghost_I, exit_I], chi500, chi500_up, g500,
+                                          dir, ds, vec, wj, maser_dtau_max, maser_Inu_max)
         d.solver = SOLVERS[solver]
         d.flags = (M3DRT_SPLIT_RATES if split_rates else 0) | (M3DRT_GET_RATES if get_rates else 0)
-        d.ds = (C.c_double * 3)(*[float(v) for v in ds])
-        d.vec = (C.c_double * 3)(*[float(v) for v in vec])
-        d.maser_dtau_max, d.maser_Inu_max, d.wj = maser_dtau_max, maser_Inu_max, float(wj)
-        d.eta_ld = 0
         if eta_mask is not None:
             if eta_mask.dtype != torch.uint8 or eta_mask.dim() != 2 or eta_mask.shape[1] < nf:
                 raise M3drtError("eta_mask is uint8 of shape (nrow, eta_ld >= nf)")
@@ -177,14 +201,6 @@ class Context:
             d.eta_nrow = eta_mask.shape[0]
         if eta_row is not None and (eta_row.dtype != torch.int32 or eta_row.numel() != nx * ny * nz):
             raise M3drtError("eta_row is int32 of nx*ny*nz cells")
-        if layout == "fmajor":
-            # (nf, nz, ny, nx): each frequency's cube contiguous, cells flat as in the
-            # (f, cell) spectra of the opacity and rate kernels; planes as native
-            _check(lib().m3drt_native_strides(C.byref(d)), "strides")
-            d.cs_f, d.cs_z = nz * ny * nx, ny * nx
-        else:
-            strides = lib().m3drt_native_strides if layout == "native" else lib().m3drt_reference_strides
-            _check(strides(C.byref(d)), "strides")
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         b = SweepBufs(p(chi), p(S), p(ghost_chi), p(ghost_S), p(ghost_I), p(chi500), p(chi500_up), p(g500),
                       p(eta_row), p(eta_mask), p(J), p(PsiJ), p(I_out), p(Psi_out), p(Irem_out),
@@ -206,6 +222,30 @@ class Context:
             _check(L.m3drt_sc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h)), "m3drt_sc_sweep_hermite")
             return
         _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")
+
+    def sc_adjoint(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, I, *, dir, ds, vec,
+                   chibar, Sbar, layout="native", wj=0.0, Jbar=None, Ibar=None, exit_bar=None,
+                   ghost_chibar=None, ghost_Sbar=None, ghost_Ibar=None, maser_dtau_max=2.0, maser_Inu_max=1e30):
+        """Reverse of one direction's short_char2 sweep (m3drt_sc_adjoint): from the
+        cotangents of J (the forward added wj*I), of the intensity cube and of the
+        exit plane, accumulates (+=) those of chi and S and of the upstream ghost
+        plane.  I is the forward's intensity cube (its I_out); chi500 is held fixed."""
+        d, _ = _sweep_desc(chi, layout, [chi, S, I, Jbar, Ibar, chibar, Sbar],
+                           [ghost_chi, ghost_S, ghost_I, exit_bar, ghost_chibar, ghost_Sbar, ghost_Ibar],
+                           chi500, chi500_up, g500, dir, ds, vec, wj, maser_dtau_max, maser_Inu_max)
+        d.solver = SOLVERS["linear2"]
+        d.flags = 0
+        p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        b = AdjointBufs(p(chi), p(S), p(ghost_chi), p(ghost_S), p(ghost_I), p(chi500), p(chi500_up), p(g500),
+                        p(I), p(Jbar), p(Ibar), p(exit_bar), p(chibar), p(Sbar), p(ghost_chibar), p(ghost_Sbar),
+                        p(ghost_Ibar))
+        self._bind_stream()
+        L = lib()
+        if not getattr(L, "_adj_ok", False):
+            L.m3drt_sc_adjoint.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(AdjointBufs)]
+            L.m3drt_sc_adjoint.restype = C.c_int
+            L._adj_ok = True
+        _check(L.m3drt_sc_adjoint(self._h, C.byref(d), C.byref(b)), "m3drt_sc_adjoint")
 
 
 # ------------------------------------------------------------------ line opacity

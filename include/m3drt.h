@@ -115,6 +115,32 @@ typedef struct {
  */
 int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs);
 
+/* ------------------------------------------------------------------ adjoint sweep */
+
+/* Buffers of one direction's adjoint (reverse) sweep.  Layouts and strides as the
+ * forward's (m3drt_sweep_desc).  Optional pointers may be NULL. */
+typedef struct {
+  const float *chi, *S;                  /* the forward's inputs */
+  const float *ghost_chi, *ghost_S, *ghost_I;
+  const float *chi500, *chi500_up, *g500;
+  const float *I;                        /* the forward's intensity cube (its I_out) */
+  const float *Jbar;                     /* cotangent of J (the forward added wj*I to J) */
+  const float *Ibar;                     /* cotangent of the intensity cube itself */
+  const float *exit_bar;                 /* (nf,nx,ny) cotangent of exit_I */
+  float *chibar, *Sbar;                  /* accumulated (+=): cotangents of chi and S */
+  float *ghost_chibar, *ghost_Sbar, *ghost_Ibar;  /* accumulated: of the upstream ghost plane */
+} m3drt_adjoint_bufs;
+
+/*
+ * The reverse-mode derivative of m3drt_sc_sweep's short_char2 step (solver
+ * M3DRT_LINEAR2, FP32, no eta flags, no split rates): given the cotangents of J,
+ * of the intensity cube and of the exit plane, accumulates those of chi and S (and
+ * of the upstream ghost plane), chi500 held fixed.  The reference has no adjoint;
+ * this serves the abundance and temperature fits of the Python front end.  Limits
+ * (M3DRT_ELIMIT): 13 bytes per plane cell must fit the LDS (nx*ny <= 12600).
+ */
+int m3drt_sc_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_adjoint_bufs *bufs);
+
 /* ------------------------------------------------------------------ line opacity */
 
 /* flags of m3drt_opacity_args */

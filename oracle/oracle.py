@@ -719,3 +719,39 @@ def short_char_hermite(chi, S, I, beta, pe4, temp, dpe, dlnt, hknu, dir, ds, vec
                                  f32(dpe[0]), f32(dpe[1]), f32(dpe[2]), f32(dlnt[0]), f32(dlnt[1]), f32(dlnt[2]),
                                  f32(hknu), _p(psi), _p(irem), _p(esc))
     return I, psi, irem, esc
+
+
+def short_char2_adjoint(chi, S, I, chi500, chi500_up, g500, Ibar, dir, ds, vec, maser_dtau_max=2.0,
+                        maser_Inu_max=1e30, chibar=None, Sbar=None):
+    """Reverse of short_char2's S step in FP64 (oracle/m3d_adjoint.c): work arrays
+    (nz+2, ny, nx, nf) with ghosts; I the forward solution of short_char2(fp64=True);
+    Ibar the direct cotangents of I (planes 1..nz).  Returns (Ibar total incl. the
+    upstream ghost plane, chibar, Sbar), the last two accumulated onto the given ones."""
+    L = lib()
+    if not getattr(L, "_adj_ok", False):
+        L.orc_short_char2_adjoint_f64.argtypes = [C.POINTER(ScParams)] + [C.c_void_p] * 9
+        L.orc_short_char2_adjoint_f64.restype = None
+        L.orc_lin_weights_deriv_f64.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 6
+        L.orc_lin_weights_deriv_f64.restype = None
+        L._adj_ok = True
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    chi, S, I = f64(chi), f64(S), f64(I)
+    nz2, ny, nx, nf = chi.shape
+    Ib = np.array(Ibar, np.float64, copy=True, order="C")
+    cb = np.zeros_like(chi) if chibar is None else np.array(chibar, np.float64, copy=True, order="C")
+    sb = np.zeros_like(chi) if Sbar is None else np.array(Sbar, np.float64, copy=True, order="C")
+    c5, cu, g5 = f64(chi500), f64(chi500_up), f64(g500)
+    p = _params(nf, nx, ny, nz2 - 2, dir, ds, vec, maser_dtau_max, maser_Inu_max, False, False)
+    L.orc_short_char2_adjoint_f64(C.byref(p), _p(chi), _p(S), _p(I), _p(c5), _p(cu), _p(g5), _p(Ib), _p(cb),
+                                  _p(sb))
+    return Ib, cb, sb
+
+
+def lin_weights_deriv(t):
+    """(E, W, Psi, dE/dt, dW/dt, dPsi/dt) of the exact linear step at t (FP64)."""
+    L = lib()
+    L.orc_lin_weights_deriv_f64.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 6
+    L.orc_lin_weights_deriv_f64.restype = None
+    out = [C.c_double() for _ in range(6)]
+    L.orc_lin_weights_deriv_f64(float(t), *[C.byref(o) for o in out])
+    return tuple(o.value for o in out)

@@ -45,7 +45,7 @@ MIRRORS = {  # ctypes mirror in dispatch_amd/m3drt.py -> header typedef
     "RateSolveResult": "m3drt_rate_solve_result", "BbRatesArgs": "m3drt_bb_rates_args",
     "BbOpacityArgs": "m3drt_bb_opacity_args", "BfRatesArgs": "m3drt_bf_rates_args",
     "UnitsArgs": "m3drt_units_args", "EtaArgs": "m3drt_eta_args", "BackgroundArgs": "m3drt_background_args",
-    "HermiteFields": "m3drt_hermite_fields",
+    "HermiteFields": "m3drt_hermite_fields", "AdjointBufs": "m3drt_adjoint_bufs",
 }
 
 
