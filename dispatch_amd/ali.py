@@ -1,0 +1,298 @@
+"""3D NLTE ALI iterations on one GPU: the per-iteration pipeline of OpenMULTI3D's
+rt_patch_t%execute and nlte_patch (SRC/rt_patch_mod.f90:960-1274,
+SRC/nlte_patch_mod.f90:561-838) for one model atom, in the reference's default
+formulation (rate_formulation='largest' with preconditioning: split rates).
+
+Per iteration, per frequency window of lines, per direction:
+  1. work units: the LTE continuum (chi_c, eta_c = chi_c B) converted
+     (m3drt_convert_to_work_units, SRC/rt_patch_mod.f90:3518-3536), then every line
+     of the window added with the 'largest' tracking (m3drt_bb_opacity:
+     add_bound_bound_opacities + add_largest, :2689-2831, 3150-3169);
+  2. work_to_transfer_units (m3drt_work_to_transfer, :3571-3651) -> chi, S;
+  3. the formal solution of the direction (m3drt_sc_sweep, short_char2) with the
+     intensity, Psi, Irem and escape of every (f, cell);
+  4. the line-rate integrals of the direction (m3drt_bb_rates, bound_bound_rates
+     :2852-3142, Kahan sums).
+Then the continuum rates from the continuum windows' J (m3drt_bf_rates,
+:3193-3259), normalize_profiles (m3drt_normalize_rates) and the FP64
+statistical-equilibrium update of every cell (m3drt_rate_solve_f64: rate matrices,
+batched LU without pivoting, limiter, max |dn/n|).
+
+Spectra are (f, cell) with the cells flat in (z, y, x) order; the sweep takes the
+same memory as (nf, nz, ny, nx) ("fmajor"), so the opacity, transfer and rate
+kernels share one layout with no transposes.  A window's arrays are allocated once
+and reused over directions, windows and iterations.
+
+The workload: a 60 x 60 x 120 box from the 3D LTE generator, the synthetic
+21-level atom (20 bound levels, continuum; 80 lines, 20 continua), 120 frequencies
+per line, van Regemorter collisions, 16 rays (2 Gauss mu x 4 azimuths, mirrored);
+iterate to max |dn/n| < 1e-4 or 40 iterations.  The line-of-sight velocity is the
+generator's vertical velocity projected on each ray.  The continuum windows carry
+the background alone (the atom's own bound-free opacity is not added to the
+transfer), so their J is computed once.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import m3drt, synth
+
+H_KT = synth.H_PLANCK / synth.K_BOLTZ
+TWO_H_C2 = 2.0 * synth.H_PLANCK / synth.C_LIGHT ** 2
+PI_E2_MC = 0.0265400   # pi e^2 / (m_e c) [cm^2 Hz]
+
+
+def _trapezoid(x):
+    w = np.zeros_like(x)
+    d = np.diff(x)
+    w[:-1] += 0.5 * d
+    w[1:] += 0.5 * d
+    return w
+
+
+@dataclass
+class AliConfig:
+    nx: int = 60
+    ny: int = 60
+    nz: int = 120
+    nq: int = 120             # frequencies per line
+    qmax: float = 4.0e-5      # half-width of a line window in (nu - nu0) / nu0
+    ncont_f: int = 24         # frequencies per continuum
+    lines_per_window: int = 20
+    nphi: int = 4             # azimuths per Gauss mu: 2 x 4 x 2 = 16 rays
+    nline: int = 80
+    ncont: int = 20
+    atom_seed: int = 20261204
+    box_seed: int = 20261204
+    ne_scale: float = 1.0     # multiplies the electron density (collisional coupling)
+    max_iter: int = 40
+    conv: float = 1e-4
+
+
+@dataclass
+class _Window:
+    lines: list
+    nf: int
+    chi_c: torch.Tensor
+    beta: torch.Tensor
+    Bnu: torch.Tensor
+    ghost_B: torch.Tensor
+    const: dict = field(default_factory=dict)
+
+
+class NlteProblem:
+    def __init__(self, ctx: m3drt.Context, cfg: AliConfig = AliConfig(), sweep_dtype=torch.float32):
+        self.ctx, self.cfg, self.sweep_dtype = ctx, cfg, sweep_dtype
+        box = synth.make_box(cfg.nx, cfg.ny, cfg.nz, nf=2, seed=cfg.box_seed)
+        self.box = box
+        self.ncell = ncell = cfg.nx * cfg.ny * cfg.nz
+        dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+        T = box.T.cpu().numpy().reshape(-1)
+        c5 = box.chi500.double().cpu().numpy().reshape(-1)
+        # densities from the tau500 scale: n_H near 1e17 cm^-3 where tau500 = 1
+        H = (cfg.nz - 1) * box.ds[2] / (7.0 * math.log(10.0))
+        nH = 1e17 * c5 * H
+        ne = cfg.ne_scale * 1e-4 * nH * (T / 5800.0) ** 6
+        totn = 3e-5 * nH
+        atom = synth.make_atom(nline=cfg.nline, ncont=cfg.ncont, seed=cfg.atom_seed)
+        self.atom = atom
+        nstar = synth.lte_populations(atom, T, ne, totn)
+        self.T = T
+        self.nstar = dev(nstar, torch.float64)
+        self.totn = dev(totn, torch.float64)
+        self.b = torch.ones_like(self.nstar)
+        self.e_kT64 = dev(synth.EV_K / T, torch.float64)
+        self.temp = dev(T, torch.float64)
+        self.cup = dev(synth.van_regemorter(atom, T, ne, nstar), torch.float64)
+        self.atom_dev = m3drt.AtomOnDevice(atom)
+        # per cell: Doppler width (mass 56 plus 1 km/s), damping, e/kT
+        vth = np.sqrt(2 * synth.K_BOLTZ * T / (56.0 * synth.AMU) + 1e5 ** 2) / synth.C_LIGHT
+        self.inv_df = dev(1.0 / vth, torch.float32)
+        self.e_kT32 = dev(synth.EV_K / T, torch.float32)
+        self.adamp = dev(np.tile((2e-3 + 1e-2 * (ne / 1e14) ** 0.3)[None, :], (atom.nline, 1)), torch.float32)
+        self.pop_ratio = torch.ones(ncell, dtype=torch.float64, device="cuda")
+        self.vz = box.vz.reshape(-1)
+        self.vec, self.wj = synth.gauss_rays(cfg.nphi)
+        self.cu = [synth.chi500_upwind(box, v) for v in self.vec]
+        self.r500 = (box.g500_bot.double() / box.chi500[0].double()).reshape(-1)
+        self.windows = [self._window(lines) for lines in
+                        [list(range(w, min(atom.nline, w + cfg.lines_per_window)))
+                         for w in range(0, atom.nline, cfg.lines_per_window)]]
+        self.history = []
+        self._continuum()
+
+    # ---------------------------------------------------------------- spectra
+    def _background(self, nu):
+        """LTE background continuum on frequencies nu: chi_c, beta = e^(-h nu/kT) and
+        B = beta / (1 - beta), FP32 (nf, ncell); and B of the ghost plane below."""
+        T = self.temp[None, :]
+        c5 = self.box.chi500.double().reshape(1, -1)
+        nu = torch.as_tensor(nu, dtype=torch.float64, device="cuda")[:, None]
+        chi_c = c5 * (0.7 + 0.3 * (6e14 / nu) ** 3)
+        beta = torch.exp(-H_KT * nu / T).clamp(min=np.finfo(np.float32).tiny)
+        B = beta / (1.0 - beta)
+        bg = torch.exp(-H_KT * nu / self.box.T_ghost_bot.reshape(1, -1)).clamp(min=np.finfo(np.float32).tiny)
+        return chi_c.float(), beta.float(), B.float(), (bg / (1.0 - bg)).float().contiguous()
+
+    def _window(self, lines):
+        cfg, atom = self.cfg, self.atom
+        q = np.linspace(-cfg.qmax, cfg.qmax, cfg.nq)
+        q32 = q.astype(np.float32)
+        wq = _trapezoid(q32.astype(np.float64))
+        nline, nf = len(lines), len(lines) * cfg.nq
+        nu = np.concatenate([atom.nu0[l] * (1.0 + q) for l in lines])
+        qa = np.zeros((nline, nf), np.float32)
+        wqa = np.zeros((nline, nf))
+        wnu = np.zeros((nline, nf))
+        iblue = np.zeros(nline, np.int32)
+        ired = np.zeros(nline, np.int32)
+        for k, l in enumerate(lines):
+            fs = k * cfg.nq
+            iblue[k], ired[k] = fs + 1, fs + cfg.nq
+            qa[k, fs:fs + cfg.nq] = q32
+            wqa[k, fs:fs + cfg.nq] = wq
+            wnu[k, fs:fs + cfg.nq] = atom.nu0[l] * wq
+        li = atom.ti[lines].astype(np.int32)
+        lj = atom.tj[lines].astype(np.int32)
+        gi, gj = atom.g[li - 1], atom.g[lj - 1]
+        Aij = atom.Aji[lines] * gj / gi
+        # opacity per unit profile in (nu - nu0)/nu0: chi_line = n_i Bij phi (b_i - b_j beta)
+        Bij = PI_E2_MC * atom.f[lines] / atom.nu0[lines]
+        chi_c, beta, B, gB = self._background(nu)
+        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BB_DTYPES[k], device="cuda")
+        const = dict(iblue=d(iblue, "iblue"), ired=d(ired, "ired"), lev_i=d(li, "lev_i"), lev_j=d(lj, "lev_j"),
+                     Aij=d(Aij, "Aij"), Bij=d(Bij, "Bij"), lower_off=d(np.zeros(nline), "lower_off"),
+                     q=d(qa, "q"), wnu=d(wnu, "wnu"), wq=d(wqa, "wq"), g=d(atom.g, "g"))
+        return _Window(lines, nf, chi_c, beta, B, gB, const)
+
+    # ---------------------------------------------------------------- sweeps
+    def _sweep(self, r, chi, S, ghost_B, **out):
+        """Direction r over a window's (nf, ncell) chi, S (sweep dtype)."""
+        cfg, box = self.cfg, self.box
+        nf = chi.shape[0]
+        v = self.vec[r]
+        up = v[2] > 0
+        shp = (nf, cfg.nz, cfg.ny, cfg.nx)
+        if up:
+            # set_upstream_boundary / lte_ghost (SRC/rt_patch_mod.f90:3352-3373): the LTE
+            # plane below, chi = chi(z=1) chi500_ghost / chi500(1), S = I = B(T_ghost)
+            gchi = (chi[:, :cfg.nx * cfg.ny].double() * self.r500[None]).to(chi.dtype).contiguous()
+            gS = ghost_B.to(chi.dtype)
+            gI = gS
+        else:
+            gchi = torch.zeros((nf, cfg.nx * cfg.ny), dtype=chi.dtype, device="cuda")
+            gS = gI = gchi
+        pl = lambda t: t.view(nf, cfg.ny, cfg.nx)
+        self.ctx.sc_sweep(chi.view(shp), S.view(shp), pl(gchi), pl(gS), pl(gI), box.chi500, self.cu[r],
+                          box.g500_bot if up else box.g500_top, dir=1 if up else -1, ds=box.ds, vec=v,
+                          layout="fmajor", wj=float(self.wj[r]),
+                          **{k: (t.view(shp) if isinstance(t, torch.Tensor) and t.dim() == 2 else t)
+                             for k, t in out.items()})
+
+    def _continuum(self):
+        """Frequencies, cross sections and the angle-averaged J, PsiJ of the continua."""
+        cfg, atom = self.cfg, self.atom
+        nl, nc = atom.nline, len(atom.ti) - atom.nline
+        nus = [atom.nu0[nl + j] * np.exp(np.linspace(0.0, math.log(3.0), cfg.ncont_f)) for j in range(nc)]
+        nu = np.concatenate(nus)
+        nf = len(nu)
+        al = np.zeros((nc, nf), np.float32)
+        wnu = np.zeros((nc, nf))
+        for j in range(nc):
+            fs = j * cfg.ncont_f
+            al[j, fs:fs + cfg.ncont_f] = atom.sigma0[j] * (atom.nu0[nl + j] / nus[j]) ** 3
+            wnu[j, fs:fs + cfg.ncont_f] = _trapezoid(nus[j])
+        chi_c, beta, B, gB = self._background(nu)
+        self.cont = dict(chi_c=chi_c, B=B, ghost_B=gB)
+        dt = self.sweep_dtype
+        chi, S = chi_c.to(dt), B.to(dt)
+        J = torch.zeros_like(chi)
+        PsiJ = torch.zeros_like(chi)
+        for r in range(len(self.vec)):
+            self._sweep(r, chi, S, gB, J=J, PsiJ=PsiJ, get_rates=True)
+        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BF_DTYPES[k], device="cuda")
+        self.bf = dict(iblue=d(np.arange(nc) * cfg.ncont_f + 1, "iblue"), ired=d((np.arange(nc) + 1) * cfg.ncont_f, "ired"),
+                       lev_i=d(atom.ti[nl:], "lev_i"), lev_j=d(atom.tj[nl:], "lev_j"),
+                       active=d(np.ones(nc), "active"), ev=d(atom.ev, "ev"), wnu=d(wnu, "wnu"), alpha=d(al, "alpha"),
+                       twohnu3_c2=d(TWO_H_C2 * nu ** 3, "twohnu3_c2"), e_kT=self.e_kT64, nstar=self.nstar,
+                       Jnu=J.float().contiguous(), beta=beta, PsiJ=PsiJ.float().contiguous())
+
+    # ---------------------------------------------------------------- iteration
+    def _line_rates(self, w: _Window, gij, gji):
+        ctx, nc = self.ctx, self.ncell
+        nf = w.nf
+        f32 = lambda: torch.empty((nf, nc), dtype=torch.float32, device="cuda")
+        i32 = lambda: torch.empty((nf, nc), dtype=torch.int32, device="cuda")
+        xhat, ehat = f32(), f32()
+        sp = dict(split_opac1=f32(), split_opac2=f32(), split_chi1=f32(), split_chi2=f32(), split_id=i32(),
+                  split_chi_id=i32())
+        I, Psi, Irem, esc = f32(), f32(), f32(), f32()
+        sweep64 = self.sweep_dtype == torch.float64
+        l0, l1 = w.lines[0], w.lines[-1] + 1
+        cells = dict(b=self.b, nstar=self.nstar, pop_ratio=self.pop_ratio, inv_df=self.inv_df, e_kT=self.e_kT32,
+                     adamp=self.adamp[l0:l1])
+        for r, v in enumerate(self.vec):
+            vproj = (self.vz * (v[2] / synth.C_LIGHT)).float().contiguous()
+            d = dict(w.const, vproj=vproj, **cells)
+            xhat.zero_()
+            torch.mul(w.chi_c, w.Bnu, out=ehat)
+            ctx.convert_to_work_units(w.chi_c, xhat, ehat, w.beta)
+            for t in sp.values():
+                t.zero_()
+            ctx.bb_opacity(d, xhat, ehat, wmu=float(self.wj[r]), split=sp, split_offset=0)
+            ctx.work_to_transfer(xhat, ehat, w.beta, split_opac1=sp["split_opac1"], split_opac2=sp["split_opac2"])
+            chi, S = xhat, ehat
+            if sweep64:
+                o64 = [t.double() for t in (I, Psi, Irem, esc)]
+                self._sweep(r, chi.double(), S.double(), w.ghost_B, I_out=o64[0], Psi_out=o64[1], Irem_out=o64[2],
+                            escape_out=o64[3], split_rates=True, get_rates=True)
+                for a, b64 in zip((I, Psi, Irem, esc), o64):
+                    a.copy_(b64)
+            else:
+                self._sweep(r, chi, S, w.ghost_B, I_out=I, Psi_out=Psi, Irem_out=Irem, escape_out=esc,
+                            split_rates=True, get_rates=True)
+            ctx.bb_rates(dict(d, Inu=I, Psi=Psi, Bnu=w.Bnu, beta=w.beta), float(self.wj[r]), 1.0,
+                         gij[l0:l1], gji[l0:l1], precond=True, own_norm=True, largest=True,
+                         split=dict(sp, Irem=Irem, escape=esc, chi=chi), split_offset=0)
+
+    def rates(self):
+        """The radiative rate sums of one iteration: (gamma_ij, gamma_ji), (nrad, ncell)
+        float32, normalised (normalize_profiles)."""
+        atom = self.atom
+        nrad = len(atom.ti)
+        gij = torch.zeros((nrad, self.ncell), dtype=torch.float32, device="cuda")
+        gji = torch.zeros_like(gij)
+        for w in self.windows:
+            self._line_rates(w, gij, gji)
+        self.bf["b"] = self.b
+        self.ctx.bf_rates(self.bf, atom.nline, gij, gji, precond=False)
+        self.ctx.normalize_rates(self.atom_dev, self.nstar, gij, gji)
+        return gij, gji
+
+    def update(self, gij, gji):
+        """update_populations of every cell in place; returns the convergence measure."""
+        return self.ctx.rate_solve(self.atom_dev, self.e_kT64, self.temp, self.totn, self.nstar, self.b, gij, gji,
+                                   self.cup)
+
+    def iterate(self):
+        """One ALI iteration; returns the convergence measure of m3drt_rate_solve_f64."""
+        return self.update(*self.rates())
+
+    def solve(self, log=None):
+        """Iterate to max |dn/n| < conv or max_iter; returns the iteration count."""
+        for it in range(1, self.cfg.max_iter + 1):
+            t0 = time.perf_counter()
+            res = self.iterate()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            self.history.append((res["maxerr"], dt))
+            if log:
+                log(f"[ali] iteration {it}: max|dn/n| = {res['maxerr']:.3e}  ({dt:.2f} s)")
+            if abs(res["maxerr"]) < self.cfg.conv:
+                return it
+        return self.cfg.max_iter

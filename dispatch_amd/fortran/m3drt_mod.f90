@@ -20,8 +20,16 @@ module m3drt
   private
 
   integer(c_int), parameter, public :: M3DRT_OK = 0, M3DRT_F32 = 0, M3DRT_F64 = 1
-  integer(c_int), parameter, public :: M3DRT_LINEAR2 = 0
+  integer(c_int), parameter, public :: M3DRT_LINEAR2 = 0, M3DRT_QUADRATIC = 1, M3DRT_LINEAR = 2
   integer(c_int), parameter, public :: M3DRT_SPLIT_RATES = 1, M3DRT_GET_RATES = 2
+  integer(c_int), parameter, public :: M3DRT_OPAC_NLTE = 1, M3DRT_OPAC_SCAT = 2
+  integer(c_int), parameter, public :: M3DRT_RATES_PRECOND = 1, M3DRT_RATES_OWN_NORM = 2, &
+                                       M3DRT_RATES_LARGEST = 4, M3DRT_RATES_SCAT = 8, M3DRT_RATES_FLAGS = 16
+  integer(c_int), parameter, public :: M3DRT_BBOP_LTE = 1, M3DRT_BBOP_SPLIT = 2
+  integer(c_int), parameter, public :: M3DRT_UNITS_SPLIT = 1, M3DRT_UNITS_SCAT = 2, &
+                                       M3DRT_UNITS_SUPPRESS_MASERS = 4
+  integer(c_int), parameter, public :: M3DRT_BG_SCAT_ALI = 1, M3DRT_BG_J_TO_B = 2
+  integer(c_int), parameter, public :: M3DRT_ETA_GHOST = 1
 
   type, bind(C), public :: m3drt_sweep_desc
     integer(c_int32_t) :: nf, nx, ny, nz
@@ -44,9 +52,301 @@ module m3drt
     type(c_ptr) :: exit_I = c_null_ptr
   end type m3drt_sweep_bufs
 
+  type, bind(C), public :: m3drt_adjoint_bufs
+    type(c_ptr) :: chi = c_null_ptr
+    type(c_ptr) :: S = c_null_ptr
+    type(c_ptr) :: ghost_chi = c_null_ptr
+    type(c_ptr) :: ghost_S = c_null_ptr
+    type(c_ptr) :: ghost_I = c_null_ptr
+    type(c_ptr) :: chi500 = c_null_ptr
+    type(c_ptr) :: chi500_up = c_null_ptr
+    type(c_ptr) :: g500 = c_null_ptr
+    type(c_ptr) :: I = c_null_ptr
+    type(c_ptr) :: Jbar = c_null_ptr
+    type(c_ptr) :: Ibar = c_null_ptr
+    type(c_ptr) :: exit_bar = c_null_ptr
+    type(c_ptr) :: chibar = c_null_ptr
+    type(c_ptr) :: Sbar = c_null_ptr
+    type(c_ptr) :: ghost_chibar = c_null_ptr
+    type(c_ptr) :: ghost_Sbar = c_null_ptr
+    type(c_ptr) :: ghost_Ibar = c_null_ptr
+  end type m3drt_adjoint_bufs
+
+  type, bind(C), public :: m3drt_linelist
+    integer(c_int32_t) :: nline
+    type(c_ptr) :: llam0 = c_null_ptr
+    type(c_ptr) :: lower_ev = c_null_ptr
+    type(c_ptr) :: gpref = c_null_ptr
+    type(c_ptr) :: grad = c_null_ptr
+    type(c_ptr) :: gw1 = c_null_ptr
+    type(c_ptr) :: gw2 = c_null_ptr
+    type(c_ptr) :: gstk = c_null_ptr
+    type(c_ptr) :: unsold = c_null_ptr
+    type(c_ptr) :: lev_i = c_null_ptr
+    type(c_ptr) :: lev_j = c_null_ptr
+    type(c_ptr) :: ipart = c_null_ptr
+    type(c_ptr) :: used = c_null_ptr
+    type(c_ptr) :: strength = c_null_ptr
+    type(c_ptr) :: reach = c_null_ptr
+  end type m3drt_linelist
+
+  type, bind(C), public :: m3drt_opacity_args
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: ncell
+    integer(c_int32_t) :: npart
+    integer(c_int32_t) :: nlev
+    integer(c_int32_t) :: flags
+    real(c_float) :: tolerance
+    real(c_float) :: line_reject_frac
+    integer(c_int64_t) :: sf
+    integer(c_int64_t) :: sc
+    type(c_ptr) :: llam = c_null_ptr
+    type(c_ptr) :: vproj = c_null_ptr
+    type(c_ptr) :: inv_df = c_null_ptr
+    type(c_ptr) :: vbar2 = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: ne = c_null_ptr
+    type(c_ptr) :: tothi = c_null_ptr
+    type(c_ptr) :: stark_pe = c_null_ptr
+    type(c_ptr) :: lnbase = c_null_ptr
+    type(c_ptr) :: dep = c_null_ptr
+    type(c_ptr) :: chi = c_null_ptr
+    type(c_ptr) :: eta = c_null_ptr
+    type(c_ptr) :: cont = c_null_ptr
+    type(c_ptr) :: logref = c_null_ptr
+    type(c_ptr) :: one_m_eps = c_null_ptr
+    type(c_ptr) :: chi_scat = c_null_ptr
+  end type m3drt_opacity_args
+
+  type, bind(C), public :: m3drt_rate_solve_args
+    integer(c_int32_t) :: nk
+    integer(c_int32_t) :: nrad
+    integer(c_int32_t) :: nline
+    integer(c_int32_t) :: npair
+    integer(c_int32_t) :: ncell
+    type(c_ptr) :: trans_i = c_null_ptr
+    type(c_ptr) :: trans_j = c_null_ptr
+    type(c_ptr) :: trans_active = c_null_ptr
+    type(c_ptr) :: g = c_null_ptr
+    type(c_ptr) :: ev = c_null_ptr
+    type(c_ptr) :: Aji = c_null_ptr
+    type(c_ptr) :: pair_i = c_null_ptr
+    type(c_ptr) :: pair_j = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: totn = c_null_ptr
+    type(c_ptr) :: nstar = c_null_ptr
+    type(c_ptr) :: b = c_null_ptr
+    type(c_ptr) :: gamma_ij = c_null_ptr
+    type(c_ptr) :: gamma_ji = c_null_ptr
+    type(c_ptr) :: cup = c_null_ptr
+    real(c_double) :: max_update
+    real(c_double) :: min_pop_frac
+    real(c_double) :: conv_pop_floor
+    real(c_double) :: sor_w
+    real(c_double) :: cswitch
+    real(c_double) :: cut_x
+    real(c_double) :: bf_cut_rate
+    real(c_double) :: tmax_update
+  end type m3drt_rate_solve_args
+
+  type, bind(C), public :: m3drt_rate_solve_result
+    real(c_double) :: maxerr
+    integer(c_int32_t) :: epos
+    integer(c_int32_t) :: elvl
+    integer(c_int64_t) :: nclamp
+    type(c_ptr) :: lvlerr = c_null_ptr
+  end type m3drt_rate_solve_result
+
+  type, bind(C), public :: m3drt_hermite_fields
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: pe4 = c_null_ptr
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: dpe_dx = c_null_ptr
+    type(c_ptr) :: dpe_dy = c_null_ptr
+    type(c_ptr) :: dpe_dz = c_null_ptr
+    type(c_ptr) :: dlnt_dx = c_null_ptr
+    type(c_ptr) :: dlnt_dy = c_null_ptr
+    type(c_ptr) :: dlnt_dz = c_null_ptr
+    type(c_ptr) :: hknu = c_null_ptr
+  end type m3drt_hermite_fields
+
+  type, bind(C), public :: m3drt_bb_rates_args
+    integer(c_int32_t) :: nline
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: ncell
+    integer(c_int32_t) :: nk
+    integer(c_int32_t) :: flags
+    real(c_float) :: wmu
+    real(c_float) :: wtot
+    integer(c_int64_t) :: sf
+    integer(c_int64_t) :: sc
+    type(c_ptr) :: iblue = c_null_ptr
+    type(c_ptr) :: ired = c_null_ptr
+    type(c_ptr) :: lev_i = c_null_ptr
+    type(c_ptr) :: lev_j = c_null_ptr
+    type(c_ptr) :: Aij = c_null_ptr
+    type(c_ptr) :: Bij = c_null_ptr
+    type(c_ptr) :: lower_off = c_null_ptr
+    type(c_ptr) :: q = c_null_ptr
+    type(c_ptr) :: wnu = c_null_ptr
+    type(c_ptr) :: wq = c_null_ptr
+    type(c_ptr) :: g = c_null_ptr
+    type(c_ptr) :: b = c_null_ptr
+    type(c_ptr) :: nstar = c_null_ptr
+    type(c_ptr) :: pop_ratio = c_null_ptr
+    type(c_ptr) :: inv_df = c_null_ptr
+    type(c_ptr) :: vproj = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: adamp = c_null_ptr
+    type(c_ptr) :: Inu = c_null_ptr
+    type(c_ptr) :: Psi = c_null_ptr
+    type(c_ptr) :: Bnu = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: gamma_ij = c_null_ptr
+    type(c_ptr) :: gamma_ji = c_null_ptr
+    integer(c_int32_t) :: split_offset
+    integer(c_int32_t) :: npw
+    type(c_ptr) :: Irem = c_null_ptr
+    type(c_ptr) :: escape = c_null_ptr
+    type(c_ptr) :: chi = c_null_ptr
+    type(c_ptr) :: split_opac1 = c_null_ptr
+    type(c_ptr) :: split_opac2 = c_null_ptr
+    type(c_ptr) :: split_chi1 = c_null_ptr
+    type(c_ptr) :: split_chi2 = c_null_ptr
+    type(c_ptr) :: split_id = c_null_ptr
+    type(c_ptr) :: split_chi_id = c_null_ptr
+    type(c_ptr) :: chi_scat = c_null_ptr
+    type(c_ptr) :: dJnu = c_null_ptr
+    type(c_ptr) :: ntrans = c_null_ptr
+    type(c_ptr) :: psi_flags_in = c_null_ptr
+    type(c_ptr) :: psi_flags_out = c_null_ptr
+  end type m3drt_bb_rates_args
+
+  type, bind(C), public :: m3drt_bb_opacity_args
+    integer(c_int32_t) :: nline
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: ncell
+    integer(c_int32_t) :: nk
+    integer(c_int32_t) :: flags
+    integer(c_int32_t) :: split_offset
+    real(c_float) :: wmu
+    integer(c_int64_t) :: sf
+    integer(c_int64_t) :: sc
+    type(c_ptr) :: iblue = c_null_ptr
+    type(c_ptr) :: ired = c_null_ptr
+    type(c_ptr) :: lev_i = c_null_ptr
+    type(c_ptr) :: lev_j = c_null_ptr
+    type(c_ptr) :: Bij = c_null_ptr
+    type(c_ptr) :: lower_off = c_null_ptr
+    type(c_ptr) :: q = c_null_ptr
+    type(c_ptr) :: wq = c_null_ptr
+    type(c_ptr) :: wphi_line = c_null_ptr
+    type(c_ptr) :: b = c_null_ptr
+    type(c_ptr) :: nstar = c_null_ptr
+    type(c_ptr) :: pop_ratio = c_null_ptr
+    type(c_ptr) :: inv_df = c_null_ptr
+    type(c_ptr) :: vproj = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: adamp = c_null_ptr
+    type(c_ptr) :: xhat = c_null_ptr
+    type(c_ptr) :: ehat = c_null_ptr
+    type(c_ptr) :: split_opac1 = c_null_ptr
+    type(c_ptr) :: split_opac2 = c_null_ptr
+    type(c_ptr) :: split_chi1 = c_null_ptr
+    type(c_ptr) :: split_chi2 = c_null_ptr
+    type(c_ptr) :: split_id = c_null_ptr
+    type(c_ptr) :: split_chi_id = c_null_ptr
+    type(c_ptr) :: wphi = c_null_ptr
+  end type m3drt_bb_opacity_args
+
+  type, bind(C), public :: m3drt_bf_rates_args
+    integer(c_int32_t) :: ncont
+    integer(c_int32_t) :: nline
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: ncell
+    integer(c_int32_t) :: nk
+    integer(c_int32_t) :: precond
+    real(c_double) :: cut_x
+    integer(c_int64_t) :: sf
+    integer(c_int64_t) :: sc
+    type(c_ptr) :: iblue = c_null_ptr
+    type(c_ptr) :: ired = c_null_ptr
+    type(c_ptr) :: lev_i = c_null_ptr
+    type(c_ptr) :: lev_j = c_null_ptr
+    type(c_ptr) :: active = c_null_ptr
+    type(c_ptr) :: ev = c_null_ptr
+    type(c_ptr) :: wnu = c_null_ptr
+    type(c_ptr) :: alpha = c_null_ptr
+    type(c_ptr) :: twohnu3_c2 = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: nstar = c_null_ptr
+    type(c_ptr) :: b = c_null_ptr
+    type(c_ptr) :: Jnu = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: PsiJ = c_null_ptr
+    type(c_ptr) :: gamma_ij = c_null_ptr
+    type(c_ptr) :: gamma_ji = c_null_ptr
+  end type m3drt_bf_rates_args
+
+  type, bind(C), public :: m3drt_units_args
+    integer(c_int64_t) :: n
+    integer(c_int32_t) :: flags
+    real(c_double) :: maser_chi_floor
+    type(c_ptr) :: work1 = c_null_ptr
+    type(c_ptr) :: work2 = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: split_opac1 = c_null_ptr
+    type(c_ptr) :: split_opac2 = c_null_ptr
+    type(c_ptr) :: chi_scat = c_null_ptr
+    type(c_ptr) :: dJnu = c_null_ptr
+    type(c_ptr) :: n_floored = c_null_ptr
+  end type m3drt_units_args
+
+  type, bind(C), public :: m3drt_background_args
+    integer(c_int64_t) :: n
+    integer(c_int32_t) :: flags
+    real(c_float) :: wscat
+    real(c_double) :: scat_op_max
+    type(c_ptr) :: J = c_null_ptr
+    type(c_ptr) :: B = c_null_ptr
+    type(c_ptr) :: chi_cont = c_null_ptr
+    type(c_ptr) :: sigma = c_null_ptr
+    type(c_ptr) :: PsiJ = c_null_ptr
+    type(c_ptr) :: Jali_old = c_null_ptr
+    type(c_ptr) :: Jali = c_null_ptr
+    type(c_ptr) :: eta = c_null_ptr
+    type(c_ptr) :: dJnu = c_null_ptr
+    type(c_ptr) :: n_neg = c_null_ptr
+    type(c_ptr) :: diag = c_null_ptr
+  end type m3drt_background_args
+
+  type, bind(C), public :: m3drt_eta_args
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: nx
+    integer(c_int32_t) :: ny
+    integer(c_int32_t) :: nz
+    integer(c_int32_t) :: dir
+    integer(c_int32_t) :: flags
+    integer(c_int32_t) :: rx
+    integer(c_int32_t) :: ry
+    type(c_ptr) :: chi = c_null_ptr
+    type(c_ptr) :: S = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: ghost_chi = c_null_ptr
+    type(c_ptr) :: ghost_S = c_null_ptr
+    type(c_ptr) :: src = c_null_ptr
+    type(c_ptr) :: mask = c_null_ptr
+    type(c_ptr) :: ghost_src = c_null_ptr
+  end type m3drt_eta_args
+
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
+  public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi
+  public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
+  public :: m3drt_convert_to_work_units, m3drt_background_emission, m3drt_update_eta_mask
+  public :: m3drt_normalize_rates, m3drt_rate_solve_f64
   public :: short_char2_gpu, m3drt_error_message
 
   interface
@@ -102,6 +402,74 @@ module m3drt
       import :: c_int, m3drt_sweep_desc
       type(m3drt_sweep_desc), intent(inout) :: desc
     end function m3drt_native_strides
+    integer(c_int) function m3drt_sc_sweep_hermite(ctx, desc, bufs, fields) bind(C, name='m3drt_sc_sweep_hermite')
+      import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_sweep_bufs, m3drt_hermite_fields
+      type(c_ptr), value :: ctx
+      type(m3drt_sweep_desc), intent(in) :: desc
+      type(m3drt_sweep_bufs), intent(in) :: bufs
+      type(m3drt_hermite_fields), intent(in) :: fields
+    end function m3drt_sc_sweep_hermite
+    integer(c_int) function m3drt_sc_adjoint(ctx, desc, bufs) bind(C, name='m3drt_sc_adjoint')
+      import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_adjoint_bufs
+      type(c_ptr), value :: ctx
+      type(m3drt_sweep_desc), intent(in) :: desc
+      type(m3drt_adjoint_bufs), intent(in) :: bufs
+    end function m3drt_sc_adjoint
+    integer(c_int) function m3drt_line_opacity_equi(ctx, ll, args) bind(C, name='m3drt_line_opacity_equi')
+      import :: c_int, c_ptr, m3drt_linelist, m3drt_opacity_args
+      type(c_ptr), value :: ctx
+      type(m3drt_linelist), intent(in) :: ll
+      type(m3drt_opacity_args), intent(in) :: args
+    end function m3drt_line_opacity_equi
+    integer(c_int) function m3drt_bb_opacity(ctx, args) bind(C, name='m3drt_bb_opacity')
+      import :: c_int, c_ptr, m3drt_bb_opacity_args
+      type(c_ptr), value :: ctx
+      type(m3drt_bb_opacity_args), intent(in) :: args
+    end function m3drt_bb_opacity
+    integer(c_int) function m3drt_bb_rates(ctx, args) bind(C, name='m3drt_bb_rates')
+      import :: c_int, c_ptr, m3drt_bb_rates_args
+      type(c_ptr), value :: ctx
+      type(m3drt_bb_rates_args), intent(in) :: args
+    end function m3drt_bb_rates
+    integer(c_int) function m3drt_bf_rates(ctx, args) bind(C, name='m3drt_bf_rates')
+      import :: c_int, c_ptr, m3drt_bf_rates_args
+      type(c_ptr), value :: ctx
+      type(m3drt_bf_rates_args), intent(in) :: args
+    end function m3drt_bf_rates
+    integer(c_int) function m3drt_work_to_transfer(ctx, args) bind(C, name='m3drt_work_to_transfer')
+      import :: c_int, c_ptr, m3drt_units_args
+      type(c_ptr), value :: ctx
+      type(m3drt_units_args), intent(in) :: args
+    end function m3drt_work_to_transfer
+    integer(c_int) function m3drt_convert_to_work_units(ctx, n, chi_cont, xhat, ehat, beta) &
+        bind(C, name='m3drt_convert_to_work_units')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: chi_cont, xhat, ehat, beta
+    end function m3drt_convert_to_work_units
+    integer(c_int) function m3drt_background_emission(ctx, args) bind(C, name='m3drt_background_emission')
+      import :: c_int, c_ptr, m3drt_background_args
+      type(c_ptr), value :: ctx
+      type(m3drt_background_args), intent(in) :: args
+    end function m3drt_background_emission
+    integer(c_int) function m3drt_update_eta_mask(ctx, args) bind(C, name='m3drt_update_eta_mask')
+      import :: c_int, c_ptr, m3drt_eta_args
+      type(c_ptr), value :: ctx
+      type(m3drt_eta_args), intent(in) :: args
+    end function m3drt_update_eta_mask
+    integer(c_int) function m3drt_normalize_rates(ctx, args, wphi) bind(C, name='m3drt_normalize_rates')
+      import :: c_int, c_ptr, m3drt_rate_solve_args
+      type(c_ptr), value :: ctx
+      type(m3drt_rate_solve_args), intent(in) :: args
+      type(c_ptr), value :: wphi
+    end function m3drt_normalize_rates
+    integer(c_int) function m3drt_rate_solve_f64(ctx, args, res) bind(C, name='m3drt_rate_solve_f64')
+      import :: c_int, c_ptr, m3drt_rate_solve_args, m3drt_rate_solve_result
+      type(c_ptr), value :: ctx
+      type(m3drt_rate_solve_args), intent(in) :: args
+      type(m3drt_rate_solve_result), intent(inout) :: res
+    end function m3drt_rate_solve_f64
   end interface
 
 contains

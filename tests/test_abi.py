@@ -102,3 +102,54 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(m3drt, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(m3drt.M3drtError):
         m3drt.lib()
+
+
+def _header_structs():
+    """(typedef name, [field names]) of every struct of include/m3drt.h."""
+    import re
+    src = open(os.path.join(ROOT, "include", "m3drt.h")).read()
+    out = []
+    for body, name in re.findall(r"typedef struct \{(.*?)\} (m3drt_[a-z_]+);", src, re.S):
+        fields = []
+        for line in body.split("\n"):
+            line = re.sub(r"/\*.*?\*/", "", line).strip().rstrip(";").replace("const ", "")
+            if not line:
+                continue
+            rest = re.match(r"(?:int32_t|int64_t|float|double|void|uint8_t)\s+(.*)", line).group(1)
+            fields += [re.sub(r"\[\d+\]", "", v.strip().lstrip("*").strip()) for v in rest.split(",")]
+        out.append((name, fields))
+    return out
+
+
+def test_fortran_module_layouts_match_header():
+    """Every bind(C) type of dispatch_amd/fortran/m3drt_mod.f90 has the C compiler's
+    size and field offsets (amdflang against gcc)."""
+    import shutil
+    import tempfile
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    structs = _header_structs()
+    clines, flines = [], []
+    for i, (name, fields) in enumerate(structs):
+        clines.append(f'  printf("%zu\\n", sizeof({name}));')
+        clines += [f'  printf("%zu\\n", offsetof({name}, {f}));' for f in fields]
+        flines.append(f"  print '(i0)', c_sizeof(v{i})")
+        flines += [f"  print '(i0)', transfer(c_loc(v{i}%{f}), 0_c_intptr_t) - transfer(c_loc(v{i}), 0_c_intptr_t)"
+                   for f in fields]
+    decl = "\n".join(f"  type({name}), target :: v{i}" for i, (name, _) in enumerate(structs))
+    fsrc = "program layouts\n  use iso_c_binding\n  use m3drt\n  implicit none\n" + decl + "\n" + \
+           "\n".join(flines) + "\nend program layouts\n"
+    csrc = "#include <stdio.h>\n#include <stddef.h>\n#include \"m3drt.h\"\nint main(void) {\n" + \
+           "\n".join(clines) + "\n  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "t.c"), "w").write(csrc)
+        open(os.path.join(d, "t.f90"), "w").write(fsrc)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "t.c", "-o", "tc"], cwd=d, check=True)
+        subprocess.run([fc, "-c", os.path.join(ROOT, "dispatch_amd", "fortran", "m3drt_mod.f90")], cwd=d, check=True)
+        # the module procedures name library symbols this program never calls
+        subprocess.run([fc, "t.f90", "m3drt_mod.o", "-Wl,--unresolved-symbols=ignore-all", "-o", "tf"], cwd=d,
+                       check=True)
+        want = subprocess.run([os.path.join(d, "tc")], capture_output=True, text=True, check=True).stdout.split()
+        got = subprocess.run([os.path.join(d, "tf")], capture_output=True, text=True, check=True).stdout.split()
+    assert got == want

@@ -1,0 +1,56 @@
+"""The NLTE ALI driver (dispatch_amd.ali) on small boxes of its workload generator:
+one iteration's rate sums and departure coefficients against the oracle's
+composition of the same steps (tests/ali_cases.py); convergence; the LTE limit
+under strong collisional coupling."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def small(**kw):
+    from dispatch_amd.ali import AliConfig
+    base = dict(nx=6, ny=6, nz=16, nq=16, nline=6, ncont=3, lines_per_window=3, nphi=2, ncont_f=8)
+    base.update(kw)
+    return AliConfig(**base)
+
+
+def test_ali_iteration_matches_oracle(ctx):
+    from dispatch_amd.ali import NlteProblem
+    from ali_cases import oracle_rates, oracle_update
+    p = NlteProblem(ctx, small())
+    print("ali iteration vs oracle, 6x6x16, 6 lines, 3 continua, 8 rays", flush=True)
+    for it in range(2):
+        gij, gji = p.rates()
+        torch.cuda.synchronize()
+        rij, rji = oracle_rates(p)
+        for got, ref, what in ((gij, rij, "gamma_ij"), (gji, rji, "gamma_ji")):
+            g = got.double().cpu().numpy()
+            scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
+            err = np.abs(g - ref) / scale
+            assert err.max() < 2e-4, f"iteration {it} {what}: {err.max():.3e} (row {np.unravel_index(err.argmax(), err.shape)})"
+        b_ref, res_ref = oracle_update(p, rij, rji)
+        res = p.update(gij, gji)
+        b = p.b.cpu().numpy()
+        assert np.abs(b / b_ref - 1).max() < 2e-4, f"iteration {it} b: {np.abs(b / b_ref - 1).max():.3e}"
+        assert abs(res["maxerr"] - res_ref["maxerr"]) <= 2e-4 * abs(res_ref["maxerr"]) + 1e-6
+        p.b.copy_(torch.from_numpy(b_ref))   # both continue from the oracle's populations
+
+
+def test_ali_converges_and_departs_from_lte(ctx):
+    from dispatch_amd.ali import NlteProblem
+    p = NlteProblem(ctx, small(max_iter=40))
+    n = p.solve(log=lambda m: print(m, flush=True))
+    errs = [abs(e) for e, _ in p.history]
+    assert errs[-1] < 1e-4 and n <= 40, errs
+    b = p.b.cpu().numpy()
+    assert np.abs(b - 1).max() > 1e-2   # the lines' NLTE departures are there
+
+
+def test_ali_strong_collisions_give_lte(ctx):
+    from dispatch_amd.ali import NlteProblem
+    p = NlteProblem(ctx, small(ne_scale=1e7, max_iter=40))
+    p.solve()
+    b = p.b.cpu().numpy()
+    assert np.abs(b - 1).max() < 1e-3, np.abs(b - 1).max()
