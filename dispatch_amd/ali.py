@@ -65,6 +65,7 @@ class AliConfig:
     ncont_f: int = 24         # frequencies per continuum
     lines_per_window: int = 20
     nphi: int = 4             # azimuths per Gauss mu: 2 x 4 x 2 = 16 rays
+    nbound: int = 20          # bound levels; every one has a continuum when ncont = nbound
     nline: int = 80
     ncont: int = 20
     atom_seed: int = 20261204
@@ -94,12 +95,13 @@ class NlteProblem:
         dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
         T = box.T.cpu().numpy().reshape(-1)
         c5 = box.chi500.double().cpu().numpy().reshape(-1)
-        # densities from the tau500 scale: n_H near 1e17 cm^-3 where tau500 = 1
+        # densities from the tau500 scale: n_H near 1e17 cm^-3 where tau500 = 1, and
+        # electrons from the metals below ~6000 K, rising with T above
         H = (cfg.nz - 1) * box.ds[2] / (7.0 * math.log(10.0))
         nH = 1e17 * c5 * H
-        ne = cfg.ne_scale * 1e-4 * nH * (T / 5800.0) ** 6
+        ne = cfg.ne_scale * 1e-4 * nH * np.minimum((T / 5800.0) ** 6, 30.0)
         totn = 3e-5 * nH
-        atom = synth.make_atom(nline=cfg.nline, ncont=cfg.ncont, seed=cfg.atom_seed)
+        atom = synth.make_atom(nbound=cfg.nbound, nline=cfg.nline, ncont=cfg.ncont, seed=cfg.atom_seed)
         self.atom = atom
         nstar = synth.lte_populations(atom, T, ne, totn)
         self.T = T
@@ -108,7 +110,9 @@ class NlteProblem:
         self.b = torch.ones_like(self.nstar)
         self.e_kT64 = dev(synth.EV_K / T, torch.float64)
         self.temp = dev(T, torch.float64)
-        self.cup = dev(synth.van_regemorter(atom, T, ne, nstar), torch.float64)
+        # Seaton's ionisation with gbar = 0.1 and the edge cross section
+        self.cup = dev(synth.van_regemorter(atom, T, ne, nstar, ion_scale=0.1 * float(atom.sigma0[0])),
+                       torch.float64)
         self.atom_dev = m3drt.AtomOnDevice(atom)
         # per cell: Doppler width (mass 56 plus 1 km/s), damping, e/kT
         vth = np.sqrt(2 * synth.K_BOLTZ * T / (56.0 * synth.AMU) + 1e5 ** 2) / synth.C_LIGHT
@@ -296,3 +300,49 @@ class NlteProblem:
             if abs(res["maxerr"]) < self.cfg.conv:
                 return it
         return self.cfg.max_iter
+
+
+def main():
+    """Run the NLTE workload to convergence and print one JSON line (iterations, time
+    per iteration, the convergence history); with --fp64-sweeps also the same
+    iterations with the plain FP64 sweep, and the departure coefficients' largest
+    relative difference between the two."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nx", type=int, default=60)
+    ap.add_argument("--nz", type=int, default=120)
+    ap.add_argument("--max-iter", type=int, default=40)
+    ap.add_argument("--fp64-sweeps", action="store_true")
+    args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    ctx = m3drt.Context(0)
+    cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter)
+    t0 = time.perf_counter()
+    p = NlteProblem(ctx, cfg)
+    torch.cuda.synchronize()
+    log(f"[ali] set-up {time.perf_counter() - t0:.1f} s")
+    n = p.solve(log=log)
+    its = [dt for _, dt in p.history]
+    out = dict(workload="3d_nlte_ali", box=[cfg.nx, cfg.ny, cfg.nz], levels=p.atom.nk, lines=p.atom.nline,
+               continua=len(p.atom.ti) - p.atom.nline, frequencies_per_line=cfg.nq, rays=len(p.vec),
+               iterations=n, converged=abs(p.history[-1][0]) < cfg.conv, maxerr=[e for e, _ in p.history],
+               s_per_iteration=float(np.median(its)),
+               line_updates_per_iteration=p.ncell * p.atom.nline * cfg.nq * len(p.vec),
+               b_min=float(p.b.min()), b_max=float(p.b.max()))
+    out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
+    if args.fp64_sweeps:
+        b32 = p.b.clone()
+        del p
+        torch.cuda.empty_cache()
+        q = NlteProblem(ctx, cfg, sweep_dtype=torch.float64)
+        n64 = q.solve(log=log)
+        out["fp64_sweeps"] = dict(iterations=n64, s_per_iteration=float(np.median([dt for _, dt in q.history])),
+                                  b_max_rel_diff=float((b32 / q.b - 1).abs().max()))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

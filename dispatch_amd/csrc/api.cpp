@@ -229,25 +229,35 @@ int m3drt_sc_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjo
   return m3drt::sweep_adjoint(ctx, d, b);
 }
 
-int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
-  if (!ctx || !ll || !o) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NULL argument");
-  if (ll->nline < 0 || o->ncell < 0 || o->nf < 0)
-    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: negative size");
-  if (ll->nline == 0 || o->ncell == 0 || o->nf < 2) return M3DRT_OK;  // as the reference: nothing to add
+static int line_opacity_entry(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o,
+                              bool general) {
+  const std::string w = general ? "m3drt_line_opacity: " : "m3drt_line_opacity_equi: ";
+  if (!ctx || !ll || !o) return set_error(M3DRT_EINVAL, w + "NULL argument");
+  if (ll->nline < 0 || o->ncell < 0 || o->nf < 0) return set_error(M3DRT_EINVAL, w + "negative size");
+  // as the reference: nothing to add (the equidistant form needs two nodes for its step)
+  if (ll->nline == 0 || o->ncell == 0 || o->nf < (general ? 1 : 2)) return M3DRT_OK;
   if (!ll->llam0 || !ll->lower_ev || !ll->gpref || !ll->grad || !ll->gw1 || !ll->gw2 || !ll->gstk ||
       !ll->unsold || !ll->lev_i || !ll->lev_j)
-    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: missing line-list array");
+    return set_error(M3DRT_EINVAL, w + "missing line-list array");
   if (!o->llam || !o->vproj || !o->inv_df || !o->vbar2 || !o->e_kT || !o->ne || !o->tothi ||
       !o->stark_pe || !o->lnbase || !o->chi || !o->eta || !o->cont || !o->logref)
-    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: missing cell array");
-  if (o->npart < 1) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: npart < 1");
+    return set_error(M3DRT_EINVAL, w + "missing cell array");
+  if (o->npart < 1) return set_error(M3DRT_EINVAL, w + "npart < 1");
   if ((o->flags & M3DRT_OPAC_NLTE) && (!o->dep || o->nlev < 1))
-    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: NLTE needs dep and nlev");
+    return set_error(M3DRT_EINVAL, w + "NLTE needs dep and nlev");
   if ((o->flags & M3DRT_OPAC_SCAT) && (!o->one_m_eps || !o->chi_scat))
-    return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: scattering needs one_m_eps and chi_scat");
-  if (o->sf < 1 || o->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_line_opacity_equi: strides must be >= 1");
+    return set_error(M3DRT_EINVAL, w + "scattering needs one_m_eps and chi_scat");
+  if (o->sf < 1 || o->sc < 1) return set_error(M3DRT_EINVAL, w + "strides must be >= 1");
   M3DRT_HIP(hipSetDevice(ctx->device));
-  return m3drt::line_opacity_equi(ctx, ll, o);
+  return m3drt::line_opacity(ctx, ll, o, general);
+}
+
+int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
+  return line_opacity_entry(ctx, ll, o, false);
+}
+
+int m3drt_line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
+  return line_opacity_entry(ctx, ll, o, true);
 }
 
 static int check_atom(const m3drt_rate_solve_args *r, const char *who) {

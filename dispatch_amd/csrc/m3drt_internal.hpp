@@ -31,7 +31,7 @@ int ensure_scratch(m3drt_ctx *ctx, size_t bytes);
 int sweep_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_lc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int qlg_tables(m3drt_ctx *ctx);
-int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
+int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general);
 int rate_solve(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res);
 int normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi);
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);

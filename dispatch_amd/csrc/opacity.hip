@@ -1,6 +1,10 @@
-// Line opacity on an equidistant ln(lambda) grid: the summation of
-// add_opacity_equi (HOP/linelist_mod.f90:729-960) with the QLG-bot Voigt kernel
-// (HOP/qlg_mod.f90:223-256, 524-653), for gfx950.
+// Line opacity: the summation of add_opacity_equi (HOP/linelist_mod.f90:729-960,
+// an equidistant ln(lambda) grid) and of add_opacity (:469-719, a grid of any
+// spacing) with the QLG-bot Voigt kernel (HOP/qlg_mod.f90:223-256, 524-653), for
+// gfx950.  The two share the line loop and its tolerance logic; they differ in how
+// a line centre finds its node (index arithmetic, or a forward cursor taking the
+// nearer node), the abscissa of the profile (grid steps, or ln(lambda) itself) and
+// the window (nearest node +- nint(half-width), or bisection).
 //
 // Mapping.  Each cell walks its line list in order, because whether a line is
 // kept depends on the opacity the earlier lines already deposited at its centre
@@ -74,7 +78,21 @@ __device__ __forceinline__ void atomic_max_pos(float *p, float v) {
   atomicMax(reinterpret_cast<int *>(p), __float_as_int(v));
 }
 
-__global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
+// bisect_nearest (HOP/linelist_mod.f90:1075-1095), 1-based: the first node of
+// llam(ws:we) at or above x, one down when the node below is strictly nearer
+__device__ __forceinline__ int bisect_nearest(const double *llam, int ws, int we, double x) {
+  int lo = ws, hi = we;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (llam[mid - 1] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo > ws && fabs(llam[lo - 2] - x) < fabs(llam[lo - 1] - x)) lo = lo - 1;
+  return lo;
+}
+
+template <bool GENERAL>
+__global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];  // TGX then TH2X
   for (int i = threadIdx.x; i <= QLG_NX; i += blockDim.x) {
     tab[i] = a.tgx[i];
@@ -96,9 +114,10 @@ __global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
   const float *cont = a.cont + cidx * a.sc, *logref = a.logref + cidx * a.sc;
   const int64_t sf = a.sf;
 
-  // the cell's Doppler width in grid steps and its cell-invariant terms (qlg_bot_cell)
+  // the cell's Doppler width (in grid steps, or in ln(lambda) on a general grid) and
+  // its cell-invariant terms (qlg_bot_cell)
   const float idf = a.idf[cidx];
-  const float D = sdl / idf, D2 = D * D, SQ = 0.8325546112f * D2;
+  const float D = GENERAL ? 1.0f / idf : sdl / idf, D2 = D * D, SQ = 0.8325546112f * D2;
   const float D2h = 0.5f * D2, D2G = 1.5f * D2, D2L = 5.0f * D2;
   const float vb2 = a.vb2[cidx];
   const float unsold_c = (float)pow((double)vb2, (double)0.3f) * a.tothi[cidx];
@@ -106,13 +125,20 @@ __global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
   const float ekt = a.ekt[cidx], ne_c = a.ne[cidx], tothi_c = a.tothi[cidx];
   const float starkpe = a.spe[cidx], vproj_c = a.vproj[cidx];
 
-  int f0_prev = -1;
+  int f0_prev = -1, fc = 1;
   float refl_c = 0.0f, lref_c = 0.0f;
   for (int l = 0; l < a.nline; ++l) {
     const double llc = a.llam0[l] - (double)vproj_c;
     const double pos = (llc - llam1) * inv_dllam;
-    int f0 = nint(pos) + 1;
-    f0 = min(nf, max(1, f0));
+    int f0;
+    if (GENERAL) {
+      // the cursor only advances: the lines ascend and vproj is the cell's
+      // (HOP/linelist_mod.f90:578-590)
+      while (fc < nf && !(a.llam[fc - 1] >= llc)) ++fc;
+      f0 = fc > 1 ? (fabs(a.llam[fc - 1] - llc) < fabs(a.llam[fc - 2] - llc) ? fc : fc - 1) : 1;
+    } else {
+      f0 = min(nf, max(1, nint(pos) + 1));
+    }
     const float ex = a.lower_ev[l] * ekt;
     const int ip = a.ipart ? a.ipart[l] : 1;
     const float arg = a.lnbase[(int64_t)(ip - 1) * a.ncell + cidx] + a.gpref[l] - ex;
@@ -149,9 +175,48 @@ __global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
       gam = gam + a.gstk[l] * ne_c;
     else
       gam = gam - a.gstk[l] * starkpe;
-    gam = gam * sdl;
+    if (!GENERAL) gam = gam * sdl;
     if (a.strength) atomic_max_pos(&a.strength[l], voigt_peak * Rij / cont[(f0 - 1) * sf]);
     const float lnq = arg - lref_c;
+    const bool plain = !(a.scat && !nlte);
+    if (GENERAL) {
+      // qlg_bot_line on the ln(lambda) abscissa; the window by bisection unless the
+      // line covers the grid (HOP/linelist_mod.f90:660-705)
+      const Coef c = bot_setup(gam, D, SQ, Rij * D);
+      const float AL = 0.3183098862f * gam * (Rij * D) / ref;
+      const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
+                                         AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
+      if (a.used) a.used[l] = 1;
+      if (a.reach) atomic_max_pos(&a.reach[l], hw);
+      const double dl = (double)hw;
+      int fa = 1, fb = nf;
+      if (!(dl >= fmax(a.llam[nf - 1] - llc, llc - a.llam[0]))) {
+        fa = bisect_nearest(a.llam, 1, f0, llc - dl);
+        fb = bisect_nearest(a.llam, f0, nf, llc + dl);
+      }
+      for (int f = fa; f <= fb; ++f) {
+        const float x = (float)(a.llam[f - 1] - llc);
+        const float xs = fabsf(x) * c.ih;
+        const float u = xs * xs;
+        const float L = __builtin_amdgcn_rcpf(1.0f + u);
+        const int ig = nint(fminf(xs * 800.0f, (float)QLG_NX));
+        const float G = TGX[ig], W = TH2X[ig];
+        float p = c.cL2 + c.cZH * W;
+        p = c.bL + L * p;
+        const float lte = G * c.gG + L * p;
+        const int64_t o = (int64_t)(f - 1) * sf;
+        if (plain) {
+          chi[o] = chi[o] + lte * bi;
+          eta[o] = eta[o] + lte * bj;
+        } else {
+          chi[o] = chi[o] + lte;
+          eta[o] = eta[o] + lte;
+          a.csc[cidx * a.sc + o] = a.csc[cidx * a.sc + o] + a.ome[cidx * a.sc + o] * lte;
+        }
+      }
+      f0_prev = -1;
+      continue;
+    }
 
     // qlg_bot_line_sweep (HOP/qlg_mod.f90:630-653): setup, window, sweep
     const float S = Rij * D;
@@ -169,7 +234,6 @@ __global__ void __launch_bounds__(256) opacity_equi_kernel(const OpacArgs a) {
     if (fa > fb) continue;
     if (a.used) a.used[l] = 1;
     const float x0 = x1 - (float)(i1 - fa);
-    const bool plain = !(a.scat && !nlte);
     for (int f = fa; f <= fb; ++f) {
       // qlg_bot_line_eval (HOP/qlg_mod.f90:542-556)
       const float x = x0 + (float)(f - fa);
@@ -271,7 +335,7 @@ int qlg_tables(m3drt_ctx *ctx) {
   return M3DRT_OK;
 }
 
-int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o) {
+int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general) {
   int rc = qlg_tables(ctx);
   if (rc) return rc;
   OpacArgs a;
@@ -296,9 +360,9 @@ int line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opac
   const int threads = 256;
   const int blocks = (o->ncell + threads - 1) / threads;
   const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
-  M3DRT_HIP(hipFuncSetAttribute((const void *)opacity_equi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-  hipLaunchKernelGGL(opacity_equi_kernel, dim3(blocks), dim3(threads), lds, ctx->stream, a);
+  auto kern = general ? opacity_kernel<true> : opacity_kernel<false>;
+  M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
   return M3DRT_OK;
 }

@@ -12,9 +12,10 @@ at offsets of +-0.2 dex with Gaussian noise at S/N 300.
 
 Gradients: the adjoint sweep (dispatch_amd.adjoint, m3drt_sc_adjoint) takes the
 flux cotangent back to chi and S; the abundance gradient is then
-dL/dd_e = ln10 10^d_e sum chi500 L_e chibar (a reduction per element), and the
-temperature gradient comes from torch autograd through the opacity of each slab
-of planes, recomputed.  chi500 (the tau500 step scale) and the velocities are held
+dL/dd_e = ln10 10^d_e sum chi500 L_e chibar (a reduction per element, over the
+element cubes chi500 L_e, which are computed once: they do not depend on the
+abundances), and the temperature gradient comes from torch autograd through the
+opacity of each plane, recomputed.  chi500 (the tau500 step scale) and the velocities are held
 fixed.  The FP64 path (FP64 cubes and the plain FP64 sweep) gives the central
 finite differences the gradients are checked against.
 """
@@ -80,16 +81,48 @@ class FitProblem:
         return c5 * tot, S
 
     def cubes(self, d, dtype=torch.float32):
+        """chi and S for abundance offsets d.  FP32 from the cached element cubes (the
+        opacity's T dependence is fixed while only the abundances move); FP64 (the
+        finite-difference reference) from the opacity at the current T, each value
+        rounded to FP32 first so both see the same inputs."""
         box = self.box
+        if dtype == torch.float32:
+            self._prepare()
+            chi = self._cont.clone()
+            for e in range(len(ELEMENTS)):
+                chi.add_(self._L[e], alpha=10.0 ** float(d[e]))
+            return chi, self._S.clone()
         chi = torch.empty((box.nz, box.nf, box.ny, box.nx), dtype=dtype, device="cuda")
         S = torch.empty_like(chi)
         d = torch.as_tensor(d, dtype=torch.float64, device="cuda")
         for z0 in range(0, box.nz, SLAB):
             z1 = min(box.nz, z0 + SLAB)
             c, s = self.chi_S(d, z0, z1)
-            chi[z0:z1] = c.float().to(dtype)   # FP64 cubes hold the FP32 values widened
+            chi[z0:z1] = c.float().to(dtype)
             S[z0:z1] = s.float().to(dtype)
         return chi, S
+
+    def _prepare(self):
+        """The FP32 cubes chi500 cont, chi500 L_e (e = 1..4) and S at the current T."""
+        if getattr(self, "_cont", None) is not None:
+            return
+        box = self.box
+        shape = (box.nz, box.nf, box.ny, box.nx)
+        self._cont = torch.empty(shape, dtype=torch.float32, device="cuda")
+        self._L = [torch.empty_like(self._cont) for _ in ELEMENTS]
+        self._S = torch.empty_like(self._cont)
+        for z0 in range(0, box.nz, SLAB):
+            z1 = min(box.nz, z0 + SLAB)
+            cont, L, S = self.slab(z0, z1)
+            c5 = box.chi500[z0:z1].double()[:, None]
+            self._cont[z0:z1] = (c5 * cont).float()
+            for e in range(len(ELEMENTS)):
+                self._L[e][z0:z1] = (c5 * L[e]).float()
+            self._S[z0:z1] = S.float()
+
+    def release(self):
+        """Drop the cached cubes (after a change of T)."""
+        self._cont = self._L = self._S = None
 
     def ghosts(self, chi):
         """The LTE plane below the bottom (chi(z=1) chi500_ghost/chi500(1), S = I =
@@ -158,16 +191,17 @@ class FitProblem:
         t2 = _now()
         chibar = chi.grad
         chibar[0] += (gu[0].grad.double() * self.r500[None]).float()  # the ghost's chi follows chi(z=1)
+        # dL/dd_e = ln10 10^d_e <chi500 L_e, chibar>: one dot product per element and
+        # slab of planes (FP32 within a slab, FP64 across)
         gd_ = torch.zeros(len(ELEMENTS), dtype=torch.float64, device="cuda")
         box = self.box
         with torch.no_grad():
-            for z0 in range(0, box.nz, SLAB):
-                z1 = min(box.nz, z0 + SLAB)
-                c5 = box.chi500[z0:z1].double()[:, None]
-                cb = chibar[z0:z1].double()
-                _, L, _ = self.slab(z0, z1)
-                for e in range(len(ELEMENTS)):
-                    gd_[e] += math.log(10.0) * 10.0 ** float(d[e]) * (c5 * L[e] * cb).sum()
+            for e in range(len(ELEMENTS)):
+                acc = torch.zeros((), dtype=torch.float64, device="cuda")
+                for z0 in range(0, box.nz, SLAB):
+                    z1 = min(box.nz, z0 + SLAB)
+                    acc += torch.dot(self._L[e][z0:z1].reshape(-1), chibar[z0:z1].reshape(-1)).double()
+                gd_[e] = math.log(10.0) * 10.0 ** float(d[e]) * acc
         gT = None
         if want_T:
             # one plane at a time: autograd keeps every intermediate of the profiles
@@ -187,7 +221,7 @@ class FitProblem:
             gT[1] -= Tg.grad
         if timing is not None:
             timing.update(forward_s=t1 - t0, adjoint_s=t2 - t1, chain_s=t3 - t2)
-        return float(loss), gd_, gT
+        return float(loss.detach()), gd_, gT
 
 
 def _now():

@@ -343,7 +343,7 @@ module m3drt
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
-  public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi
+  public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi, m3drt_line_opacity
   public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
   public :: m3drt_convert_to_work_units, m3drt_background_emission, m3drt_update_eta_mask
   public :: m3drt_normalize_rates, m3drt_rate_solve_f64
@@ -421,6 +421,12 @@ module m3drt
       type(m3drt_linelist), intent(in) :: ll
       type(m3drt_opacity_args), intent(in) :: args
     end function m3drt_line_opacity_equi
+    integer(c_int) function m3drt_line_opacity(ctx, ll, args) bind(C, name='m3drt_line_opacity')
+      import :: c_int, c_ptr, m3drt_linelist, m3drt_opacity_args
+      type(c_ptr), value :: ctx
+      type(m3drt_linelist), intent(in) :: ll
+      type(m3drt_opacity_args), intent(in) :: args
+    end function m3drt_line_opacity
     integer(c_int) function m3drt_bb_opacity(ctx, args) bind(C, name='m3drt_bb_opacity')
       import :: c_int, c_ptr, m3drt_bb_opacity_args
       type(c_ptr), value :: ctx

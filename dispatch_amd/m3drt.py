@@ -270,20 +270,22 @@ class OpacityArgs(C.Structure):
 def _opacity_lib():
     L = lib()
     if not getattr(L, "_opac_ok", False):
-        L.m3drt_line_opacity_equi.argtypes = [C.c_void_p, C.POINTER(Linelist), C.POINTER(OpacityArgs)]
-        L.m3drt_line_opacity_equi.restype = C.c_int
+        for n in ("m3drt_line_opacity_equi", "m3drt_line_opacity"):
+            getattr(L, n).argtypes = [C.c_void_p, C.POINTER(Linelist), C.POINTER(OpacityArgs)]
+            getattr(L, n).restype = C.c_int
         L._opac_ok = True
     return L
 
 
 def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None,
                        nlte=False, one_m_eps=None, chi_scat=None, layout="native", tolerance=1e-4,
-                       line_reject_frac=0.0, used=None, strength=None, reach=None):
+                       line_reject_frac=0.0, used=None, strength=None, reach=None, general=False):
     """add_opacity_equi of one isotope's line list on the device.  lines: dict of
     device tensors (llam0 float64; lower_ev, gpref, grad, gw1, gw2, gstk float32;
     unsold, lev_i, lev_j, optional ipart int32).  Spectra (chi, eta, cont, logref,
     one_m_eps, chi_scat) are (nf, ncell) in the native layout (cells at unit
-    stride) or (ncell, nf) in the reference layout (f at unit stride)."""
+    stride) or (ncell, nf) in the reference layout (f at unit stride).  general: the
+    grid's spacing may vary (add_opacity, m3drt_line_opacity)."""
     L = _opacity_lib()
     nl = int(lines["llam0"].numel())
     if layout == "native":
@@ -311,10 +313,18 @@ def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase,
                         "vproj", "idf", "vb2", "ekt", "ne", "tothi", "spe")],
                     p(lnbase), p(dep), p(chi), p(eta), p(cont), p(logref), p(one_m_eps), p(chi_scat))
     self._bind_stream()
-    _check(L.m3drt_line_opacity_equi(self._h, C.byref(ll), C.byref(o)), "m3drt_line_opacity_equi")
+    fn = "m3drt_line_opacity" if general else "m3drt_line_opacity_equi"
+    _check(getattr(L, fn)(self._h, C.byref(ll), C.byref(o)), fn)
+
+
+def _line_opacity(self, *args, **kw):
+    """add_opacity of one isotope's line list on a grid of any spacing; arguments as
+    line_opacity_equi."""
+    return _line_opacity_equi(self, *args, general=True, **kw)
 
 
 Context.line_opacity_equi = _line_opacity_equi
+Context.line_opacity = _line_opacity
 
 
 # ------------------------------------------------------------------ statistical equilibrium

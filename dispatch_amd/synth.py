@@ -279,10 +279,14 @@ def lte_populations(atom: Atom, T, ne, totn):
     return w
 
 
-def van_regemorter(atom: Atom, T, ne, nstar):
+def van_regemorter(atom: Atom, T, ne, nstar, ion_scale=1e-2):
     """Upward collision rates (npair, ncell): van Regemorter for the lines
     (14.5 Allen Ryd/kT f max(0.2 e^-u, sqrt3/2pi E1(u)) / dE, SRC/atom_mod.f90:
-    1790-1838 with the recipe's constants) and a Seaton-type ionisation rate."""
+    1790-1838 with the recipe's constants) and a Seaton-type ionisation rate
+    1.55e13 ne T^-1/2 s e^-u / u.  Seaton's s is gbar sigma0 (0.1 times the edge
+    cross section in cm^2, ~1e-19); the default s = 1e-2 makes the continuum
+    strongly collisional, which keeps the statistical-equilibrium test cases, with
+    their randomised radiative rates, away from negative populations."""
     from scipy.special import exp1
     T = np.asarray(T, np.float64)
     sqt = np.sqrt(T)
@@ -301,5 +305,5 @@ def van_regemorter(atom: Atom, T, ne, nstar):
             f = max(atom.f[q], 1e-6 * atom.g[j - 1])
             out[q] = regem * f * np.maximum(0.2 * ex, np.sqrt(3) / (2 * np.pi) * exp1(u0)) / dev
         else:
-            out[q] = seaton * 0.1 * 0.1 / u0 * ex
+            out[q] = seaton * ion_scale / u0 * ex
     return out

@@ -187,6 +187,16 @@ typedef struct {
  */
 int m3drt_line_opacity_equi(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
 
+/*
+ * add_opacity (HOP/linelist_mod.f90:469-719): the same summation on an ascending
+ * ln(lambda) grid of any spacing (spectrum%equidistant false): per cell a forward
+ * cursor finds each line centre's nearest node, the profile is set up on the
+ * ln(lambda) abscissa, and its window is found by bisection (bisect_nearest,
+ * :1075-1095).  Every line of the list is visited (select_lines without reaches).
+ * Arguments as m3drt_line_opacity_equi; nf >= 1.
+ */
+int m3drt_line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o);
+
 /* ------------------------------------------------------------------ statistical equilibrium */
 
 /* A model atom and its cells, FP64 except the radiative rate sums.  Per-cell

@@ -339,3 +339,133 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
   }
   free(Kbuf);
 }
+
+/* bisect_nearest, HOP/linelist_mod.f90:1075-1095: the first index of llam(ws:we)
+ * (1-based) at or above x, moved one down when the node below is strictly nearer. */
+static int bisect_nearest(const double *llam, int ws, int we, double x)
+{
+  int lo = ws, hi = we;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (llam[mid - 1] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo > ws && fabs(llam[lo - 2] - x) < fabs(llam[lo - 1] - x)) lo = lo - 1;
+  return lo;
+}
+
+/*
+ * add_opacity, HOP/linelist_mod.f90:469-719, on an ascending ln(lambda) grid of any
+ * spacing, for one isotope's line list over ncell cells (no chunking, reaches or
+ * per-line windows: every line is selected, select_lines :1110-1142 without
+ * reaches).  Per cell a cursor marches forward to each (ascending) line centre and
+ * takes the nearer node (:578-590); the profile is set up on the ln(lambda)
+ * abscissa itself (qlg_bot_cell(1/idf), the physical damping, qlg_bot_line) and its
+ * window found by bisection unless it covers the whole grid (:679-689).  Arrays as
+ * orc_add_opacity_equi.
+ */
+void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int ncell, float *chi, float *eta,
+                     const float *cont, const float *logref, const float *vproj, const float *inv_df,
+                     const float *vbar2, const float *e_kT, const float *ne, const float *tothi,
+                     const float *stark_pe, int np, const float *lnbase, int nlev, const float *dep,
+                     int do_nlte, const float *one_m_eps, float *chi_scat, int do_scat, float tol,
+                     float line_reject_frac)
+{
+  const float sqrt_pi = 1.7724538509f, ex_cutoff = -85.0f, voigt_peak = 0.5641895835f;
+  const int nl = ll->nline;
+  (void)nlev;
+  (void)np;
+  if (nl == 0 || nf < 1) return;
+  orc_qlg_init();
+  for (int cidx = 0; cidx < ncell; cidx++) {
+    float *chic = chi + (size_t)nf * cidx, *etac = eta + (size_t)nf * cidx;
+    const float *contc = cont + (size_t)nf * cidx, *lrc = logref + (size_t)nf * cidx;
+    int f0_prev = -1, fc = 1;
+    const int ws = 1, we = nf;
+    float refl_c = 0, lref_c = 0;
+    const float idf = inv_df[cidx];
+    orc_qlg_cell qw;
+    orc_qlg_bot_cell(1.0f / idf, &qw);
+    const float unsold_c = powf(vbar2[cidx], 0.3f) * tothi[cidx];
+    const float lvb2 = logf(vbar2[cidx]);
+    const float ekt = e_kT[cidx], ne_c = ne[cidx], tothi_c = tothi[cidx];
+    const float starkpe = stark_pe[cidx], vproj_c = vproj[cidx];
+    for (int l = 0; l < nl; l++) {
+      const double llc = ll->llam0[l] - (double)vproj_c;
+      while (fc < nf && !(llam[fc - 1] >= llc)) fc++;
+      int f0;
+      if (fc > 1) f0 = fabs(llam[fc - 1] - llc) < fabs(llam[fc - 2] - llc) ? fc : fc - 1;
+      else f0 = 1;
+      const float ex = ll->lower_ev[l] * ekt;
+      const int ip = ll->ipart ? ll->ipart[l] : 1;
+      const float arg = lnbase[cidx + (size_t)ncell * (ip - 1)] + ll->gpref[l] - ex;
+      float Rij, ref, a, hw, bi, bj;
+      int fa, fb, nlte;
+      if (arg < ex_cutoff) continue;
+      if (f0 != f0_prev) {
+        f0_prev = f0;
+        refl_c = tol * contc[f0 - 1];
+        const float flo = fmaxf(tol, line_reject_frac) * tol * chic[f0 - 1];
+        if (flo > refl_c) {
+          refl_c = flo;
+          lref_c = logf(flo * sqrt_pi);
+        } else {
+          lref_c = lrc[f0 - 1];
+        }
+      }
+      if (arg <= lref_c) continue;
+      Rij = expf(arg);
+      ref = refl_c;
+      if (voigt_peak * Rij <= ref) continue;
+      nlte = do_nlte && ll->lev_i[l] > 0 && ll->lev_j[l] > 0;
+      if (nlte) {
+        bi = dep[cidx + (size_t)ncell * (ll->lev_i[l] - 1)];
+        bj = dep[cidx + (size_t)ncell * (ll->lev_j[l] - 1)];
+      } else {
+        bi = 1.0f;
+        bj = 1.0f;
+      }
+      a = ll->grad[l];
+      if (ll->unsold[l])
+        a = a + ll->gw1[l] * unsold_c;
+      else
+        a = a + ll->gw1[l] * expf(ll->gw2[l] * lvb2) * tothi_c;
+      if (ll->gstk[l] >= 0.0f)
+        a = a + ll->gstk[l] * ne_c;
+      else
+        a = a - ll->gstk[l] * starkpe;
+      ll->used[l] = 1;
+      {
+        const float st = voigt_peak * Rij / contc[f0 - 1];
+        if (st > ll->strength[l]) ll->strength[l] = st;
+      }
+      orc_qlg_coef qc;
+      orc_qlg_bot_line(&qw, a, Rij * qw.D, ref, arg - lref_c, &qc, &hw);
+      if (hw > ll->reach[l]) ll->reach[l] = hw;
+      {
+        const double dl = (double)hw;
+        const double span = fmax(llam[we - 1] - llc, llc - llam[ws - 1]);
+        if (dl >= span) {
+          fa = ws;
+          fb = we;
+        } else {
+          fa = bisect_nearest(llam, ws, f0, llc - dl);
+          fb = bisect_nearest(llam, f0, we, llc + dl);
+        }
+      }
+      for (int f = fa; f <= fb; f++) {
+        const float lte = orc_qlg_bot_line_eval(&qc, (float)(llam[f - 1] - llc));
+        if (do_scat && !nlte) {
+          chic[f - 1] = chic[f - 1] + lte;
+          etac[f - 1] = etac[f - 1] + lte;
+          chi_scat[(size_t)nf * cidx + f - 1] = chi_scat[(size_t)nf * cidx + f - 1]
+                                                + one_m_eps[(size_t)nf * cidx + f - 1] * lte;
+        } else {
+          chic[f - 1] = chic[f - 1] + lte * bi;
+          etac[f - 1] = etac[f - 1] + lte * bj;
+        }
+      }
+      f0_prev = -1;
+    }
+  }
+}

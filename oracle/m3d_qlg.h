@@ -42,6 +42,14 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
                           const float *stark_pe, int np, const float *lnbase, int nlev,
                           const float *dep, int do_nlte, const float *one_m_eps, float *chi_scat,
                           int do_scat, float tol, float line_reject_frac);
+/* add_opacity (HOP/linelist_mod.f90:469-719) on an ascending grid of any spacing;
+ * arguments as orc_add_opacity_equi */
+void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int ncell, float *chi, float *eta,
+                     const float *cont, const float *logref, const float *vproj, const float *inv_df,
+                     const float *vbar2, const float *e_kT, const float *ne, const float *tothi,
+                     const float *stark_pe, int np, const float *lnbase, int nlev, const float *dep,
+                     int do_nlte, const float *one_m_eps, float *chi_scat, int do_scat, float tol,
+                     float line_reject_frac);
 
 #ifdef __cplusplus
 }

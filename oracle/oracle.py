@@ -255,6 +255,8 @@ def _qlg_declare():
                                        + [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                           C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float])
     L.orc_add_opacity_equi.restype = None
+    L.orc_add_opacity.argtypes = L.orc_add_opacity_equi.argtypes
+    L.orc_add_opacity.restype = None
     L._qlg_ok = True
     return L
 
@@ -303,8 +305,15 @@ def qlg_line_sweep(D, gL, S, ref, lnq, llam, llc, pos, idl, cap):
     return fa.value, fb.value, hw.value, K
 
 
+def add_opacity(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None, do_nlte=False,
+                one_m_eps=None, chi_scat=None, tol=1e-4, line_reject_frac=0.0):
+    """Restated add_opacity (a grid of any spacing); arguments as add_opacity_equi."""
+    return add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep, do_nlte, one_m_eps,
+                            chi_scat, tol, line_reject_frac, _general=True)
+
+
 def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None, do_nlte=False,
-                     one_m_eps=None, chi_scat=None, tol=1e-4, line_reject_frac=0.0):
+                     one_m_eps=None, chi_scat=None, tol=1e-4, line_reject_frac=0.0, _general=False):
     """Restated add_opacity_equi.  lines: dict of SoA arrays (llam0, lower_ev, gpref,
     grad, gw1, gw2, gstk, unsold, lev_i, lev_j, optional ipart); chi/eta/cont/logref
     (ncell, nf) float32, updated copies returned; cells: dict of per-cell arrays
@@ -334,7 +343,8 @@ def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=Non
     np_ = lnb.shape[0]
     dep = np.ones((1, ncell), np.float32) if dep is None else np.ascontiguousarray(dep, np.float32)
     csc = None if chi_scat is None else np.array(chi_scat, np.float32, copy=True, order="C")
-    L.orc_add_opacity_equi(C.byref(ll), nf, np.ascontiguousarray(llam, np.float64), ncell,
+    fn = L.orc_add_opacity if _general else L.orc_add_opacity_equi
+    fn(C.byref(ll), nf, np.ascontiguousarray(llam, np.float64), ncell,
                            chi.ctypes.data_as(C.c_void_p), eta.ctypes.data_as(C.c_void_p),
                            arr(cont, np.float32), arr(logref, np.float32),
                            *[arr(cells[k], np.float32) for k in ("vproj", "idf", "vb2", "ekt", "ne",

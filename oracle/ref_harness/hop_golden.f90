@@ -6,8 +6,8 @@
 !   hop_golden.x <kind> <infile> <outfile>
 !
 ! kinds: expneg (HOP/exp_neg.inc), qlgbot / qlgref / qlgline (HOP/qlg_mod.f90),
-! lineq (HOP/op_math_mod.f90 lineq_no_pivot), opac_equi (HOP/linelist_mod.f90
-! add_opacity_equi).  All files are raw little-endian streams.
+! lineq (HOP/op_math_mod.f90 lineq_no_pivot), opac_equi / opac_gen (HOP/linelist_mod.f90
+! add_opacity_equi / add_opacity on a grid of any spacing).  All files are raw little-endian streams.
 !===============================================================================
 module expneg_h
   implicit none
@@ -20,7 +20,7 @@ program hop_golden
   use qlg
   use op_math_mod, only: lineq_no_pivot
   use line_types_mod, only: linelist_type
-  use linelist_mod, only: add_opacity_equi
+  use linelist_mod, only: add_opacity_equi, add_opacity
   implicit none
   character(len=256) :: kind, fin, fout
   integer :: ui, uo
@@ -42,7 +42,9 @@ program hop_golden
   case ('lineq')
     call do_lineq()
   case ('opac_equi')
-    call do_opac_equi()
+    call do_opac_equi(.false.)
+  case ('opac_gen')
+    call do_opac_equi(.true.)
   case default
     error stop 'hop_golden: unknown kind'
   end select
@@ -125,7 +127,8 @@ contains
   ! One isotope's line list over a block of cells, as add_particle_opacity hands
   ! it to the kernel (HOP/linelist_mod.f90:1040-1046).
   !-----------------------------------------------------------------------------
-  subroutine do_opac_equi()
+  subroutine do_opac_equi(general)
+    logical, intent(in) :: general
     type(linelist_type) :: ll
     integer :: nl, nf, m(3), np, nlev, flags, l
     logical :: do_nlte, has_ipart, do_scat
@@ -170,8 +173,13 @@ contains
     else
       allocate(ome(0,0,0,0), csc(0,0,0,0))
     end if
-    call add_opacity_equi(ll, llam, chi, eta, cont, logref, vproj, idf, vb2, ekt, lnbase, ne, &
-                          tothi, spe, dep, do_nlte, ome, csc, do_scat)
+    if (general) then
+      call add_opacity(ll, llam, chi, eta, cont, logref, vproj, idf, vb2, ekt, lnbase, ne, &
+                       tothi, spe, dep, do_nlte, ome, csc, do_scat)
+    else
+      call add_opacity_equi(ll, llam, chi, eta, cont, logref, vproj, idf, vb2, ekt, lnbase, ne, &
+                            tothi, spe, dep, do_nlte, ome, csc, do_scat)
+    end if
     write(uo) chi, eta
     write(uo) (merge(1, 0, ll%used(l)), l = 1, nl)
     write(uo) ll%strength, ll%reach

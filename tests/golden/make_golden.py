@@ -40,7 +40,12 @@ def run(kind, payload: bytes) -> bytes:
         return open(fo, "rb").read()
 
 
+ONLY: set = set()
+
+
 def save(name, **arrays):
+    if ONLY and name not in ONLY:
+        return
     prov = provenance()
     h = hashlib.sha256()
     for k in sorted(arrays):
@@ -128,10 +133,16 @@ def gen_lineq(rng):
     save("hop_lineq.npz", in_A=np.array(As), in_b=np.array(bs), x=x)
 
 
-def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4):
+def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4, general=False):
+    """general: add_opacity on a grid whose steps vary from 0.3 to 1.7 of the mean,
+    else add_opacity_equi on an equidistant grid."""
     ncell = int(np.prod(m))
     dl = np.log(5010 / 5000) / (nf - 1)
-    llam = np.log(5000.0) + np.arange(nf) * dl
+    if general:
+        steps = dl * rng.uniform(0.3, 1.7, nf - 1)
+        llam = np.log(5000.0) + np.concatenate([[0.0], np.cumsum(steps)])
+    else:
+        llam = np.log(5000.0) + np.arange(nf) * dl
     llam0 = np.sort(rng.uniform(llam[0] - 40 * dl, llam[-1] + 40 * dl, nl))
     lower_ev = rng.uniform(0, 5, nl).astype(np.float32)
     gpref = (rng.uniform(-3, 0, nl) * np.log(10) - 37.7).astype(np.float32)
@@ -179,7 +190,7 @@ def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4):
         ome = rng.uniform(0, 1, (nf, ncell)).astype(np.float32)
         csc = (cont * 0.1).astype(np.float32)
         payload += ome.T.tobytes() + csc.T.tobytes()
-    out = run("opac_equi", payload)
+    out = run("opac_gen" if general else "opac_equi", payload)
     n2 = nf * ncell
     o = np.frombuffer(out, np.uint8)
     chi_o = np.frombuffer(out[:4 * n2], np.float32).reshape(ncell, nf)
@@ -202,8 +213,12 @@ def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4):
 
 
 def main():
+    """python make_golden.py [name ...]: every fixture, or only the named ones (the
+    seeded draws are the same either way: each fixture has its own generator)."""
     if not os.path.exists(EXE):
         sys.exit(f"{EXE} missing: run oracle/build_ref.sh first")
+    only = set(sys.argv[1:])
+    ONLY.update(only)
     rng = np.random.default_rng(20261202)
     gen_expneg(rng)
     gen_qlg(rng)
@@ -211,6 +226,11 @@ def main():
     gen_opac(rng, "hop_opac_equi_lte.npz", flags=0)
     gen_opac(rng, "hop_opac_equi_nlte.npz", flags=1 | 2, np_=2)
     gen_opac(rng, "hop_opac_equi_scat.npz", flags=4, nl=30, nf=200, m=(3, 2, 2))
+    for i, (name, kw) in enumerate([("hop_opac_gen_lte.npz", dict(flags=0)),
+                                    ("hop_opac_gen_nlte.npz", dict(flags=1 | 2, np_=2)),
+                                    ("hop_opac_gen_scat.npz", dict(flags=4, nl=30, nf=200, m=(3, 2, 2)))]):
+        if not only or name in only:
+            gen_opac(np.random.default_rng(20261210 + i), name, general=True, **kw)
 
 
 if __name__ == "__main__":

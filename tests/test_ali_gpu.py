@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def small(**kw):
     from dispatch_amd.ali import AliConfig
-    base = dict(nx=6, ny=6, nz=16, nq=16, nline=6, ncont=3, lines_per_window=3, nphi=2, ncont_f=8)
+    base = dict(nx=6, ny=6, nz=48, nq=16, nbound=4, nline=6, ncont=4, lines_per_window=3, nphi=2, ncont_f=8)
     base.update(kw)
     return AliConfig(**base)
 
@@ -20,7 +20,7 @@ def test_ali_iteration_matches_oracle(ctx):
     from dispatch_amd.ali import NlteProblem
     from ali_cases import oracle_rates, oracle_update
     p = NlteProblem(ctx, small())
-    print("ali iteration vs oracle, 6x6x16, 6 lines, 3 continua, 8 rays", flush=True)
+    print("ali iteration vs oracle, 6x6x48, 5 levels, 6 lines, 4 continua, 8 rays", flush=True)
     for it in range(2):
         gij, gji = p.rates()
         torch.cuda.synchronize()

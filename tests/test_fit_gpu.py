@@ -32,25 +32,26 @@ def test_abundance_gradient_matches_fp64_differences(problem):
 
 
 def test_temperature_gradient_matches_fp64_differences(problem):
+    """Along a random per-cell direction v (unit normal, scaled to 1 K rms): the
+    central difference of the FP64 loss in T + eps v against <dL/dT, v>.  (Single
+    cells are a poorer check: the loss is piecewise smooth through the upwind clamp,
+    and a one-cell FP64 difference moves by up to 0.5 % with the step.)"""
     p = problem
     d0 = np.zeros(4)
     print("fit temperature gradient", flush=True)
     loss, _, gT = p.loss_and_grad(d0, want_T=True)
-    flat = gT.abs().flatten()
-    cells = torch.topk(flat, 3).indices.tolist()
+    g = torch.Generator(device="cuda").manual_seed(11)
+    v = torch.randn(p.box.T.shape, generator=g, dtype=torch.float64, device="cuda")
     T0 = p.box.T.clone()
     Tg0 = p.box.T_ghost_bot.clone()
-    h = 2.0
-    for c in cells:
-        z, rem = divmod(c, p.box.ny * p.box.nx)
-        y, x = divmod(rem, p.box.nx)
+    want = float((gT * v).sum())
+    for eps in (1.0, 0.25):
         vals = []
         for sgn in (1, -1):
-            p.box.T.copy_(T0)
-            p.box.T[z, y, x] += sgn * h
+            p.box.T.copy_(T0 + sgn * eps * v)
             p.box.T_ghost_bot.copy_(2 * p.box.T[0] - p.box.T[1])
             vals.append(float(p.loss_of_flux(p.flux_fp64(d0))))
-        fd = (vals[0] - vals[1]) / (2 * h)
+        fd = (vals[0] - vals[1]) / (2 * eps)
         p.box.T.copy_(T0)
         p.box.T_ghost_bot.copy_(Tg0)
-        assert abs(float(gT[z, y, x]) - fd) <= 1e-3 * abs(fd), ((z, y, x), float(gT[z, y, x]), fd)
+        assert abs(want - fd) <= 1e-3 * abs(fd), (eps, want, fd)

@@ -96,18 +96,22 @@ def test_lineq_no_pivot_reference_solution():
 
 
 @pytest.mark.parametrize("name", ["hop_opac_equi_lte.npz", "hop_opac_equi_nlte.npz",
-                                  "hop_opac_equi_scat.npz"])
+                                  "hop_opac_equi_scat.npz", "hop_opac_gen_lte.npz", "hop_opac_gen_nlte.npz",
+                                  "hop_opac_gen_scat.npz"])
 def test_add_opacity_equi_matches_reference_build(name):
+    """add_opacity_equi, and add_opacity on grids whose steps vary by a factor 5
+    (the hop_opac_gen fixtures), against the reference's compiled outputs."""
     g = load(name)
+    fn = O.add_opacity if "_gen_" in name else O.add_opacity_equi
     flags = int(g["in_flags"])
     lines = {k: g["in_" + k] for k in ("llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk",
                                         "unsold", "lev_i", "lev_j")}
     lines["ipart"] = g["in_ipart"] if flags & 2 else None
     cells = {k: g["in_" + k] for k in ("vproj", "idf", "vb2", "ekt", "ne", "tothi", "spe")}
-    r = O.add_opacity_equi(lines, g["in_llam"], g["in_chi"], g["in_eta"], g["in_cont"], g["in_logref"],
-                           cells, g["in_lnbase"], dep=g["in_dep"], do_nlte=bool(flags & 1),
-                           one_m_eps=g["in_one_m_eps"] if flags & 4 else None,
-                           chi_scat=g["in_chi_scat"] if flags & 4 else None)
+    r = fn(lines, g["in_llam"], g["in_chi"], g["in_eta"], g["in_cont"], g["in_logref"],
+           cells, g["in_lnbase"], dep=g["in_dep"], do_nlte=bool(flags & 1),
+           one_m_eps=g["in_one_m_eps"] if flags & 4 else None,
+           chi_scat=g["in_chi_scat"] if flags & 4 else None)
     assert np.array_equal(r["used"], g["used"])
     assert r["used"].sum() > 5                                   # the case exercises the kernel
     assert np.allclose(r["chi"], g["chi"], rtol=2e-6, atol=0)

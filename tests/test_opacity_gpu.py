@@ -25,7 +25,7 @@ def dev(a, dt=None):
     return (t.to(dt) if dt is not None else t).cuda()
 
 
-def run(ctx, g, layout, flags):
+def run(ctx, g, layout, flags, general=False):
     lines = {"llam0": dev(g["in_llam0"], torch.float64)}
     lines.update({k: dev(g["in_" + k], torch.float32) for k in LINE_KEYS_F})
     lines.update({k: dev(g["in_" + k], torch.int32) for k in LINE_KEYS_I})
@@ -43,7 +43,7 @@ def run(ctx, g, layout, flags):
                           sp(g["in_logref"]), {k: dev(g["in_" + k]) for k in CELL_KEYS},
                           dev(g["in_lnbase"]), dep=dev(g["in_dep"]), nlte=bool(flags & 1),
                           one_m_eps=ome, chi_scat=csc, layout=layout, used=used, strength=strength,
-                          reach=reach)
+                          reach=reach, general=general)
     torch.cuda.synchronize()
     back = (lambda t: t.cpu().numpy().T) if layout == "native" else (lambda t: t.cpu().numpy())
     out = dict(chi=back(chi), eta=back(eta), used=used.cpu().numpy(), strength=strength.cpu().numpy(),
@@ -54,13 +54,16 @@ def run(ctx, g, layout, flags):
 
 
 @pytest.mark.parametrize("name", ["hop_opac_equi_lte.npz", "hop_opac_equi_nlte.npz",
-                                  "hop_opac_equi_scat.npz"])
+                                  "hop_opac_equi_scat.npz", "hop_opac_gen_lte.npz", "hop_opac_gen_nlte.npz",
+                                  "hop_opac_gen_scat.npz"])
 @pytest.mark.parametrize("layout", ["native", "reference"])
 def test_opacity_matches_reference_build(ctx, name, layout):
+    """m3drt_line_opacity_equi on the equidistant fixtures, m3drt_line_opacity
+    (add_opacity) on the hop_opac_gen ones, whose grid steps vary by a factor 5."""
     g = np.load(os.path.join(GOLD, name))
     flags = int(g["in_flags"])
     print(f"opacity {name} {layout}", flush=True)
-    r = run(ctx, g, layout, flags)
+    r = run(ctx, g, layout, flags, general="_gen_" in name)
     assert np.array_equal(r["used"], g["used"])
     for k in ("chi", "eta", "strength", "reach") + (("chi_scat",) if flags & 4 else ()):
         err = np.max(np.abs(r[k] - g[k]) / np.maximum(np.abs(g[k]), 1e-30))
