@@ -73,6 +73,7 @@ class AliConfig:
     ne_scale: float = 1.0     # multiplies the electron density (collisional coupling)
     max_iter: int = 40
     conv: float = 1e-4
+    ng: bool = True           # Ng acceleration (the reference's automatic trigger)
 
 
 @dataclass
@@ -86,13 +87,104 @@ class _Window:
     const: dict = field(default_factory=dict)
 
 
+class NgAcceleration:
+    """Ng's three-difference extrapolation in ln b (SRC/ng_mod.f90, applied as
+    atom_type%ng_acceleration does, SRC/atom_mod.f90:2469-2554): the Gram matrix of
+    the differences of the last four iterates is accumulated each iteration and
+    promoted at the next, whose gate fires only in the iteration's asymptotic
+    regime (stationary contraction rate between 0.5 and 1, few limited corrections,
+    a well-conditioned fit, headroom left); the extrapolated b is renormalised to
+    the total density and the step vetoed if it moves a level by more than a factor
+    11.  The gate's constants are the reference's &nlte_params defaults."""
+    rate_min, rate_tol, clamp_max, min_iters_left = 0.5, 0.15, 1e-3, 3
+    cond_min, max_change, min_gap = 1e-6, 10.0, 4
+
+    def __init__(self):
+        self.d_cur = np.zeros((3, 3))
+        self.d_done = np.zeros((3, 3))
+        self.s_cur = self.s_done = 0.0
+        self.s_prev = -1.0
+        self.r = self.r_prev = -1.0
+        self.nclamp_cur = self.ntot_cur = self.nclamp_done = self.ntot_done = 0
+        self.iter_fired = -(1 << 30)
+
+    def promote(self):
+        self.d_done, self.s_done = self.d_cur, self.s_cur
+        self.nclamp_done, self.ntot_done = self.nclamp_cur, self.ntot_cur
+        self.d_cur, self.s_cur = np.zeros((3, 3)), 0.0
+        self.nclamp_cur = self.ntot_cur = 0
+        if self.s_done > 0:
+            self.r_prev = self.r
+            self.r = math.sqrt(self.s_done / self.s_prev) if self.s_prev > 0 else -1.0
+            self.s_prev = self.s_done
+
+    def condition(self):
+        d = self.d_done
+        scale = d[1, 1] * d[2, 2]
+        return (d[1, 1] * d[2, 2] - d[1, 2] * d[2, 1]) / scale if scale > 0 else 0.0
+
+    def should_fire(self, it, err, convlim):
+        if self.s_done <= 0 or self.s_prev <= 0:
+            return False
+        if it - self.iter_fired < self.min_gap or self.ntot_done <= 0:
+            return False
+        if self.nclamp_done / self.ntot_done > self.clamp_max:
+            return False
+        if self.r <= 0 or self.r_prev <= 0 or abs(self.r - self.r_prev) > self.rate_tol * self.r:
+            return False
+        if self.r < self.rate_min or self.r >= 1.0:
+            return False
+        if abs(err) < convlim * self.r ** (-self.min_iters_left):
+            return False
+        return self.condition() > self.cond_min
+
+    def accumulate(self, b, oldb, nclamp):
+        n1, n2, n3 = (torch.log(o) for o in oldb)
+        n4 = torch.log(b)
+        d0 = n4 - n3
+        d1 = d0 - (n3 - n2)
+        d2 = d0 - (n2 - n1)
+        dot = lambda u, v: float((u * v).sum())
+        self.d_cur[0, 1] += dot(d0, d1)
+        self.d_cur[0, 2] += dot(d0, d2)
+        self.d_cur[1, 1] += dot(d1, d1)
+        self.d_cur[1, 2] += dot(d1, d2)
+        self.d_cur[2, 1] = self.d_cur[1, 2]
+        self.d_cur[2, 2] += dot(d2, d2)
+        self.s_cur += dot(d0, d0)
+        self.nclamp_cur += int(nclamp)
+        self.ntot_cur += b.numel()
+
+    def extrapolate(self, b, oldb, nstar, totn):
+        """Apply the step to b in place; returns the largest relative change, or None
+        when vetoed."""
+        d = self.d_done
+        den = d[1, 1] * d[2, 2] - d[1, 2] * d[2, 1]
+        a = (d[0, 1] * d[2, 2] - d[0, 2] * d[2, 1]) / den
+        bb = (d[0, 2] * d[1, 1] - d[0, 1] * d[1, 2]) / den
+        n2, n3, n4 = torch.log(oldb[1]), torch.log(oldb[2]), torch.log(b)
+        e = (1.0 - a - bb) * n4 + a * n3 + bb * n2
+        worst = float((e - n4).abs().max())
+        if not (math.exp(worst) - 1.0 <= self.max_change and math.isfinite(a) and math.isfinite(bb)):
+            return None
+        bn = torch.exp(e)
+        bn *= totn[None, :] / (bn * nstar).sum(dim=0, keepdim=True)
+        change = float((bn / b - 1.0).abs().max())
+        b.copy_(bn)
+        return change
+
+
 class NlteProblem:
-    def __init__(self, ctx: m3drt.Context, cfg: AliConfig = AliConfig(), sweep_dtype=torch.float32):
-        self.ctx, self.cfg, self.sweep_dtype = ctx, cfg, sweep_dtype
-        box = synth.make_box(cfg.nx, cfg.ny, cfg.nz, nf=2, seed=cfg.box_seed)
+    def __init__(self, ctx: m3drt.Context, cfg: AliConfig = AliConfig(), sweep_dtype=torch.float32,
+                 device="cuda"):
+        """ctx: the m3drt context whose entry points every step calls; device: where
+        the arrays live (the library's device; another one only for a context object
+        with the same methods)."""
+        self.ctx, self.cfg, self.sweep_dtype, self.device = ctx, cfg, sweep_dtype, device
+        box = synth.make_box(cfg.nx, cfg.ny, cfg.nz, nf=2, seed=cfg.box_seed, device=device)
         self.box = box
         self.ncell = ncell = cfg.nx * cfg.ny * cfg.nz
-        dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+        dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
         T = box.T.cpu().numpy().reshape(-1)
         c5 = box.chi500.double().cpu().numpy().reshape(-1)
         # densities from the tau500 scale: n_H near 1e17 cm^-3 where tau500 = 1, and
@@ -113,13 +205,13 @@ class NlteProblem:
         # Seaton's ionisation with gbar = 0.1 and the edge cross section
         self.cup = dev(synth.van_regemorter(atom, T, ne, nstar, ion_scale=0.1 * float(atom.sigma0[0])),
                        torch.float64)
-        self.atom_dev = m3drt.AtomOnDevice(atom)
+        self.atom_dev = m3drt.AtomOnDevice(atom) if device == "cuda" else atom
         # per cell: Doppler width (mass 56 plus 1 km/s), damping, e/kT
         vth = np.sqrt(2 * synth.K_BOLTZ * T / (56.0 * synth.AMU) + 1e5 ** 2) / synth.C_LIGHT
         self.inv_df = dev(1.0 / vth, torch.float32)
         self.e_kT32 = dev(synth.EV_K / T, torch.float32)
         self.adamp = dev(np.tile((2e-3 + 1e-2 * (ne / 1e14) ** 0.3)[None, :], (atom.nline, 1)), torch.float32)
-        self.pop_ratio = torch.ones(ncell, dtype=torch.float64, device="cuda")
+        self.pop_ratio = torch.ones(ncell, dtype=torch.float64, device=self.device)
         self.vz = box.vz.reshape(-1)
         self.vec, self.wj = synth.gauss_rays(cfg.nphi)
         self.cu = [synth.chi500_upwind(box, v) for v in self.vec]
@@ -128,6 +220,9 @@ class NlteProblem:
                         [list(range(w, min(atom.nline, w + cfg.lines_per_window)))
                          for w in range(0, atom.nline, cfg.lines_per_window)]]
         self.history = []
+        self.ng = NgAcceleration() if cfg.ng else None
+        self._oldb = []
+        self._iter = 0
         self._continuum()
 
     # ---------------------------------------------------------------- spectra
@@ -136,7 +231,7 @@ class NlteProblem:
         B = beta / (1 - beta), FP32 (nf, ncell); and B of the ghost plane below."""
         T = self.temp[None, :]
         c5 = self.box.chi500.double().reshape(1, -1)
-        nu = torch.as_tensor(nu, dtype=torch.float64, device="cuda")[:, None]
+        nu = torch.as_tensor(nu, dtype=torch.float64, device=self.device)[:, None]
         chi_c = c5 * (0.7 + 0.3 * (6e14 / nu) ** 3)
         beta = torch.exp(-H_KT * nu / T).clamp(min=np.finfo(np.float32).tiny)
         B = beta / (1.0 - beta)
@@ -144,12 +239,17 @@ class NlteProblem:
         return chi_c.float(), beta.float(), B.float(), (bg / (1.0 - bg)).float().contiguous()
 
     def _window(self, lines):
+        """A window of lines: each line's nq frequencies with the reference's abscissa
+        q = 1 - lambda/lambda0 (SRC/atom_mod.f90:2083) uniform in +-qmax, its rate
+        weights wnu = trapezoid(nu)/nu and profile weights wq = trapezoid(q)
+        (:2086-2094)."""
         cfg, atom = self.cfg, self.atom
         q = np.linspace(-cfg.qmax, cfg.qmax, cfg.nq)
         q32 = q.astype(np.float32)
         wq = _trapezoid(q32.astype(np.float64))
         nline, nf = len(lines), len(lines) * cfg.nq
-        nu = np.concatenate([atom.nu0[l] * (1.0 + q) for l in lines])
+        nus = [atom.nu0[l] / (1.0 - q) for l in lines]
+        nu = np.concatenate(nus)
         qa = np.zeros((nline, nf), np.float32)
         wqa = np.zeros((nline, nf))
         wnu = np.zeros((nline, nf))
@@ -160,7 +260,7 @@ class NlteProblem:
             iblue[k], ired[k] = fs + 1, fs + cfg.nq
             qa[k, fs:fs + cfg.nq] = q32
             wqa[k, fs:fs + cfg.nq] = wq
-            wnu[k, fs:fs + cfg.nq] = atom.nu0[l] * wq
+            wnu[k, fs:fs + cfg.nq] = _trapezoid(nus[k]) / nus[k]
         li = atom.ti[lines].astype(np.int32)
         lj = atom.tj[lines].astype(np.int32)
         gi, gj = atom.g[li - 1], atom.g[lj - 1]
@@ -168,7 +268,7 @@ class NlteProblem:
         # opacity per unit profile in (nu - nu0)/nu0: chi_line = n_i Bij phi (b_i - b_j beta)
         Bij = PI_E2_MC * atom.f[lines] / atom.nu0[lines]
         chi_c, beta, B, gB = self._background(nu)
-        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BB_DTYPES[k], device="cuda")
+        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BB_DTYPES[k], device=self.device)
         const = dict(iblue=d(iblue, "iblue"), ired=d(ired, "ired"), lev_i=d(li, "lev_i"), lev_j=d(lj, "lev_j"),
                      Aij=d(Aij, "Aij"), Bij=d(Bij, "Bij"), lower_off=d(np.zeros(nline), "lower_off"),
                      q=d(qa, "q"), wnu=d(wnu, "wnu"), wq=d(wqa, "wq"), g=d(atom.g, "g"))
@@ -189,7 +289,7 @@ class NlteProblem:
             gS = ghost_B.to(chi.dtype)
             gI = gS
         else:
-            gchi = torch.zeros((nf, cfg.nx * cfg.ny), dtype=chi.dtype, device="cuda")
+            gchi = torch.zeros((nf, cfg.nx * cfg.ny), dtype=chi.dtype, device=self.device)
             gS = gI = gchi
         pl = lambda t: t.view(nf, cfg.ny, cfg.nx)
         self.ctx.sc_sweep(chi.view(shp), S.view(shp), pl(gchi), pl(gS), pl(gI), box.chi500, self.cu[r],
@@ -210,7 +310,8 @@ class NlteProblem:
         for j in range(nc):
             fs = j * cfg.ncont_f
             al[j, fs:fs + cfg.ncont_f] = atom.sigma0[j] * (atom.nu0[nl + j] / nus[j]) ** 3
-            wnu[j, fs:fs + cfg.ncont_f] = _trapezoid(nus[j])
+            # 4 pi / (h nu) dnu (SRC/atom_mod.f90:2053-2055)
+            wnu[j, fs:fs + cfg.ncont_f] = _trapezoid(nus[j]) / (synth.H_PLANCK / (4 * math.pi) * nus[j])
         chi_c, beta, B, gB = self._background(nu)
         self.cont = dict(chi_c=chi_c, B=B, ghost_B=gB)
         dt = self.sweep_dtype
@@ -219,7 +320,7 @@ class NlteProblem:
         PsiJ = torch.zeros_like(chi)
         for r in range(len(self.vec)):
             self._sweep(r, chi, S, gB, J=J, PsiJ=PsiJ, get_rates=True)
-        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BF_DTYPES[k], device="cuda")
+        d = lambda a, k: torch.as_tensor(np.ascontiguousarray(a), dtype=m3drt.BF_DTYPES[k], device=self.device)
         self.bf = dict(iblue=d(np.arange(nc) * cfg.ncont_f + 1, "iblue"), ired=d((np.arange(nc) + 1) * cfg.ncont_f, "ired"),
                        lev_i=d(atom.ti[nl:], "lev_i"), lev_j=d(atom.tj[nl:], "lev_j"),
                        active=d(np.ones(nc), "active"), ev=d(atom.ev, "ev"), wnu=d(wnu, "wnu"), alpha=d(al, "alpha"),
@@ -230,8 +331,8 @@ class NlteProblem:
     def _line_rates(self, w: _Window, gij, gji):
         ctx, nc = self.ctx, self.ncell
         nf = w.nf
-        f32 = lambda: torch.empty((nf, nc), dtype=torch.float32, device="cuda")
-        i32 = lambda: torch.empty((nf, nc), dtype=torch.int32, device="cuda")
+        f32 = lambda: torch.empty((nf, nc), dtype=torch.float32, device=self.device)
+        i32 = lambda: torch.empty((nf, nc), dtype=torch.int32, device=self.device)
         xhat, ehat = f32(), f32()
         sp = dict(split_opac1=f32(), split_opac2=f32(), split_chi1=f32(), split_chi2=f32(), split_id=i32(),
                   split_chi_id=i32())
@@ -269,7 +370,7 @@ class NlteProblem:
         float32, normalised (normalize_profiles)."""
         atom = self.atom
         nrad = len(atom.ti)
-        gij = torch.zeros((nrad, self.ncell), dtype=torch.float32, device="cuda")
+        gij = torch.zeros((nrad, self.ncell), dtype=torch.float32, device=self.device)
         gji = torch.zeros_like(gij)
         for w in self.windows:
             self._line_rates(w, gij, gji)
@@ -284,15 +385,34 @@ class NlteProblem:
                                    self.cup)
 
     def iterate(self):
-        """One ALI iteration; returns the convergence measure of m3drt_rate_solve_f64."""
-        return self.update(*self.rates())
+        """One ALI iteration: the rates, the statistical-equilibrium update and, when
+        its gate fires, Ng's extrapolation; returns the update's convergence measure
+        (with 'ng', the extrapolation's largest relative change, when it was applied)."""
+        gij, gji = self.rates()
+        self._iter += 1
+        self._oldb = (self._oldb + [self.b.clone()])[-3:]
+        res = self.update(gij, gji)
+        if self.ng is not None:
+            ng = self.ng
+            ng.promote()
+            fired = False
+            if ng.should_fire(self._iter, res["maxerr"], self.cfg.conv):
+                change = ng.extrapolate(self.b, self._oldb, self.nstar, self.totn)
+                if change is not None:
+                    ng.iter_fired = self._iter
+                    res["ng"] = change
+                    fired = True
+            if len(self._oldb) == 3 and not fired:
+                ng.accumulate(self.b, self._oldb, res["nclamp"])
+        return res
 
     def solve(self, log=None):
         """Iterate to max |dn/n| < conv or max_iter; returns the iteration count."""
         for it in range(1, self.cfg.max_iter + 1):
             t0 = time.perf_counter()
             res = self.iterate()
-            torch.cuda.synchronize()
+            if self.device == "cuda":
+                torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             self.history.append((res["maxerr"], dt))
             if log:

@@ -128,9 +128,10 @@ __device__ __forceinline__ void stb(T *base, uint32_t off, T v) {
 // EXACT: every strip holds exactly LMAX rows (ny = LMAX * strips), so the cell loop
 // is straight-line code: no lane-dependent branch joins the registers of the load
 // pipeline, and the loads stay in flight instead of being waited on at every cell.
-// OUT: the outputs, fixed at compile time (OUT_J, OUT_J | OUT_PJ), or OUT_ANY for
+// OUT: the outputs, fixed at compile time (OUT_J, OUT_J | OUT_PJ, OUT_I: the
+// intensity cube alone, as the adjoint's recomputation asks), or OUT_ANY for
 // the general form that tests every output pointer at run time.
-constexpr int OUT_ANY = 0, OUT_J = 1, OUT_PJ = 2;
+constexpr int OUT_ANY = 0, OUT_J = 1, OUT_PJ = 2, OUT_I = 4;
 
 template <typename T, int LMAX, bool EXACT, int OUT, bool X_LDS, bool S_LDS, bool ETA, bool HOLD>
 __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const Sc2Args<T> a) {
@@ -178,7 +179,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) sc2_sweep(const S
     P.rc = a.rc + (int64_t)z * nxy;
     P.J = a.J ? a.J + zoff : nullptr;
     P.PJ = a.PsiJ ? a.PsiJ + zoff : nullptr;
-    P.Iout = OUT == OUT_ANY && a.Iout ? a.Iout + zoff : nullptr;
+    P.Iout = (OUT == OUT_ANY || (OUT & OUT_I)) && a.Iout ? a.Iout + zoff : nullptr;
     P.Psiout = OUT == OUT_ANY && a.Psiout ? a.Psiout + zoff : nullptr;
     P.Irem = a.Irem ? a.Irem + zoff : nullptr;
     P.esc = a.esc ? a.esc + zoff : nullptr;
@@ -401,6 +402,7 @@ static int exact_lds(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
 
 template <typename T, int L>
 static int exact_out(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
+  if (!a.J) return exact_lds<T, L, OUT_I>(ctx, a, nt, nplanes);
   if (a.PsiJ) return exact_lds<T, L, OUT_J | OUT_PJ>(ctx, a, nt, nplanes);
   return exact_lds<T, L, OUT_J>(ctx, a, nt, nplanes);
 }
@@ -435,11 +437,13 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
   const int smax = std::max(1, tmax / nx);
   int L = (ny + smax - 1) / smax;
   if (L > 32) return set_error(M3DRT_ELIMIT, "m3drt_sc_sweep: ny too large for the strip mapping");
-  // the straight-line form when only J (and PsiJ) are asked for and an instantiated
-  // strip length divides ny
+  // the straight-line form when only J (and PsiJ), or only the intensity cube, are
+  // asked for and an instantiated strip length divides ny
   int Lexact = 0;
-  const bool plain_out = !b->eta_row && !b->I_out && !(d->flags & M3DRT_SPLIT_RATES) && b->J &&
-                         !((d->flags & M3DRT_GET_RATES) && b->Psi_out);
+  const bool psi_out = (d->flags & M3DRT_GET_RATES) && (b->Psi_out || b->PsiJ);
+  const bool plain_out = !b->eta_row && !(d->flags & M3DRT_SPLIT_RATES) &&
+                         ((!b->I_out && b->J && !((d->flags & M3DRT_GET_RATES) && b->Psi_out)) ||
+                          (b->I_out && !b->J && !psi_out));
   if (plain_out && !getenv("M3DRT_NO_EXACT")) {
     for (int c : (sizeof(T) == 4 ? kExactL32 : kExactL64))
       if (c >= L && ny % c == 0 && !Lexact) Lexact = c;

@@ -15,6 +15,80 @@ def h(t):
     return t.detach().cpu().numpy()
 
 
+# ------------------------------------------------------------------ host context
+
+n = lambda t: None if t is None else t.detach().cpu().numpy()
+
+
+class OracleCtx:
+    """The entry points NlteProblem calls, each by its oracle restatement on host
+    tensors, in place: the driver's orchestration (windows, directions, Ng) runs on
+    the CPU against the checker's arithmetic."""
+    def sc_sweep(self, chi, S, gchi, gS, gI, chi500, chi500_up, g500, *, dir, ds, vec, layout, wj=0.0, J=None,
+                 PsiJ=None, I_out=None, Psi_out=None, Irem_out=None, escape_out=None, exit_I=None,
+                 split_rates=False, get_rates=None):
+        assert layout == "fmajor"
+        nf, nz, ny, nx = chi.shape
+        def work(c, g):
+            cube = np.moveaxis(n(c), 0, 3)           # (nz, ny, nx, nf)
+            gp = np.moveaxis(n(g), 0, 2)[None]       # (1, ny, nx, nf)
+            z = np.zeros_like(gp)
+            return np.ascontiguousarray(np.concatenate([gp, cube, z] if dir > 0 else [z, cube, gp]), np.float32)
+        I, psi, irem, esc = O.short_char2(work(chi, gchi), work(S, gS), work(torch.zeros_like(chi), gI), n(chi500),
+                                          n(chi500_up), n(g500), dir, ds, vec, split_rates=split_rates,
+                                          get_rates=True)
+        back = lambda a: torch.from_numpy(np.ascontiguousarray(np.moveaxis(a, 3, 0)))
+        Ii = back(I[1:-1])
+        if J is not None: J.add_(float(np.float32(wj)) * Ii.to(J.dtype))
+        if PsiJ is not None: PsiJ.add_(float(np.float32(wj)) * back(psi).to(PsiJ.dtype))
+        if I_out is not None: I_out.copy_(Ii)
+        if Psi_out is not None: Psi_out.copy_(back(psi))
+        if Irem_out is not None: Irem_out.copy_(back(irem))
+        if escape_out is not None: escape_out.copy_(back(esc))
+
+    def convert_to_work_units(self, chi_c, xhat, ehat, beta):
+        x, e = O.convert_to_work_units(n(chi_c), n(ehat), n(beta))
+        xhat.copy_(torch.from_numpy(x)); ehat.copy_(torch.from_numpy(e))
+
+    def bb_opacity(self, d, xhat, ehat, wmu=0.0, split=None, split_offset=0):
+        dn = {k: n(v) for k, v in d.items()}
+        sp = {k: n(v) for k, v in split.items()}
+        r = O.bb_opacity(dn, n(xhat), n(ehat), wmu=wmu, split=sp, split_offset=split_offset)
+        xhat.copy_(torch.from_numpy(r["xhat"])); ehat.copy_(torch.from_numpy(r["ehat"]))
+        for k in split:
+            split[k].copy_(torch.from_numpy(np.asarray(r[k])))
+
+    def work_to_transfer(self, xhat, ehat, beta, split_opac1=None, split_opac2=None):
+        s1, s2 = np.ascontiguousarray(n(split_opac1)), np.ascontiguousarray(n(split_opac2))
+        c, s, _ = O.work_to_transfer(n(xhat), n(ehat), n(beta), split_opac1=s1, split_opac2=s2)
+        xhat.copy_(torch.from_numpy(c)); ehat.copy_(torch.from_numpy(s))
+        split_opac1.copy_(torch.from_numpy(s1)); split_opac2.copy_(torch.from_numpy(s2))
+
+    def bb_rates(self, d, wmu, wtot, gij, gji, precond=False, own_norm=False, largest=False, split=None,
+                 split_offset=0):
+        out = O.bb_rates({k: n(v) for k, v in d.items()}, wmu, wtot, n(gij), n(gji), precond=precond,
+                         own_norm=own_norm, largest=largest, kahan=True,
+                         split={k: n(v) for k, v in split.items()}, split_offset=split_offset)
+        gij.copy_(torch.from_numpy(out[0])); gji.copy_(torch.from_numpy(out[1]))
+
+    def bf_rates(self, d, nline, gij, gji, precond=False):
+        a, b = O.bf_rates({k: n(v) for k, v in d.items()}, nline, n(gij), n(gji), precond=precond)
+        gij.copy_(torch.from_numpy(a)); gji.copy_(torch.from_numpy(b))
+
+    def normalize_rates(self, atom, nstar, gij, gji, wphi=None):
+        a, b = O.normalize_rates(atom, n(nstar), n(gij), n(gji))
+        gij.copy_(torch.from_numpy(a)); gji.copy_(torch.from_numpy(b))
+
+    def rate_solve(self, atom, e_kT, temp, totn, nstar, b, gij, gji, cup):
+        bn, res = O.update_populations(atom, n(e_kT), n(temp), n(totn), n(nstar), n(b), n(gij), n(gji), n(cup))
+        self.last = dict(gij=n(gij), gji=n(gji), b_in=n(b).copy())
+        if res["status"] != 0:
+            raise RuntimeError(f"rate solve status {res['status']} epos {res['epos']}")
+        b.copy_(torch.from_numpy(bn))
+        return res
+
+
+
 def to_work(a, p, ghost_lo, ghost_hi):
     """(nf, ncell) -> (nz+2, ny, nx, nf) with the two ghost planes (nf, ny*nx)."""
     c = p.cfg

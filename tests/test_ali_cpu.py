@@ -1,0 +1,39 @@
+"""The NLTE ALI driver (dispatch_amd.ali) run on the host through the oracle's
+restatements of every entry point (tests/ali_cases.OracleCtx): the orchestration
+itself — windows, directions, the ALI step, Ng's acceleration — on a small box of
+the workload generator."""
+import numpy as np
+
+from ali_cases import OracleCtx
+from dispatch_amd.ali import AliConfig, NlteProblem
+
+
+def small(**kw):
+    base = dict(nx=6, ny=6, nz=48, nq=16, nbound=4, nline=6, ncont=4, lines_per_window=3, nphi=2, ncont_f=8)
+    base.update(kw)
+    return AliConfig(**base)
+
+
+def test_converges_with_ng_and_departs_from_lte():
+    p = NlteProblem(OracleCtx(), small(), device="cpu")
+    n = p.solve()
+    errs = [abs(e) for e, _ in p.history]
+    assert errs[-1] < 1e-4 and n <= 40, errs
+    assert np.abs(p.b.numpy() - 1).max() > 1e-2
+
+
+def test_ng_shortens_the_iteration():
+    q = NlteProblem(OracleCtx(), small(ng=False, max_iter=25), device="cpu")
+    q.solve()
+    p = NlteProblem(OracleCtx(), small(max_iter=25), device="cpu")
+    p.solve()
+    # after 25 iterations the accelerated run is an order of magnitude closer
+    assert abs(p.history[-1][0]) < 0.1 * abs(q.history[-1][0]), (p.history[-1][0], q.history[-1][0])
+
+
+def test_strong_collisions_give_lte():
+    cfg = small(ne_scale=1e7)
+    p = NlteProblem(OracleCtx(), cfg, device="cpu")
+    p.solve()
+    b = p.b.numpy().reshape(p.atom.nk, cfg.nz, -1)
+    assert np.abs(b[:, :cfg.nz // 2] - 1).max() < 1e-3
