@@ -74,6 +74,7 @@ class AliConfig:
     max_iter: int = 40
     conv: float = 1e-4
     ng: bool = True           # Ng acceleration (the reference's automatic trigger)
+    precond: bool = True      # preconditioned (ALI) rates; False: plain Lambda iteration
 
 
 @dataclass
@@ -347,8 +348,13 @@ class NlteProblem:
             xhat.zero_()
             torch.mul(w.chi_c, w.Bnu, out=ehat)
             ctx.convert_to_work_units(w.chi_c, xhat, ehat, w.beta)
-            for t in sp.values():
-                t.zero_()
+            # the split tracking starts from everything that is not a line of the atom
+            # (the continuum) as the one contribution of index 0
+            # (SRC/rt_patch_mod.f90:1467-1488)
+            sp["split_opac1"].copy_(ehat)
+            sp["split_chi1"].copy_(xhat)
+            for k in ("split_opac2", "split_chi2", "split_id", "split_chi_id"):
+                sp[k].zero_()
             ctx.bb_opacity(d, xhat, ehat, wmu=float(self.wj[r]), split=sp, split_offset=0)
             ctx.work_to_transfer(xhat, ehat, w.beta, split_opac1=sp["split_opac1"], split_opac2=sp["split_opac2"])
             chi, S = xhat, ehat
@@ -362,7 +368,7 @@ class NlteProblem:
                 self._sweep(r, chi, S, w.ghost_B, I_out=I, Psi_out=Psi, Irem_out=Irem, escape_out=esc,
                             split_rates=True, get_rates=True)
             ctx.bb_rates(dict(d, Inu=I, Psi=Psi, Bnu=w.Bnu, beta=w.beta), float(self.wj[r]), 1.0,
-                         gij[l0:l1], gji[l0:l1], precond=True, own_norm=True, largest=True,
+                         gij[l0:l1], gji[l0:l1], precond=self.cfg.precond, own_norm=True, largest=True,
                          split=dict(sp, Irem=Irem, escape=esc, chi=chi), split_offset=0)
 
     def rates(self):

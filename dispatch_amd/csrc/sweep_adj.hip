@@ -30,6 +30,7 @@
 #include "sweep_sc2.hpp"
 #include <cmath>
 #include <algorithm>
+#include <type_traits>
 
 namespace m3drt {
 
@@ -64,7 +65,12 @@ __device__ __forceinline__ void lin_weights_deriv(float t, float &E, float &W, f
   dW = E - dPs;
 }
 
-template <int MAXC>
+// MAXC: cells per thread.  FULL: every thread owns exactly MAXC cells (nx*ny =
+// MAXC * blockDim), so the per-cell loops carry no guards and the loads of all of a
+// thread's cells can be in flight together.  SEED: a J or intensity-cube cotangent
+// is given (the exit plane's alone otherwise).  The upstream ghost plane's step is
+// peeled off the plane loop, so no plane of the loop branches on it.
+template <int MAXC, bool FULL, bool SEED>
 __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, nxy = nx * ny;
@@ -77,9 +83,7 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
   const float *ghost_rc = a.rc + (int64_t)a.nz * nxy;
   auto plane_of = [&](int step) { return a.dir > 0 ? step : a.nz - 1 - step; };
   auto in_plane = [&](uint32_t xy) { return (int64_t)(xy & 0xffff) * a.cs_x + (int64_t)(xy >> 16) * a.cs_y; };
-  auto in_ghost = [&](uint32_t xy) {
-    return (int64_t)f * a.gs_f + (int64_t)(xy & 0xffff) * a.gs_x + (int64_t)(xy >> 16) * a.gs_y;
-  };
+  auto in_ghost = [&](uint32_t xy) { return (int64_t)(xy & 0xffff) * a.gs_x + (int64_t)(xy >> 16) * a.gs_y; };
   // a cell's coordinates made opaque at each use: everything derived from them (the
   // stencil's LDS addresses, the cube offsets) is then recomputed per plane, a few
   // integer ops, instead of being hoisted out of the plane loop into registers held
@@ -95,30 +99,31 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
   // both read-modify-writes of a cell are in one thread's program order)
   uint32_t xy[MAXC];
   float acc[MAXC];
-  const int zl = plane_of(a.nz - 1);
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
     const int c = tid + k * nt;
-    xy[k] = c < nxy ? ((uint32_t)(c / nx) << 16) | (uint32_t)(c % nx) : 0xffffffffu;
-    acc[k] = (c < nxy && a.exitbar) ? a.exitbar[in_ghost(xy[k])] : 0.0f;
+    const bool ok = FULL || c < nxy;
+    xy[k] = ok ? ((uint32_t)(c / nx) << 16) | (uint32_t)(c % nx) : 0xffffffffu;
+    acc[k] = (ok && a.exitbar) ? a.exitbar[f * a.gs_f + in_ghost(xy[k])] : 0.0f;
   }
+  auto valid = [&](int k) { return FULL || xy[k] != 0xffffffffu; };
 
-  for (int step = a.nz - 1; step >= 0; --step) {
+  auto plane = [&](int step, auto ghost_c) {
+    constexpr bool GHOST = decltype(ghost_c)::value;
     const int z = plane_of(step);
-    const bool ghost = step == 0;
-    const int zu = ghost ? 0 : plane_of(step - 1);
-    const float *rcu = ghost ? ghost_rc : a.rc + (int64_t)zu * nxy;
+    const int zu = GHOST ? 0 : plane_of(step - 1);
+    const float *rcu = GHOST ? ghost_rc : a.rc + (int64_t)zu * nxy;
     const int64_t bz = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f;
     const int64_t bu = (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f;
     // (a) the upwind plane
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
-      if (xy[k] != 0xffffffffu) {
+      if (valid(k)) {
         const uint32_t q = opaque(xy[k]);
         const int c = (int)(q >> 16) * nx + (int)(q & 0xffff);
         float iv, cv, sv;
-        if (ghost) {
-          const int64_t g = in_ghost(q) - (int64_t)f * a.gs_f;
+        if (GHOST) {
+          const int64_t g = in_ghost(q);
           cv = gchi[g];
           sv = gS[g];
           iv = gI[g];
@@ -140,9 +145,8 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
     const float *dtp = a.dt500 + (int64_t)z * nxy, *rcp = a.rc + (int64_t)z * nxy;
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
-      __builtin_amdgcn_sched_barrier(0);
       gi[k] = gx[k] = gs[k] = 0.0f;
-      if (xy[k] != 0xffffffffu) {
+      if (valid(k)) {
         const uint32_t q = opaque(xy[k]);
         const int x = (int)(q & 0xffff), y = (int)(q >> 16), c = y * nx + x;
         int xs[4], ys[4];
@@ -185,8 +189,10 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
         lin_weights_deriv(dt, E, W, Ps, dE, dW, dPs);
         const float Inu = Iup * E + W * ssup + Ps * ss;
         float g = acc[k];
-        if (a.Jbar) g += a.wj * a.Jbar[o];
-        if (a.Ibar) g += a.Ibar[o];
+        if (SEED) {
+          if (a.Jbar) g += a.wj * a.Jbar[o];
+          if (a.Ibar) g += a.Ibar[o];
+        }
         g = (Inu >= -a.imax && Inu <= a.imax) ? g : 0.0f;
         const float gdt = dtr > a.mdtau ? g * (Iup * dE + ssup * dW + ss * dPs) : 0.0f;
         const float gxx = 0.5f * gdt * dt500;
@@ -198,12 +204,11 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
         codes |= (uint64_t)(clamp ? arg + 1 : 0) << (3 * k);
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
     dev::lds_barrier();
     // (c) the targets' upwind cotangents replace the upwind plane
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
-      if (xy[k] != 0xffffffffu) {
+      if (valid(k)) {
         const uint32_t q = opaque(xy[k]);
         const int c = (int)(q >> 16) * nx + (int)(q & 0xffff);
         sI[c] = gi[k];
@@ -217,8 +222,7 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
     // target column x - xo - i sees this column in stencil slot i
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
-      __builtin_amdgcn_sched_barrier(0);
-      if (xy[k] != 0xffffffffu) {
+      if (valid(k)) {
         const uint32_t q = opaque(xy[k]);
         const int x = (int)(q & 0xffff), y = (int)(q >> 16), c = y * nx + x;
         int tx[4], ty[4];
@@ -248,8 +252,8 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
           xb += a.wy[j] * (a.wx[0] * sX[row + tx[1]] + a.wx[1] * sX[row + tx[2]]);
           sb += a.wy[j] * (a.wx[0] * sS[row + tx[1]] + a.wx[1] * sS[row + tx[2]]);
         }
-        if (ghost) {
-          const int64_t g = in_ghost(q);
+        if (GHOST) {
+          const int64_t g = f * a.gs_f + in_ghost(q);
           if (a.gIbar) a.gIbar[g] = a.gIbar[g] + ib;
           if (a.gchibar) a.gchibar[g] = a.gchibar[g] + xb * rcu[c];
           if (a.gSbar) a.gSbar[g] = a.gSbar[g] + sb;
@@ -261,18 +265,27 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
         }
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
     dev::lds_barrier();
-  }
+  };
+
+  for (int step = a.nz - 1; step >= 1; --step) plane(step, std::false_type{});
+  plane(0, std::true_type{});
+}
+
+template <int MAXC, bool FULL, bool SEED>
+int launch_adj(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds) {
+  auto kern = sc2_adjoint<MAXC, FULL, SEED>;
+  M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(a.nf), dim3(nt), lds, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
 }
 
 template <int MAXC>
-int launch_adj(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds) {
-  M3DRT_HIP(hipFuncSetAttribute((const void *)sc2_adjoint<MAXC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-  hipLaunchKernelGGL(sc2_adjoint<MAXC>, dim3(a.nf), dim3(nt), lds, ctx->stream, a);
-  M3DRT_HIP(hipGetLastError());
-  return M3DRT_OK;
+int launch_full(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds, bool full) {
+  const bool seed = a.Jbar || a.Ibar;
+  if (full) return seed ? launch_adj<MAXC, true, true>(ctx, a, nt, lds) : launch_adj<MAXC, true, false>(ctx, a, nt, lds);
+  return seed ? launch_adj<MAXC, false, true>(ctx, a, nt, lds) : launch_adj<MAXC, false, false>(ctx, a, nt, lds);
 }
 
 }  // namespace
@@ -283,7 +296,14 @@ int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint
   const size_t lds = (size_t)nxy * (3 * sizeof(float) + 1);
   if (lds > (size_t)ctx->lds_max)
     return set_error(M3DRT_ELIMIT, "m3drt_sc_adjoint: three FP32 planes and the clamp bytes exceed the LDS");
-  const int nt = (int)std::min<int64_t>(1024, (nxy + 63) / 64 * 64);
+  // a workgroup size that divides the plane exactly when one does (every thread then
+  // owns the same number of cells), else the fewest cells per thread
+  int nt = (int)std::min<int64_t>(1024, (nxy + 63) / 64 * 64);
+  for (int t = 1024; t >= 256; t -= 64)
+    if (nxy % t == 0 && nxy / t <= 13) {
+      nt = t;
+      break;
+    }
   const int64_t maxc = (nxy + nt - 1) / nt;
   float *dt500 = nullptr, *rc = nullptr;
   int st = sc2_prep_f32(ctx, d, b->chi500, b->chi500_up, b->g500, &dt500, &rc);
@@ -321,12 +341,13 @@ int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint
   a.mdtau = -(float)d->maser_dtau_max;
   a.imax = (float)d->maser_Inu_max;
   a.wj = (float)d->wj;
-  if (maxc <= 1) return launch_adj<1>(ctx, a, nt, lds);
-  if (maxc <= 2) return launch_adj<2>(ctx, a, nt, lds);
-  if (maxc <= 4) return launch_adj<4>(ctx, a, nt, lds);
-  if (maxc <= 6) return launch_adj<6>(ctx, a, nt, lds);
-  if (maxc <= 9) return launch_adj<9>(ctx, a, nt, lds);
-  if (maxc <= 13) return launch_adj<13>(ctx, a, nt, lds);
+  const auto full = [&](int m) { return nxy == (int64_t)m * nt; };
+  if (maxc <= 1) return launch_full<1>(ctx, a, nt, lds, full(1));
+  if (maxc <= 2) return launch_full<2>(ctx, a, nt, lds, full(2));
+  if (maxc <= 4) return launch_full<4>(ctx, a, nt, lds, full(4));
+  if (maxc <= 6) return launch_full<6>(ctx, a, nt, lds, full(6));
+  if (maxc <= 9) return launch_full<9>(ctx, a, nt, lds, full(9));
+  if (maxc <= 13) return launch_full<13>(ctx, a, nt, lds, full(13));
   return set_error(M3DRT_ELIMIT, "m3drt_sc_adjoint: plane too large");
 }
 

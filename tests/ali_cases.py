@@ -147,8 +147,9 @@ def oracle_rates(p):
             xhat, ehat = O.convert_to_work_units(chi_c, ehat, beta)
             z = np.zeros((nf, p.ncell), np.float32)
             zi = np.zeros((nf, p.ncell), np.int32)
-            sp0 = dict(split_opac1=z, split_opac2=z.copy(), split_chi1=z.copy(), split_chi2=z.copy(), split_id=zi,
-                       split_chi_id=zi.copy())
+            # the continuum seeds the split tracking as contribution 0
+            sp0 = dict(split_opac1=ehat.copy(), split_opac2=z, split_chi1=xhat.copy(), split_chi2=z.copy(),
+                       split_id=zi, split_chi_id=zi.copy())
             op = O.bb_opacity(d, xhat, ehat, wmu=float(p.wj[r]), split=sp0, split_offset=0)
             so1 = np.ascontiguousarray(op["split_opac1"], np.float32)
             so2 = np.ascontiguousarray(op["split_opac2"], np.float32)
