@@ -1,7 +1,7 @@
 // Straight-line form of the 3D short-characteristics sweep (short_char2,
 // SRC/rt_solvers_mod.f90:224-507, with integrate_intensity,
 // SRC/rt_patch_mod.f90:3399-3431, fused) for the case the formal solution of an
-// iteration runs: J, or J and PsiJ, as the only outputs, no eta flags, no split
+// iteration runs: J, or J and PsiJ, or only the exit plane as the outputs, no eta flags, no split
 // rates.  Same mapping and arithmetic as sweep_sc2.hip's general kernel (one
 // workgroup per frequency owning the whole periodic plane, strips of L rows per
 // thread, a sliding 4-row window, PD cells of global loads in flight), with the
@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(1024) sc2_fast(const FastArgs<float> a) {
     const int64_t o = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f;
     P.chi = rsrc(a.chi + o, a.plane_bytes);
     P.S = rsrc(a.S + o, a.plane_bytes);
-    P.J = rsrc(a.J + o, a.plane_bytes);
-    P.PsJ = rsrc(PJ ? a.PsiJ + o : a.J + o, a.plane_bytes);
+    P.J = a.J ? rsrc(a.J + o, a.plane_bytes) : rsrc(a.chi + o, 0u);
+    P.PsJ = PJ ? rsrc(a.PsiJ + o, a.plane_bytes) : P.J;
     P.dr = rsrc(a.dr + (int64_t)z * nxy, (uint32_t)(nxy * sizeof(Pair<T>)));
     return P;
   };
@@ -405,8 +405,10 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   static_assert(sizeof(T) == 4, "the straight-line sweep is instantiated for FP32");
   const int nx = d->nx, ny = d->ny, nz = d->nz;
   const bool get_rates = (d->flags & M3DRT_GET_RATES) != 0;
-  if (b->eta_row || b->I_out || (d->flags & M3DRT_SPLIT_RATES) || !b->J || (get_rates && b->Psi_out))
-    return -1;
+  // J, or J and PsiJ, or only the exit plane (no J: its descriptor then has size 0,
+  // so the kernel's J loads read zero and its J stores are dropped)
+  if (b->eta_row || b->I_out || (d->flags & M3DRT_SPLIT_RATES) || (get_rates && b->Psi_out)) return -1;
+  if (!b->J && (!b->exit_I || (get_rates && b->PsiJ))) return -1;
   if (nx < 4 || ny < 4 || nx > 1024 || getenv("M3DRT_NO_FAST")) return -1;
   // one column per lane.  Two (M3DRT_FAST_CPL=2, where the rows are contiguous, nx is
   // even and J is the only output: with PsiJ the pair form needs more registers than a

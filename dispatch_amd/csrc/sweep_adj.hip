@@ -29,6 +29,7 @@
 #include "m3drt_internal.hpp"
 #include "sweep_sc2.hpp"
 #include <cmath>
+#include <cstdlib>
 #include <algorithm>
 #include <type_traits>
 
@@ -272,6 +273,252 @@ __global__ void __launch_bounds__(1024) sc2_adjoint(const AdjArgs a) {
   plane(0, std::true_type{});
 }
 
+// The same reverse sweep with the forward's strip mapping (every strip exactly L
+// rows: ny = L * strips, nx * strips threads).  Thread (x, strip) owns the cells of
+// column x in its L rows; its stencil columns are fixed, and down the strip the
+// 4-row stencils of consecutive cells share three rows, so in (b) each target reads
+// ONE new upwind row (4 I, 2 chi/chi500, 2 S) and in (d) each upwind cell ONE new
+// target row (4 intensity cotangents and clamp codes, 2 + 2 chi/S cotangents), the
+// row reductions sliding over a 4-deep register window.  Global accesses of a
+// wavefront are rows of consecutive x.
+//
+// The upwind cells of one step are the targets of the next, and a thread owns the
+// same (x, y) in both: the chi and S shares (d) gathers for a cell stay in registers
+// with its intensity cotangent and are added to the cell's own in the next step's
+// (b), so each cell's chi and S cotangents take one read-modify-write per ray, and
+// (d) touches no global memory between ghost steps.  (b) keeps the global reads of
+// the next two cells in flight while a cell is computed (each cell's read-modify-
+// write is ordered after the previous cell's, which the compiler cannot reorder:
+// the cotangent arrays may alias the inputs as far as it knows).  Arithmetic as
+// sc2_adjoint's, except that a cell's two chi shares are summed before the one
+// factor 1/chi500 they both carry.
+template <int L, bool SEED>
+__global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const AdjArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nx = a.nx, ny = a.ny, nxy = nx * ny;
+  float *sI = reinterpret_cast<float *>(smem_raw);
+  float *sX = sI + nxy;
+  float *sS = sX + nxy;
+  uint8_t *sC = reinterpret_cast<uint8_t *>(sS + nxy);
+  const int f = blockIdx.x, tid = threadIdx.x;
+  const int x = tid % nx, ybeg = (tid / nx) * L;
+  const float *gchi = a.gchi + f * a.gs_f, *gS = a.gS + f * a.gs_f, *gI = a.gI + f * a.gs_f;
+  const float *ghost_rc = a.rc + (int64_t)a.nz * nxy;
+  auto plane_of = [&](int step) { return a.dir > 0 ? step : a.nz - 1 - step; };
+  int xs[4], tx[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    xs[i] = dev::wrap_near(x + a.xo + i, nx);
+    tx[i] = dev::wrap_near(x - a.xo - i, nx);
+  }
+  const int64_t ox = (int64_t)x * a.cs_x, gox = (int64_t)x * a.gs_x;
+  // per owned cell, carried from one step's (d) to the next step's (b): the
+  // intensity cotangent and the chi (before its 1/chi500) and S shares sent upwind
+  float acc[L], ub[L], vb[L];
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    acc[k] = a.exitbar ? a.exitbar[f * a.gs_f + gox + (int64_t)(ybeg + k) * a.gs_y] : 0.0f;
+    ub[k] = 0.0f;
+    vb[k] = 0.0f;
+  }
+
+  struct URow { float rI, i1, i2, rX, rS; };
+  struct TRow { float rG, g1, g2, rX, rS; int c1, c2; };
+  struct TIn { float chv, ss, dt500, rc, cb, sb, jb, ib; };
+
+  auto plane = [&](int step, auto ghost_c) {
+    constexpr bool GHOST = decltype(ghost_c)::value;
+    int yb = ybeg;
+    asm volatile("" : "+v"(yb));
+    const int z = plane_of(step);
+    const int zu = GHOST ? 0 : plane_of(step - 1);
+    const float *rcu = GHOST ? ghost_rc : a.rc + (int64_t)zu * nxy;
+    const int64_t bz = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f + ox;
+    const int64_t bu = (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f + ox;
+    // (a) the upwind plane
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int y = yb + k, c = y * nx + x;
+      float iv, cv, sv;
+      if (GHOST) {
+        const int64_t g = gox + (int64_t)y * a.gs_y;
+        cv = gchi[g];
+        sv = gS[g];
+        iv = gI[g];
+      } else {
+        const int64_t o = bu + (int64_t)y * a.cs_y;
+        cv = a.chi[o];
+        sv = a.S[o];
+        iv = a.I[o];
+      }
+      sI[c] = iv;
+      sX[c] = cv * rcu[c];
+      sS[c] = sv;
+    }
+    dev::lds_barrier();
+    // (b) the targets, the upwind rows sliding down the strip
+    asm volatile("" : "+v"(yb));
+    auto urow = [&](int yr) -> URow {
+      URow r;
+      const int base = dev::wrap_near(yr, ny) * nx;
+      const float *row = sI + base;
+      const float i1 = row[xs[1]], i2 = row[xs[2]];
+      r.rI = a.kx[0] * row[xs[0]] + a.kx[1] * i1 + a.kx[2] * i2 + a.kx[3] * row[xs[3]];
+      r.i1 = i1;
+      r.i2 = i2;
+      r.rX = a.wx[0] * sX[base + xs[1]] + a.wx[1] * sX[base + xs[2]];
+      r.rS = a.wx[0] * sS[base + xs[1]] + a.wx[1] * sS[base + xs[2]];
+      return r;
+    };
+    const float *dtp = a.dt500 + (int64_t)z * nxy, *rcp = a.rc + (int64_t)z * nxy;
+    auto tload = [&](int k) -> TIn {
+      TIn t;
+      const int y = yb + k, c = y * nx + x;
+      const int64_t o = bz + (int64_t)y * a.cs_y;
+      t.chv = a.chi[o];
+      t.ss = a.S[o];
+      t.dt500 = dtp[c];
+      t.rc = rcp[c];
+      t.cb = a.chibar[o];
+      t.sb = a.Sbar[o];
+      t.jb = (SEED && a.Jbar) ? a.Jbar[o] : 0.0f;
+      t.ib = (SEED && a.Ibar) ? a.Ibar[o] : 0.0f;
+      return t;
+    };
+    float gi[L], gx[L], gs[L];
+    uint64_t codes = 0;
+    TIn q0 = tload(0), q1 = tload(L > 1 ? 1 : 0);
+    URow w0 = urow(yb + a.yo), w1 = urow(yb + a.yo + 1), w2 = urow(yb + a.yo + 2);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int y = yb + k;
+      const TIn t = q0;
+      q0 = q1;
+      if (k + 2 < L) q1 = tload(k + 2);
+      const URow w3 = urow(y + a.yo + 3);
+      const float iraw = a.ky[0] * w0.rI + a.ky[1] * w1.rI + a.ky[2] * w2.rI + a.ky[3] * w3.rI;
+      const float xxup = a.wy[0] * w1.rX + a.wy[1] * w2.rX;
+      const float ssup = a.wy[0] * w1.rS + a.wy[1] * w2.rS;
+      float mn = w1.i1;
+      int arg = 0;
+      if (!(mn < w1.i2)) { mn = w1.i2; arg = 1; }
+      if (!(mn < w2.i1)) { mn = w2.i1; arg = 2; }
+      if (!(mn < w2.i2)) { mn = w2.i2; arg = 3; }
+      const bool clamp = !(iraw > mn);
+      const float Iup = clamp ? mn : iraw;
+      const float xx = t.chv * t.rc;
+      const float dtr = 0.5f * (xxup + xx) * t.dt500;
+      const float dt = fmaxf(dtr, a.mdtau);
+      float E, W, Ps, dE, dW, dPs;
+      lin_weights_deriv(dt, E, W, Ps, dE, dW, dPs);
+      const float Inu = Iup * E + W * ssup + Ps * t.ss;
+      float g = acc[k];
+      if (SEED) g += a.wj * t.jb + t.ib;
+      g = (Inu >= -a.imax && Inu <= a.imax) ? g : 0.0f;
+      const float gdt = dtr > a.mdtau ? g * (Iup * dE + ssup * dW + t.ss * dPs) : 0.0f;
+      const float gxx = 0.5f * gdt * t.dt500;
+      const int64_t o = bz + (int64_t)y * a.cs_y;
+      a.chibar[o] = t.cb + (ub[k] + gxx) * t.rc;
+      a.Sbar[o] = t.sb + (vb[k] + g * Ps);
+      gi[k] = g * E;
+      gx[k] = gxx;
+      gs[k] = g * W;
+      codes |= (uint64_t)(clamp ? arg + 1 : 0) << (3 * k);
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+    }
+    dev::lds_barrier();
+    // (c) the targets' upwind cotangents replace the upwind plane
+    asm volatile("" : "+v"(yb));
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int c = (yb + k) * nx + x;
+      sI[c] = gi[k];
+      sX[c] = gx[k];
+      sS[c] = gs[k];
+      sC[c] = (uint8_t)((codes >> (3 * k)) & 7);
+    }
+    dev::lds_barrier();
+    // (d) the gather, the target rows sliding down the strip: target row
+    // y - yo - j sees this row in stencil row j
+    asm volatile("" : "+v"(yb));
+    auto trow = [&](int yr) -> TRow {
+      TRow r;
+      const int base = dev::wrap_near(yr, ny) * nx;
+      float ri = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = sI[base + tx[i]];
+        const int cd = sC[base + tx[i]];
+        ri += cd == 0 ? a.kx[i] * v : 0.0f;
+        if (i == 1) { r.g1 = v; r.c1 = cd; }
+        if (i == 2) { r.g2 = v; r.c2 = cd; }
+      }
+      r.rG = ri;
+      r.rX = a.wx[0] * sX[base + tx[1]] + a.wx[1] * sX[base + tx[2]];
+      r.rS = a.wx[0] * sS[base + tx[1]] + a.wx[1] * sS[base + tx[2]];
+      return r;
+    };
+    TRow t1 = trow(yb - a.yo - 1), t2 = trow(yb - a.yo - 2), t3 = trow(yb - a.yo - 3);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int y = yb + k;
+      const TRow t0 = trow(y - a.yo);
+      float ib = 0.0f;
+      ib += a.ky[0] * t0.rG;
+      ib += t1.c1 == 1 ? t1.g1 : 0.0f;
+      ib += t1.c2 == 2 ? t1.g2 : 0.0f;
+      ib += a.ky[1] * t1.rG;
+      ib += t2.c1 == 3 ? t2.g1 : 0.0f;
+      ib += t2.c2 == 4 ? t2.g2 : 0.0f;
+      ib += a.ky[2] * t2.rG;
+      ib += a.ky[3] * t3.rG;
+      float xb = 0.0f, sb = 0.0f;
+      xb += a.wy[0] * t1.rX;
+      sb += a.wy[0] * t1.rS;
+      xb += a.wy[1] * t2.rX;
+      sb += a.wy[1] * t2.rS;
+      if (GHOST) {
+        const int64_t g = f * a.gs_f + gox + (int64_t)y * a.gs_y;
+        if (a.gIbar) a.gIbar[g] = a.gIbar[g] + ib;
+        if (a.gchibar) a.gchibar[g] = a.gchibar[g] + xb * rcu[y * nx + x];
+        if (a.gSbar) a.gSbar[g] = a.gSbar[g] + sb;
+      } else {
+        acc[k] = ib;
+        ub[k] = xb;
+        vb[k] = sb;
+      }
+      t3 = t2;
+      t2 = t1;
+      t1 = t0;
+    }
+    dev::lds_barrier();
+  };
+
+#pragma nounroll
+  for (int step = a.nz - 1; step >= 1; --step) plane(step, std::false_type{});
+  plane(0, std::true_type{});
+}
+
+template <int L, bool SEED>
+int launch_strip(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds) {
+  auto kern = sc2_adjoint_strip<L, SEED>;
+  M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(a.nf), dim3(nt), lds, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
+
+template <int L>
+int strip_seed(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds) {
+  return (a.Jbar || a.Ibar) ? launch_strip<L, true>(ctx, a, nt, lds) : launch_strip<L, false>(ctx, a, nt, lds);
+}
+
+// strip lengths instantiated; 8 and 12 are built for 768 threads (163 VGPRs at 12)
+static const int kStripL[] = {4, 6, 8, 12};
+static int strip_max_threads(int L) { return L >= 8 ? 768 : 1024; }
+
 template <int MAXC, bool FULL, bool SEED>
 int launch_adj(m3drt_ctx *ctx, const AdjArgs &a, int nt, size_t lds) {
   auto kern = sc2_adjoint<MAXC, FULL, SEED>;
@@ -341,7 +588,20 @@ int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint
   a.mdtau = -(float)d->maser_dtau_max;
   a.imax = (float)d->maser_Inu_max;
   a.wj = (float)d->wj;
-  const auto full = [&](int m) { return nxy == (int64_t)m * nt; };
+  // the strip form where an instantiated strip length divides ny and the strips fit
+  // one workgroup; the cell-owner form otherwise
+  if (!getenv("M3DRT_ADJ_CELLS")) {
+    for (int Ls : kStripL) {
+      if (ny % Ls || (int64_t)nx * (ny / Ls) > strip_max_threads(Ls)) continue;
+      const int nts = nx * (ny / Ls);
+      if (Ls == 4) return strip_seed<4>(ctx, a, nts, lds);
+      if (Ls == 6) return strip_seed<6>(ctx, a, nts, lds);
+      if (Ls == 8) return strip_seed<8>(ctx, a, nts, lds);
+      return strip_seed<12>(ctx, a, nts, lds);
+    }
+  }
+  // the exact-divisor specialisation only where it stays within the register file
+  const auto full = [&](int m) { return m <= 4 && nxy == (int64_t)m * nt; };
   if (maxc <= 1) return launch_full<1>(ctx, a, nt, lds, full(1));
   if (maxc <= 2) return launch_full<2>(ctx, a, nt, lds, full(2));
   if (maxc <= 4) return launch_full<4>(ctx, a, nt, lds, full(4));

@@ -6,6 +6,11 @@ The kernel differentiates the FP32 forward (hardware exponential, FP32 upwind
 sums) and takes the forward intensity from the FP32 sweep; the oracle differentiates
 the FP64 forward with exact weights.  Bar: every cotangent array within 2e-5 of its
 largest magnitude (the FP32 rounding of the forward and of the reverse sums).
+
+Two kernel forms: the strip form (a thread per column and strip of rows, where a strip
+length of 4, 6, 8 or 12 divides ny and the strips fit one workgroup) and the
+cell-owner form (forced with M3DRT_ADJ_CELLS, the only one for the other planes);
+each case runs through both.
 """
 import numpy as np
 import pytest
@@ -42,17 +47,22 @@ CASES = [dict(seed=1, vec=(0.3, -0.4, 0.866)), dict(seed=2, vec=(-0.5, 0.2, -0.8
          dict(seed=4, vec=(-0.2, -0.7, 0.69), nx=9, ny=8, nz=6, nf=2),
          dict(seed=5, vec=(0.9, 0.2, 0.39), nx=3, ny=5, nz=4),
          dict(seed=6, vec=(0.3, -0.4, 0.866), nx=40, ny=30, nz=4, nf=3),
-         dict(seed=7, vec=(-0.45, 0.35, -0.82), nx=96, ny=96, nz=3, nf=2)]
+         dict(seed=7, vec=(-0.45, 0.35, -0.82), nx=96, ny=96, nz=3, nf=2),
+         dict(seed=8, vec=(0.3, -0.4, 0.866), nx=20, ny=32, nz=3, nf=2),
+         dict(seed=9, vec=(0.55, 0.5, 0.67), nx=10, ny=12, nz=4, nf=2)]
 
 
+@pytest.mark.parametrize("form", ["strip", "cells"])
 @pytest.mark.parametrize("layout", ["native", "reference", "fmajor"])
 @pytest.mark.parametrize("kw", CASES)
-def test_adjoint_matches_oracle(ctx, kw, layout):
+def test_adjoint_matches_oracle(ctx, kw, layout, form, monkeypatch):
+    if form == "cells":
+        monkeypatch.setenv("M3DRT_ADJ_CELLS", "1")
     kw = dict(kw)
     tgz = kw.pop("top_ghost_zero", False)
     case = make_case(nf=kw.pop("nf", 3), nx=kw.pop("nx", 7), ny=kw.pop("ny", 6), nz=kw.pop("nz", 5),
                      top_ghost_zero=tgz, **kw)
-    print(f"adjoint {layout} nf={case['nf']} plane={case['nx']}x{case['ny']} nz={case['nz']} vec={case['vec']}",
+    print(f"adjoint {form} {layout} nf={case['nf']} plane={case['nx']}x{case['ny']} nz={case['nz']} vec={case['vec']}",
           flush=True)
     chi, S = case["chi"].astype(np.float64), case["S"].astype(np.float64)
     I64 = O.short_char2(chi, S, case["I"].astype(np.float64), case["chi500"], case["chi500_up"], case["g500"],

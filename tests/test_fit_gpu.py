@@ -1,7 +1,7 @@
 """The abundance-fit gradients (dispatch_amd.fit) against central finite
 differences of the plain FP64 formal solution: the four log-abundance offsets with
-h = 1e-3 dex and the temperature of the most sensitive cells, to 1e-3 relative, on
-a reduced box of the fit workload's generator."""
+h = 1e-3 dex to 1e-3 relative, and directional derivatives in T, on a reduced box
+of the fit workload's generator."""
 import numpy as np
 import pytest
 import torch
@@ -32,26 +32,44 @@ def test_abundance_gradient_matches_fp64_differences(problem):
 
 
 def test_temperature_gradient_matches_fp64_differences(problem):
-    """Along a random per-cell direction v (unit normal, scaled to 1 K rms): the
-    central difference of the FP64 loss in T + eps v against <dL/dT, v>.  (Single
-    cells are a poorer check: the loss is piecewise smooth through the upwind clamp,
-    and a one-cell FP64 difference moves by up to 0.5 % with the step.)"""
+    """Directional derivatives of the loss in T against central differences of the
+    FP64 loss (the ghost temperature 2 T(1) - T(2) moving with T):
+
+    * a uniform 1 K shift of every cell, where the cells' contributions add: to
+      1e-3 relative;
+    * a random per-cell direction v (unit normal, 1 K rms), where they nearly cancel
+      (sum |dL/dT_i v_i| is about 25 times |<dL/dT, v>| on this box) and the FP64
+      difference itself moves by 3e-3 of its value between steps of 0.25 and 2 K
+      (the perturbed opacities are rounded to FP32 as the forward's are): to 1e-3
+      of the scale of the terms, sum |dL/dT_i v_i|.
+    (Single cells are a poorer check still: the loss is piecewise smooth through
+    the upwind clamp.)"""
     p = problem
     d0 = np.zeros(4)
     print("fit temperature gradient", flush=True)
     loss, _, gT = p.loss_and_grad(d0, want_T=True)
     g = torch.Generator(device="cuda").manual_seed(11)
-    v = torch.randn(p.box.T.shape, generator=g, dtype=torch.float64, device="cuda")
     T0 = p.box.T.clone()
     Tg0 = p.box.T_ghost_bot.clone()
-    want = float((gT * v).sum())
-    for eps in (1.0, 0.25):
+
+    def fd(v, eps):
         vals = []
         for sgn in (1, -1):
             p.box.T.copy_(T0 + sgn * eps * v)
             p.box.T_ghost_bot.copy_(2 * p.box.T[0] - p.box.T[1])
             vals.append(float(p.loss_of_flux(p.flux_fp64(d0))))
-        fd = (vals[0] - vals[1]) / (2 * eps)
         p.box.T.copy_(T0)
         p.box.T_ghost_bot.copy_(Tg0)
-        assert abs(want - fd) <= 1e-3 * abs(fd), (eps, want, fd)
+        return (vals[0] - vals[1]) / (2 * eps)
+
+    u = torch.ones_like(T0)
+    want = float(gT.sum())
+    for eps in (1.0, 0.25):
+        got = fd(u, eps)
+        assert abs(want - got) <= 1e-3 * abs(got), ("uniform", eps, want, got)
+    v = torch.randn(T0.shape, generator=g, dtype=torch.float64, device="cuda")
+    want = float((gT * v).sum())
+    scale = float((gT * v).abs().sum())
+    for eps in (1.0, 0.25):
+        got = fd(v, eps)
+        assert abs(want - got) <= 1e-3 * scale, ("random", eps, want, got, scale)
