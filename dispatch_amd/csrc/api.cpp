@@ -425,4 +425,56 @@ int m3drt_update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r) {
   return m3drt::update_eta_mask(ctx, r);
 }
 
+static int check_b1d(const m3drt_b1d_batch *b, const char *who) {
+  char msg[160];
+  if (!b || b->natm < 1) {
+    snprintf(msg, sizeof msg, "%s: empty batch", who);
+    return set_error(M3DRT_EINVAL, msg);
+  }
+  if (!b->nd || !b->nf || !b->nmu || !b->nline || !b->d_off || !b->l_off || !b->ld_off || !b->mu_off ||
+      !b->f_off || !b->i_off || !b->lam0 || !b->dlam || !b->dtau || !b->hckT || !b->ivth || !b->cont_a ||
+      !b->cont_b || !b->mu || !b->wflux || !b->I_top || !b->flux) {
+    snprintf(msg, sizeof msg, "%s: missing array", who);
+    return set_error(M3DRT_EINVAL, msg);
+  }
+  if (!(b->lam_ref > 0.0)) {
+    snprintf(msg, sizeof msg, "%s: lam_ref must be positive", who);
+    return set_error(M3DRT_EINVAL, msg);
+  }
+  return M3DRT_OK;
+}
+
+int m3drt_b1d_synth_plain(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_b1d_synth_plain: NULL context");
+  int rc = check_b1d(b, "m3drt_b1d_synth_plain");
+  if (rc) return rc;
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::b1d_synth_plain(ctx, b, tile);
+}
+
+int m3drt_b1d_plan_create(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile,
+                          m3drt_b1d_plan **plan) {
+  if (!ctx || !plan) return set_error(M3DRT_EINVAL, "m3drt_b1d_plan_create: NULL argument");
+  *plan = nullptr;
+  int rc = check_b1d(b, "m3drt_b1d_plan_create");
+  if (rc) return rc;
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::b1d_plan_create(ctx, b, tile, plan);
+}
+
+int m3drt_b1d_plan_run(m3drt_b1d_plan *plan) {
+  if (!plan) return set_error(M3DRT_EINVAL, "m3drt_b1d_plan_run: NULL plan");
+  return m3drt::b1d_plan_run(plan);
+}
+
+int m3drt_b1d_plan_info(const m3drt_b1d_plan *plan, m3drt_b1d_tile *tile, double *times, int32_t *ntimes) {
+  if (!plan) return set_error(M3DRT_EINVAL, "m3drt_b1d_plan_info: NULL plan");
+  return m3drt::b1d_plan_info(plan, tile, times, ntimes);
+}
+
+int m3drt_b1d_plan_destroy(m3drt_b1d_plan *plan) {
+  if (!plan) return M3DRT_OK;
+  return m3drt::b1d_plan_destroy(plan);
+}
+
 }  // extern "C"

@@ -39,6 +39,11 @@ int sweep_lcq(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs 
 int sweep_sc1(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_sch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const m3drt_hermite_fields *h);
 int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint_bufs *b);
+int b1d_synth_plain(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile);
+int b1d_plan_create(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile, m3drt_b1d_plan **out);
+int b1d_plan_run(m3drt_b1d_plan *p);
+int b1d_plan_info(const m3drt_b1d_plan *p, m3drt_b1d_tile *tile, double *times, int32_t *ntimes);
+int b1d_plan_destroy(m3drt_b1d_plan *p);
 int update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *r);
 int work_to_transfer(m3drt_ctx *ctx, const m3drt_units_args *r);
 int background_emission(m3drt_ctx *ctx, const m3drt_background_args *r);

@@ -340,6 +340,41 @@ module m3drt
     type(c_ptr) :: ghost_src = c_null_ptr
   end type m3drt_eta_args
 
+  type, bind(C), public :: m3drt_b1d_batch
+    integer(c_int32_t) :: natm
+    integer(c_int32_t) :: reserved
+    type(c_ptr) :: nd = c_null_ptr
+    type(c_ptr) :: nf = c_null_ptr
+    type(c_ptr) :: nmu = c_null_ptr
+    type(c_ptr) :: nline = c_null_ptr
+    type(c_ptr) :: d_off = c_null_ptr
+    type(c_ptr) :: l_off = c_null_ptr
+    type(c_ptr) :: ld_off = c_null_ptr
+    type(c_ptr) :: mu_off = c_null_ptr
+    type(c_ptr) :: f_off = c_null_ptr
+    type(c_ptr) :: i_off = c_null_ptr
+    type(c_ptr) :: lam0 = c_null_ptr
+    type(c_ptr) :: dlam = c_null_ptr
+    type(c_ptr) :: dtau = c_null_ptr
+    type(c_ptr) :: hckT = c_null_ptr
+    type(c_ptr) :: ivth = c_null_ptr
+    type(c_ptr) :: cont_a = c_null_ptr
+    type(c_ptr) :: cont_b = c_null_ptr
+    type(c_ptr) :: line_off = c_null_ptr
+    type(c_ptr) :: line_cl = c_null_ptr
+    type(c_ptr) :: line_rec = c_null_ptr
+    type(c_ptr) :: mu = c_null_ptr
+    type(c_ptr) :: wflux = c_null_ptr
+    real(c_double) :: lam_ref
+    type(c_ptr) :: I_top = c_null_ptr
+    type(c_ptr) :: flux = c_null_ptr
+  end type m3drt_b1d_batch
+
+  type, bind(C), public :: m3drt_b1d_tile
+    integer(c_int32_t) :: fpl
+    integer(c_int32_t) :: threads
+  end type m3drt_b1d_tile
+
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
@@ -347,6 +382,8 @@ module m3drt
   public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
   public :: m3drt_convert_to_work_units, m3drt_background_emission, m3drt_update_eta_mask
   public :: m3drt_normalize_rates, m3drt_rate_solve_f64
+  public :: m3drt_b1d_synth_plain, m3drt_b1d_plan_create, m3drt_b1d_plan_run, m3drt_b1d_plan_info
+  public :: m3drt_b1d_plan_destroy
   public :: short_char2_gpu, m3drt_error_message
 
   interface
@@ -476,6 +513,34 @@ module m3drt
       type(m3drt_rate_solve_args), intent(in) :: args
       type(m3drt_rate_solve_result), intent(inout) :: res
     end function m3drt_rate_solve_f64
+    integer(c_int) function m3drt_b1d_synth_plain(ctx, batch, tile) bind(C, name='m3drt_b1d_synth_plain')
+      import :: c_int, c_ptr, m3drt_b1d_batch
+      type(c_ptr), value :: ctx
+      type(m3drt_b1d_batch), intent(in) :: batch
+      type(c_ptr), value :: tile
+    end function m3drt_b1d_synth_plain
+    integer(c_int) function m3drt_b1d_plan_create(ctx, batch, tile, plan) bind(C, name='m3drt_b1d_plan_create')
+      import :: c_int, c_ptr, m3drt_b1d_batch
+      type(c_ptr), value :: ctx
+      type(m3drt_b1d_batch), intent(in) :: batch
+      type(c_ptr), value :: tile
+      type(c_ptr), intent(out) :: plan
+    end function m3drt_b1d_plan_create
+    integer(c_int) function m3drt_b1d_plan_run(plan) bind(C, name='m3drt_b1d_plan_run')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan
+    end function m3drt_b1d_plan_run
+    integer(c_int) function m3drt_b1d_plan_info(plan, tile, times, ntimes) bind(C, name='m3drt_b1d_plan_info')
+      import :: c_int, c_ptr, c_double, c_int32_t, m3drt_b1d_tile
+      type(c_ptr), value :: plan
+      type(m3drt_b1d_tile), intent(out) :: tile
+      real(c_double), intent(out) :: times(*)
+      integer(c_int32_t), intent(out) :: ntimes
+    end function m3drt_b1d_plan_info
+    integer(c_int) function m3drt_b1d_plan_destroy(plan) bind(C, name='m3drt_b1d_plan_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan
+    end function m3drt_b1d_plan_destroy
   end interface
 
 contains

@@ -709,3 +709,140 @@ def _background_emission(self, J, B, chi_cont, sigma, eta, *, PsiJ=None, Jali_ol
 
 
 Context.background_emission = _background_emission
+
+
+# ------------------------------------------------------------------ batched 1D synthesis
+
+B1D_MAXMU = 8
+
+
+class B1dBatch(C.Structure):
+    _fields_ = [("natm", C.c_int32), ("reserved", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "nd", "nf", "nmu", "nline", "d_off", "l_off", "ld_off", "mu_off", "f_off", "i_off", "lam0", "dlam",
+        "dtau", "hckT", "ivth", "cont_a", "cont_b", "line_off", "line_cl", "line_rec", "mu", "wflux")] + \
+        [("lam_ref", C.c_double), ("I_top", C.c_void_p), ("flux", C.c_void_p)]
+
+
+class B1dTile(C.Structure):
+    _fields_ = [("fpl", C.c_int32), ("threads", C.c_int32)]
+
+
+# device dtypes of the batch's arrays, by field
+B1D_DTYPES = dict(nd=torch.int32, nf=torch.int32, nmu=torch.int32, nline=torch.int32, d_off=torch.int64,
+                  l_off=torch.int64, ld_off=torch.int64, mu_off=torch.int64, f_off=torch.int64, i_off=torch.int64,
+                  lam0=torch.float64, dlam=torch.float64, dtau=torch.float32, hckT=torch.float32,
+                  ivth=torch.float32, cont_a=torch.float32, cont_b=torch.float32, line_off=torch.float32,
+                  line_cl=torch.float32, line_rec=torch.float32, mu=torch.float32, wflux=torch.float32,
+                  I_top=torch.float32, flux=torch.float32)
+
+
+def _b1d_lib():
+    L = lib()
+    if not getattr(L, "_b1d_ok", False):
+        vp = C.c_void_p
+        L.m3drt_b1d_synth_plain.argtypes = [vp, C.POINTER(B1dBatch), C.POINTER(B1dTile)]
+        L.m3drt_b1d_plan_create.argtypes = [vp, C.POINTER(B1dBatch), C.POINTER(B1dTile), C.POINTER(vp)]
+        L.m3drt_b1d_plan_run.argtypes = [vp]
+        L.m3drt_b1d_plan_info.argtypes = [vp, C.POINTER(B1dTile), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.m3drt_b1d_plan_destroy.argtypes = [vp]
+        for n in ("m3drt_b1d_synth_plain", "m3drt_b1d_plan_create", "m3drt_b1d_plan_run", "m3drt_b1d_plan_info",
+                  "m3drt_b1d_plan_destroy"):
+            getattr(L, n).restype = C.c_int
+        L._b1d_ok = True
+    return L
+
+
+def b1d_check_sizes(arrays: dict):
+    """Every flat array long enough for the atmospheres' counts and offsets (the
+    library takes plain pointers and cannot check their extents)."""
+    h = {k: arrays[k].cpu().numpy().astype(np.int64) for k in
+         ("nd", "nf", "nmu", "nline", "d_off", "l_off", "ld_off", "mu_off", "f_off", "i_off")}
+    n = len(h["nd"])
+    for k in ("nf", "nmu", "nline", "d_off", "l_off", "ld_off", "mu_off", "f_off", "i_off", "lam0", "dlam"):
+        if arrays[k].numel() != n:
+            raise ValueError(f"b1d: {k} must have one entry per atmosphere")
+    if n == 0:
+        return
+    if (h["nd"] < 2).any() or (h["nf"] < 1).any() or (h["nmu"] < 1).any() or (h["nmu"] > B1D_MAXMU).any() or \
+            (h["nline"] < 0).any():
+        raise ValueError("b1d: every atmosphere needs nd >= 2, nf >= 1, 1 <= nmu <= 8, nline >= 0")
+    for k in ("d_off", "l_off", "ld_off", "mu_off", "f_off", "i_off"):
+        if (h[k] < 0).any():
+            raise ValueError(f"b1d: negative {k}")
+    need = dict(dtau=h["d_off"] + h["nd"], hckT=h["d_off"] + h["nd"], ivth=h["d_off"] + h["nd"],
+                cont_a=h["d_off"] + h["nd"], cont_b=h["d_off"] + h["nd"], line_off=h["l_off"] + h["nline"],
+                line_cl=h["l_off"] + h["nline"], line_rec=4 * (h["ld_off"] + h["nd"] * h["nline"]),
+                mu=h["mu_off"] + h["nmu"], wflux=h["mu_off"] + h["nmu"], flux=h["f_off"] + h["nf"],
+                I_top=h["i_off"] + h["nmu"] * h["nf"])
+    for k, v in need.items():
+        if arrays[k].numel() < int(v.max()):
+            raise ValueError(f"b1d: {k} has {arrays[k].numel()} entries, the offsets need {int(v.max())}")
+
+
+def _b1d_struct(arrays: dict, lam_ref: float) -> B1dBatch:
+    """The m3drt_b1d_batch of a dict of contiguous device tensors (B1D_DTYPES)."""
+    for k, dt in B1D_DTYPES.items():
+        t = arrays[k]
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"b1d: {k} must be a contiguous {dt} device tensor")
+    b1d_check_sizes(arrays)
+    b = B1dBatch()
+    b.natm = arrays["nd"].numel()
+    for k in B1D_DTYPES:
+        setattr(b, k, arrays[k].data_ptr())
+    b.lam_ref = lam_ref
+    return b
+
+
+def _b1d_tile(tile):
+    return None if tile is None else C.byref(B1dTile(*tile))
+
+
+def _b1d_synth_plain(self, arrays, lam_ref, tile=None):
+    """One launch per atmosphere (m3drt_b1d_synth_plain); tile = (fpl, threads)."""
+    L = _b1d_lib()
+    b = _b1d_struct(arrays, lam_ref)
+    self._bind_stream()
+    _check(L.m3drt_b1d_synth_plain(self._h, C.byref(b), _b1d_tile(tile)), "m3drt_b1d_synth_plain")
+
+
+class B1dPlan:
+    """m3drt_b1d_plan: the persistent, graph-captured synthesis of one batch; tile None
+    lets the autotuner choose.  Keeps the batch's tensors alive."""
+
+    def __init__(self, ctx, arrays, lam_ref, tile=None):
+        self.ctx, self.arrays = ctx, arrays
+        self._b = _b1d_struct(arrays, lam_ref)
+        L = _b1d_lib()
+        h = C.c_void_p()
+        ctx._bind_stream()
+        _check(L.m3drt_b1d_plan_create(ctx._h, C.byref(self._b), _b1d_tile(tile), C.byref(h)),
+               "m3drt_b1d_plan_create")
+        self._h = h
+
+    def run(self):
+        self.ctx._bind_stream()
+        _check(_b1d_lib().m3drt_b1d_plan_run(self._h), "m3drt_b1d_plan_run")
+
+    def info(self):
+        """(tile (fpl, threads), the autotuner's times in ms per candidate)."""
+        t = B1dTile()
+        times = (C.c_double * 6)()
+        n = C.c_int32()
+        _check(_b1d_lib().m3drt_b1d_plan_info(self._h, C.byref(t), times, C.byref(n)), "m3drt_b1d_plan_info")
+        return (t.fpl, t.threads), list(times)[:n.value]
+
+    def close(self):
+        if self._h:
+            _b1d_lib().m3drt_b1d_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+Context.b1d_synth_plain = _b1d_synth_plain
+Context.b1d_plan = lambda self, arrays, lam_ref, tile=None: B1dPlan(self, arrays, lam_ref, tile)

@@ -430,6 +430,80 @@ typedef struct {
  * compaction of them.  Clearing the three arrays is the reference's eta reset. */
 int m3drt_update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *args);
 
+/* ------------------------------------------------------------------ batched 1D synthesis */
+
+/* A batch of plane-parallel LTE atmospheres of uneven depth, frequency and angle
+ * counts, each on its own equidistant wavelength grid.  Per (atmosphere, frequency)
+ * one lane walks the depth points from the bottom up: it sums the atmosphere's line
+ * opacities (QLG-bot Voigt profiles, HOP/qlg_mod.f90:538-627, each line only inside
+ * its window, as add_opacity_equi windows them, HOP/linelist_mod.f90:853-906) onto
+ * the continuum, forms S = B_nu(T), and integrates every upward mu with the linear
+ * step of long_char2 in tau500 (SRC/rt_solvers_mod.f90:527-663); the deepest point
+ * is thermalised (I = S).  In the reference each such atmosphere is one dispatch.x
+ * run (1D: long_char2 per frequency window and direction, add_opacity_equi per
+ * species); the batch has no single counterpart.
+ *
+ * Every array is a device array.  Per-depth, per-line, per-mu and per-(line, depth)
+ * arrays are flat over the batch, each atmosphere's entries starting at its offset.
+ * Wavelengths in cm, opacities relative to chi500, intensities in occupation units
+ * (I / (2 h nu^3 / c^2)). */
+#define M3DRT_B1D_MAXMU 8
+
+typedef struct {
+  int32_t natm;
+  int32_t reserved;
+  const int32_t *nd, *nf, *nmu, *nline;  /* (natm): depth points (>= 2), frequencies, upward mu, lines */
+  const int64_t *d_off, *l_off, *ld_off, *mu_off;  /* (natm) first depth / line / (line, depth) record / mu */
+  const int64_t *f_off;                 /* (natm) first element of flux */
+  const int64_t *i_off;                 /* (natm) first element of I_top ((mu, f), f fastest) */
+  const double *lam0, *dlam;            /* (natm) lam_f = lam0 + f dlam */
+  const float *dtau;                    /* (depth) vertical tau500 step from point d+1 up to d (last unused) */
+  const float *hckT;                    /* (depth) h c / (k T) [cm]: B = 1 / (exp(hckT / lam) - 1) */
+  const float *ivth;                    /* (depth) 1 / Doppler speed [s/cm] */
+  const float *cont_a, *cont_b;         /* (depth) chi_cont / chi500 = cont_a + cont_b (lam / lam_ref)^3 */
+  const float *line_off;                /* (line) lam_l - lam0 of the line's atmosphere [cm] */
+  const float *line_cl;                 /* (line) c / lam_l [1/s]: v = |lam - lam_l| line_cl ivth */
+  const float *line_rec;                /* (line, depth), the atmosphere's records depth-major ([d][l]),
+                                           4 floats each: damping a, centre strength s (the line's
+                                           chi / chi500 is s K(a, v)), window half-width [cm] (< 0:
+                                           the line is skipped at that depth), reserved */
+  const float *mu, *wflux;              /* (mu) cosine (0 < mu <= 1) and flux weight */
+  double lam_ref;                       /* [cm] */
+  float *I_top;                         /* out: emergent intensity of every (atmosphere, mu, f) */
+  float *flux;                          /* out: sum_mu wflux I_top, every (atmosphere, f) */
+} m3drt_b1d_batch;
+
+/* Tile shape of the batched kernel: frequencies per lane (a wavefront covers 64 fpl
+ * consecutive frequencies of one atmosphere) and threads per workgroup (the waves of
+ * a workgroup share one LDS copy of the QLG-bot tables). */
+typedef struct {
+  int32_t fpl;        /* 1 or 2 */
+  int32_t threads;    /* 256, 512 or 1024 */
+} m3drt_b1d_tile;
+
+typedef struct m3drt_b1d_plan m3drt_b1d_plan;
+
+/* The plain path: one launch per atmosphere (a static grid of 64-frequency tiles), the
+ * same per-lane arithmetic.  tile NULL: fpl 1, 256 threads. */
+int m3drt_b1d_synth_plain(m3drt_ctx *ctx, const m3drt_b1d_batch *batch, const m3drt_b1d_tile *tile);
+
+/* A plan for repeated synthesis of one batch (the arrays stay where they are; their
+ * contents may change between runs): the work list of (atmosphere, frequency tile)
+ * items in decreasing cost, the per-(line, depth) profile coefficients, and the
+ * persistent launch (a counter reset and one kernel whose waves take items until the
+ * list is exhausted) captured in a HIP graph.  tile NULL: the autotuner times every
+ * tile shape on a strided sample of the work list and keeps the fastest. */
+int m3drt_b1d_plan_create(m3drt_ctx *ctx, const m3drt_b1d_batch *batch, const m3drt_b1d_tile *tile,
+                          m3drt_b1d_plan **plan);
+/* Re-forms the profile coefficients from line_rec (after its contents changed) and
+ * replays the graph, ordered after the context stream's earlier work. */
+int m3drt_b1d_plan_run(m3drt_b1d_plan *plan);
+/* The tile in use and, when the autotuner chose it, every candidate's time on the
+ * sample in ms (times[i] for candidate i of the order fpl 1/2 x threads 256/512/1024;
+ * < 0 where not timed); ntimes receives the count (6). */
+int m3drt_b1d_plan_info(const m3drt_b1d_plan *plan, m3drt_b1d_tile *tile, double *times, int32_t *ntimes);
+int m3drt_b1d_plan_destroy(m3drt_b1d_plan *plan);
+
 /* Strides of the native frequency-sliced layout (x fastest, then y, f, z) for a
  * cube and for a plane; fills cs_* and gs_* of desc from its nf, nx, ny. */
 int m3drt_native_strides(m3drt_sweep_desc *desc);

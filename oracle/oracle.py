@@ -765,3 +765,27 @@ def lin_weights_deriv(t):
     out = [C.c_double() for _ in range(6)]
     L.orc_lin_weights_deriv_f64(float(t), *[C.byref(o) for o in out])
     return tuple(o.value for o in out)
+
+
+def b1d_atmosphere(h, m, lam_ref, want_chi=False):
+    """The oracle's synthesis of atmosphere m of a batch given as host numpy arrays
+    (the fields of m3drt_b1d_batch): (I_top (nmu, nf), flux (nf)[, chi (nd, nf)]),
+    FP64."""
+    L = lib()
+    if not getattr(L, "_b1d_ok", False):
+        L.orc_b1d_atmosphere.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [f32p] * 10 + [C.c_double, f64p, f64p,
+                                                                                          C.c_void_p]
+        L.orc_b1d_atmosphere.restype = None
+        L._b1d_ok = True
+    nd, nf, nmu, nl = (int(h[k][m]) for k in ("nd", "nf", "nmu", "nline"))
+    d0, l0, r0, m0 = (int(h[k][m]) for k in ("d_off", "l_off", "ld_off", "mu_off"))
+    sl = lambda k, a, n, w=1: np.ascontiguousarray(h[k][w * a:w * (a + n)], np.float32)
+    I = np.zeros((nmu, nf))
+    F = np.zeros(nf)
+    chi = np.zeros((nd, nf)) if want_chi else None
+    L.orc_b1d_atmosphere(nd, nf, nmu, nl, float(h["lam0"][m]), float(h["dlam"][m]), sl("dtau", d0, nd),
+                         sl("hckT", d0, nd), sl("ivth", d0, nd), sl("cont_a", d0, nd), sl("cont_b", d0, nd),
+                         sl("line_off", l0, nl), sl("line_cl", l0, nl), sl("line_rec", r0, nd * nl, 4),
+                         sl("mu", m0, nmu), sl("wflux", m0, nmu), float(lam_ref), I, F,
+                         None if chi is None else chi.ctypes.data_as(C.c_void_p))
+    return (I, F, chi) if want_chi else (I, F)
