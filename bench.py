@@ -55,6 +55,7 @@ def parse():
     p.add_argument("--calib-copy", action="store_true",
                    help="run a plain 4 GiB device copy first (HBM-counter calibration)")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-batch1d", action="store_true")
     p.add_argument("--adjoint-fd", action="store_true",
                    help="check the adjoint's abundance gradients against FP64 central differences")
     p.add_argument("--nf", type=int, default=1024)
@@ -250,6 +251,57 @@ def adjoint_leg(ctx, args):
     return out
 
 
+def batch1d_leg(ctx, args):
+    """The batched uneven 1D workload (8192 plane-parallel atmospheres, one persistent
+    graph-captured launch with the autotuned tile), and against the plain
+    one-launch-per-atmosphere path on its first 512 atmospheres (a sample: the plain
+    path takes about 20 s for all 8192)."""
+    import torch
+    from dispatch_amd import batch1d as B
+    t0 = time.time()
+    arrays, meta = B.make_batch(B.B1dConfig())
+    plan = ctx.b1d_plan(arrays, B.LAM_REF)
+    tile, times = plan.info()
+    log(f"[bench] batch1d ready in {time.time() - t0:.1f}s, tile {tile}")
+    plan.run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(max(2, args.steps)):
+        a = time.perf_counter()
+        plan.run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - a)
+    t_full = min(ts)
+    plan.close()
+    sub = B.B1dConfig(natm=512)
+    sa, sm = B.make_batch(sub)
+    sp = ctx.b1d_plan(sa, B.LAM_REF, tile=tile)
+    sp.run()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    sp.run()
+    torch.cuda.synchronize()
+    t_sub = time.perf_counter() - a
+    F = sa["flux"].clone()
+    ctx.b1d_synth_plain(sa, B.LAM_REF, tile=tile)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    ctx.b1d_synth_plain(sa, B.LAM_REF, tile=tile)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - a
+    same = bool(torch.equal(F, sa["flux"]))
+    sp.close()
+    nu = B.updates(meta)
+    out = {"workload": "batched_uneven_1d_lte", "atmospheres": len(meta["nd"]), "updates": nu,
+           "profile_evaluations": B.profile_evaluations(arrays, meta), "ms": t_full * 1e3,
+           "updates_per_s": nu / t_full, "tile": {"fpl": tile[0], "threads": tile[1]}, "autotune_ms": times,
+           "sample_512": {"persistent_ms": t_sub * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_sub,
+                          "flux_identical": same}}
+    log(f"[bench] batch1d {t_full * 1e3:.1f} ms; 512-sample plain/persistent {t_plain / t_sub:.0f}x")
+    del arrays, sa
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -326,6 +378,9 @@ def main():
             s32.free()
             torch.cuda.empty_cache()
         res["adjoint"] = adjoint_leg(ctx, args)
+        torch.cuda.empty_cache()
+    if not args.no_batch1d:
+        res["batch1d"] = batch1d_leg(ctx, args)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         chi32, S32 = synth.build_cubes(box, torch.float32)

@@ -26,7 +26,7 @@ and reused over directions, windows and iterations.
 The workload: a 60 x 60 x 120 box from the 3D LTE generator, the synthetic
 21-level atom (20 bound levels, continuum; 80 lines, 20 continua), 120 frequencies
 per line, van Regemorter collisions, 16 rays (2 Gauss mu x 4 azimuths, mirrored);
-iterate to max |dn/n| < 1e-4 (48 iterations here; the command line caps at 60).  The line-of-sight velocity is the
+iterate to max |dn/n| < 1e-4 (the command line caps the count at 60).  The line-of-sight velocity is the
 generator's vertical velocity projected on each ray.  The continuum windows carry
 the background alone (the atom's own bound-free opacity is not added to the
 transfer), so their J is computed once.

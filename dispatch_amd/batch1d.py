@@ -56,11 +56,36 @@ class B1dConfig:
     mass_amu: float = 56.0
     chi_ion: float = 7.6
     log_s0: float = 4.0
+    mu_rule: str = "gauss"   # or "radau": Gauss-Radau with mu = 1 included
+
+
+# The single-atmosphere synthesis: Teff 5800 K, log g 4.4, 72 depth points, 4000
+# wavelengths over 500-510 nm, 5 Gauss-Radau mu (mu = 1 included), 60 lines.
+CONFIG0 = B1dConfig(natm=1, seed=20261202, teff=(5800.0, 5800.0), logg=(4.4, 4.4), nd=(72, 72),
+                    nf=(4000, 4000), nmu=(5, 5), nline=(60, 60), mu_rule="radau")
 
 
 def gauss_mu(n):
     """Gauss-Legendre nodes and weights on (0, 1] (weights sum to 1)."""
     x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def radau_mu(n):
+    """Gauss-Radau nodes and weights on [0, 1] with the node mu = 1 (weights sum to 1;
+    exact for polynomials of degree 2n - 2), as radau%init places them
+    (solvers/rt/radau_mod.f90:15-115): the free nodes are the roots of
+    (P_(n-1) - P_n)/(1 - x) on [-1, 1]."""
+    from numpy.polynomial import legendre as Lg
+    if n == 1:
+        return np.array([1.0]), np.array([1.0])
+    c = np.zeros(n + 1)
+    c[n - 1], c[n] = 1.0, -1.0
+    r = np.sort(np.real(Lg.legroots(c)))
+    r = r[np.abs(r - 1.0) > 1e-12]
+    x = np.append(r, 1.0)
+    pn1 = Lg.legval(x, np.eye(n)[n - 1])
+    w = np.where(np.abs(x - 1.0) < 1e-12, 2.0 / n ** 2, (1.0 + x) / (n * pn1) ** 2)
     return 0.5 * (x + 1.0), 0.5 * w
 
 
@@ -78,7 +103,7 @@ def make_batch(cfg: B1dConfig = B1dConfig(), device="cuda"):
     teff = rng.uniform(*cfg.teff, n)
     logg = rng.uniform(*cfg.logg, n)
     nd = rng.integers(cfg.nd[0], cfg.nd[1] + 1, n).astype(np.int32)
-    nf = np.exp(rng.uniform(math.log(cfg.nf[0]), math.log(cfg.nf[1]), n)).astype(np.int32)
+    nf = np.rint(np.exp(rng.uniform(math.log(cfg.nf[0]), math.log(cfg.nf[1]), n))).astype(np.int32)
     nmu = rng.integers(cfg.nmu[0], cfg.nmu[1] + 1, n).astype(np.int32)
     nline = rng.integers(cfg.nline[0], cfg.nline[1] + 1, n).astype(np.int32)
     d_off, l_off, mu_off, f_off = _offsets(nd), _offsets(nline), _offsets(nmu), _offsets(nf)
@@ -152,7 +177,7 @@ def make_batch(cfg: B1dConfig = B1dConfig(), device="cuda"):
 
     mus, wfl = [], []
     for m in range(n):
-        x, w = gauss_mu(int(nmu[m]))
+        x, w = (radau_mu if cfg.mu_rule == "radau" else gauss_mu)(int(nmu[m]))
         mus.append(x)
         wfl.append(2.0 * w * x)
     t = lambda v, dt: torch.as_tensor(np.ascontiguousarray(v), dtype=dt, device=dev)
@@ -200,11 +225,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--natm", type=int, default=8192)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--config0", action="store_true", help="the single 72-point, 4000-wavelength atmosphere")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
     t0 = time.perf_counter()
-    arrays, meta = make_batch(B1dConfig(natm=args.natm))
+    cfg = CONFIG0 if args.config0 else B1dConfig(natm=args.natm)
+    args.natm = cfg.natm
+    arrays, meta = make_batch(cfg)
     torch.cuda.synchronize()
     log(f"[b1d] batch of {args.natm} atmospheres in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()

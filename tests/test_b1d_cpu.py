@@ -134,3 +134,19 @@ def test_batch_struct_is_built_from_device_tensors_only():
     arrays, _ = B.make_batch(B.B1dConfig(natm=2, nd=(4, 6), nf=(10, 20), nline=(1, 2)), device="cpu")
     with pytest.raises(ValueError, match="device tensor"):
         m3drt._b1d_struct(arrays, LAM)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7])
+def test_radau_rule_includes_mu1_and_is_exact_to_degree_2n_minus_2(n):
+    x, w = B.radau_mu(n)
+    assert x[-1] == 1.0 and np.all(np.diff(x) > 0) and np.all(w > 0)
+    for k in range(2 * n - 1):
+        assert abs(np.sum(w * x ** k) - 1.0 / (k + 1)) < 1e-13
+    if n > 1:
+        assert abs(np.sum(w * x ** (2 * n - 1)) - 1.0 / (2 * n)) > 1e-12
+
+
+def test_config0_shape():
+    arrays, meta = B.make_batch(B.CONFIG0, device="cpu")
+    assert list(meta["nd"]) == [72] and list(meta["nf"]) == [4000] and list(meta["nmu"]) == [5]
+    assert list(meta["nline"]) == [60] and float(arrays["mu"][-1]) == 1.0

@@ -126,3 +126,16 @@ def test_rejects_bad_tiles_and_counts(ctx, batch):
     bad = dict(arrays, nmu=torch.full_like(arrays["nmu"], 9))
     with pytest.raises(ValueError):
         ctx.b1d_synth_plain(bad, B.LAM_REF)
+
+
+def test_single_atmosphere_synthesis_matches_oracle(ctx):
+    """The 72-point, 4000-wavelength, 60-line atmosphere with 5 Gauss-Radau mu through
+    the plan, against the oracle in FP64."""
+    print("b1d config0", flush=True)
+    arrays, meta = B.make_batch(B.CONFIG0, device="cuda")
+    plan = ctx.b1d_plan(arrays, B.LAM_REF, tile=(1, 256))
+    plan.run()
+    torch.cuda.synchronize()
+    host = {k: v.cpu().numpy() for k, v in arrays.items()}
+    check_oracle(arrays, meta, [O.b1d_atmosphere(host, 0, B.LAM_REF)])
+    plan.close()
