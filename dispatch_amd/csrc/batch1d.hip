@@ -120,22 +120,8 @@ __global__ void b1d_reset(int32_t *counter) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Read-only data the kernel never writes, read at wave-uniform addresses: through the
-// constant address space the compiler issues scalar loads (one per wave, into SGPRs,
-// through the scalar data cache) instead of 64-lane vector loads of one address.
-// (The host pass of the compiler parses device code too; the qualifier is for the
-// device pass.)
-#if defined(__HIP_DEVICE_COMPILE__)
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-#else
-template <typename T>
-using cptr = const T *;
-#endif
-template <typename T>
-__device__ __forceinline__ cptr<T> ro(const T *p) {
-  return (cptr<T>)(p);
-}
+using dev::cptr;
+using dev::ro;
 
 // One wave's item: atmosphere m, frequencies f0 + lane + 64 j.
 template <int FPL>

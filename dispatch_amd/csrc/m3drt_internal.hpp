@@ -59,6 +59,23 @@ int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r);
 namespace m3drt {
 namespace dev {
 
+// Read-only data the kernel never writes, read at wave-uniform addresses: through the
+// constant address space the compiler issues scalar loads (one per wave, into SGPRs,
+// through the scalar data cache) instead of 64-lane vector loads of one address.
+// (The host pass of the compiler parses device code too; the qualifier is for the
+// device pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+#else
+template <typename T>
+using cptr = const T *;
+#endif
+template <typename T>
+__device__ __forceinline__ cptr<T> ro(const T *p) {
+  return (cptr<T>)(p);
+}
+
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 

@@ -144,40 +144,47 @@ __global__ void __launch_bounds__(256) bb_rates_kernel(const RatesArgs a) {
         in_force = (a.flags_in[(int64_t)pw * nc + c] >> pb) & 1;
       }
       for (int f = fs; f <= fe; ++f) {
-        const float vph = Q.profile(TGX, TH2X, a.q[lo + f], vpr, idf);
+        // every read of the point first, unconditionally, so that they are in flight
+        // together (reads under the psi == 0 choice would wait for Psi's); the
+        // frequency's own values through the scalar cache
+        const float qf = dev::ro(a.q)[lo + f];
+        const double wnu = dev::ro(a.wnu)[lo + f];
+        const double wqf = a.own_norm ? dev::ro(a.wq)[lo + f] : 0.0;
         const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
         float psi = a.Psi[o];
-        const float bt = a.beta[o];
+        const float bt = a.beta[o], Iv = a.Inu[o], Ir = a.Irem[o], es = a.esc[o], chv = a.chi[o], nb = a.Bnu[o];
+        const float s1 = a.so1[o], s2 = a.so2[o], c1 = a.sc1[o], c2 = a.sc2[o];
+        const int32_t si = a.sid[o], ci = a.scid[o];
+        const float sct = a.scat ? a.chi_scat[o] * a.dJnu[o] : 0.0f;
+        const float vph = Q.profile(TGX, TH2X, qf, vpr, idf);
         const float eta_u = vph * eta_coef;
         const float eta_t = eta_u * bt;
         const float chi_t = vph * abs_coef - eta_t;
-        const float eta_other = bt * other_contribution(eta_u, index, a.so1[o], a.so2[o], a.sid[o]);
-        const float chi_other = other_contribution(vph * abs_coef, index, a.sc1[o], a.sc2[o], a.scid[o]) -
-                                eta_other;
+        const float eta_other = bt * other_contribution(eta_u, index, s1, s2, si);
+        const float chi_other = other_contribution(vph * abs_coef, index, c1, c2, ci) - eta_other;
         float other = eta_other;
-        if (a.scat) other = other + a.chi_scat[o] * a.dJnu[o];
+        if (a.scat) other = other + sct;
         if (chi_t < 0.0f || in_force) psi = 0.0f;
         raise = raise || psi == 0.0f;
         const bool zero = psi == 0.0f;
-        const float up = zero ? a.Inu[o] : a.Irem[o] + psi * other;
-        const float spont = zero ? 1.0f : (chi_other + a.esc[o] * chi_t) / a.chi[o];
-        const float wmunu = (float)((double)a.wmu * a.wnu[lo + f] * a.Aij[l]);
+        const float up = zero ? Iv : Ir + psi * other;
+        const float spont = zero ? 1.0f : (chi_other + es * chi_t) / chv;
+        const float wmunu = (float)((double)a.wmu * wnu * a.Aij[l]);
         const float vvv = vph * wmunu;
-        const float nb = a.Bnu[o];
         sA.add(vvv * spont * (nb * inv_nref));  // sum_T
         sI.add(vvv * (up - nb * spont));        // sum_U
-        if (a.own_norm) sw.add(vph * (float)a.wq[lo + f]);
+        if (a.own_norm) sw.add(vph * (float)wqf);
       }
       if (a.use_flags && raise && !in_force) atomicOr(&a.flags_out[(int64_t)pw * nc + c], (int32_t)(1u << pb));
     } else {
       for (int f = fs; f <= fe; ++f) {
-        const float qf = a.q[lo + f];
+        const float qf = dev::ro(a.q)[lo + f];
         const float phi = Q.profile(TGX, TH2X, qf, vpr, idf);
-        const float wmunu = (float)((double)a.wmu * a.wnu[lo + f] * a.Aij[l]);
+        const float wmunu = (float)((double)a.wmu * dev::ro(a.wnu)[lo + f] * a.Aij[l]);
         const float vvv = phi * wmunu;
         const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
         const float nb = a.Bnu[o], Iv = a.Inu[o];
-        if (a.own_norm) sw.add(phi * (float)a.wq[lo + f]);
+        if (a.own_norm) sw.add(phi * (float)dev::ro(a.wq)[lo + f]);
         sA.add(vvv * (nb * inv_nref));
         if (a.largest) {
           sI.add(vvv * (Iv - nb));
@@ -232,6 +239,35 @@ __device__ __forceinline__ void add_largest(float value, int index, float *large
   }
 }
 
+// One line's frequencies of one cell.  The arrays are distinct (restrict): the
+// compiler may then issue the reads of the next frequencies before this one's
+// read-modify-writes, which otherwise order every iteration's loads after the
+// previous iteration's stores.
+template <bool SPLIT>
+__device__ __forceinline__ double opacity_line(const Qlg &Q, const float *TGX, const float *TH2X,
+                                               const float *__restrict__ q, const double *__restrict__ wq,
+                                               bool sum_wphi, float vpr, float idf, float chi_fac, float eta_fac,
+                                               int index, int fs, int fe, int64_t sf, int64_t co,
+                                               float *__restrict__ xhat, float *__restrict__ ehat,
+                                               float *__restrict__ so1, float *__restrict__ so2,
+                                               int32_t *__restrict__ sid, float *__restrict__ sc1,
+                                               float *__restrict__ sc2, int32_t *__restrict__ scid) {
+  double ws = 0.0;
+#pragma unroll 4
+  for (int f = fs; f <= fe; ++f) {
+    const float vph = Q.profile(TGX, TH2X, dev::ro(q)[f], vpr, idf);
+    const int64_t o = (int64_t)f * sf + co;
+    xhat[o] = xhat[o] + vph * chi_fac;
+    ehat[o] = ehat[o] + vph * eta_fac;
+    if (SPLIT) {
+      add_largest(vph * eta_fac, index, &so1[o], &so2[o], &sid[o]);
+      add_largest(vph * chi_fac, index, &sc1[o], &sc2[o], &scid[o]);
+    }
+    if (sum_wphi) ws += (double)vph * dev::ro(wq)[f];  // sum(phi * line%wq) in real(8)
+  }
+  return ws;
+}
+
 __global__ void __launch_bounds__(256) bb_opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];
   load_tables(tab, a.tab);
@@ -262,18 +298,13 @@ __global__ void __launch_bounds__(256) bb_opacity_kernel(const OpacArgs a) {
     const Qlg Q(a.adamp[(int64_t)l * nc + c]);
     const int64_t lo = (int64_t)l * a.nf;
     const bool sum_wphi = a.wphi_line && a.wphi_line[l];
-    double ws = 0.0;
-    for (int f = fs; f <= fe; ++f) {
-      const float vph = Q.profile(TGX, TH2X, a.q[lo + f], vpr, idf);
-      const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
-      a.xhat[o] = a.xhat[o] + vph * chi_fac;
-      a.ehat[o] = a.ehat[o] + vph * eta_fac;
-      if (a.split) {
-        add_largest(vph * eta_fac, index, &a.so1[o], &a.so2[o], &a.sid[o]);
-        add_largest(vph * chi_fac, index, &a.sc1[o], &a.sc2[o], &a.scid[o]);
-      }
-      if (sum_wphi) ws += (double)vph * a.wq[lo + f];  // sum(phi * line%wq) in real(8)
-    }
+    const int64_t co = (int64_t)c * a.sc;
+    const double ws =
+        a.split ? opacity_line<true>(Q, TGX, TH2X, a.q + lo, a.wq + lo, sum_wphi, vpr, idf, chi_fac, eta_fac, index,
+                                     fs, fe, a.sf, co, a.xhat, a.ehat, a.so1, a.so2, a.sid, a.sc1, a.sc2, a.scid)
+                : opacity_line<false>(Q, TGX, TH2X, a.q + lo, a.wq + lo, sum_wphi, vpr, idf, chi_fac, eta_fac,
+                                      index, fs, fe, a.sf, co, a.xhat, a.ehat, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr);
     if (sum_wphi) {
       const int64_t go = (int64_t)l * nc + c;
       a.wphi[go] = a.wphi[go] + a.wmu * (float)ws;
