@@ -75,6 +75,7 @@ class AliConfig:
     conv: float = 1e-4
     ng: bool = True           # Ng acceleration (the reference's automatic trigger)
     precond: bool = True      # preconditioned (ALI) rates; False: plain Lambda iteration
+    fused: bool = True        # one kernel per (window, direction) for continuum, lines and units
 
 
 @dataclass
@@ -345,18 +346,23 @@ class NlteProblem:
         for r, v in enumerate(self.vec):
             vproj = (self.vz * (v[2] / synth.C_LIGHT)).float().contiguous()
             d = dict(w.const, vproj=vproj, **cells)
-            xhat.zero_()
-            torch.mul(w.chi_c, w.Bnu, out=ehat)
-            ctx.convert_to_work_units(w.chi_c, xhat, ehat, w.beta)
-            # the split tracking starts from everything that is not a line of the atom
-            # (the continuum) as the one contribution of index 0
-            # (SRC/rt_patch_mod.f90:1467-1488)
-            sp["split_opac1"].copy_(ehat)
-            sp["split_chi1"].copy_(xhat)
-            for k in ("split_opac2", "split_chi2", "split_id", "split_chi_id"):
-                sp[k].zero_()
-            ctx.bb_opacity(d, xhat, ehat, wmu=float(self.wj[r]), split=sp, split_offset=0)
-            ctx.work_to_transfer(xhat, ehat, w.beta, split_opac1=sp["split_opac1"], split_opac2=sp["split_opac2"])
+            if self.cfg.fused:
+                # the same steps as below in one kernel (m3drt_window_opacity)
+                ctx.window_opacity(d, w.chi_c, w.Bnu, w.beta, xhat, ehat, split=sp, split_offset=0)
+            else:
+                xhat.zero_()
+                torch.mul(w.chi_c, w.Bnu, out=ehat)
+                ctx.convert_to_work_units(w.chi_c, xhat, ehat, w.beta)
+                # the split tracking starts from everything that is not a line of the
+                # atom (the continuum) as the one contribution of index 0
+                # (SRC/rt_patch_mod.f90:1467-1488)
+                sp["split_opac1"].copy_(ehat)
+                sp["split_chi1"].copy_(xhat)
+                for k in ("split_opac2", "split_chi2", "split_id", "split_chi_id"):
+                    sp[k].zero_()
+                ctx.bb_opacity(d, xhat, ehat, wmu=float(self.wj[r]), split=sp, split_offset=0)
+                ctx.work_to_transfer(xhat, ehat, w.beta, split_opac1=sp["split_opac1"],
+                                     split_opac2=sp["split_opac2"])
             chi, S = xhat, ehat
             if sweep64:
                 o64 = [t.double() for t in (I, Psi, Irem, esc)]

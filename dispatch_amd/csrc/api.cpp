@@ -357,6 +357,28 @@ int m3drt_bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r) {
   return m3drt::bb_opacity(ctx, r);
 }
 
+int m3drt_window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r, const m3drt_window_args *w) {
+  if (!ctx || !r || !w) return set_error(M3DRT_EINVAL, "m3drt_window_opacity: NULL argument");
+  if (r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_window_opacity: inconsistent sizes");
+  if (r->nf == 0 || r->ncell == 0) return M3DRT_OK;
+  if (!r->iblue || !r->ired || !r->lev_i || !r->lev_j || !r->Bij || !r->lower_off || !r->q || !r->nstar ||
+      !r->pop_ratio || !r->inv_df || !r->vproj || !r->e_kT || !r->adamp || !r->xhat || !r->ehat || !w->chi_cont ||
+      !w->Bnu || !w->beta)
+    return set_error(M3DRT_EINVAL, "m3drt_window_opacity: missing array");
+  if (!(r->flags & M3DRT_BBOP_LTE) && !r->b)
+    return set_error(M3DRT_EINVAL, "m3drt_window_opacity: departures b needed outside LTE");
+  if ((r->flags & M3DRT_BBOP_SPLIT) && (!r->split_opac1 || !r->split_opac2 || !r->split_chi1 ||
+                                        !r->split_chi2 || !r->split_id || !r->split_chi_id))
+    return set_error(M3DRT_EINVAL, "m3drt_window_opacity: split tracking needs its six arrays");
+  if (r->wphi_line) return set_error(M3DRT_EINVAL, "m3drt_window_opacity: no wphi sums (use m3drt_bb_opacity)");
+  if (w->flags & ~M3DRT_UNITS_SUPPRESS_MASERS)
+    return set_error(M3DRT_EINVAL, "m3drt_window_opacity: flags other than M3DRT_UNITS_SUPPRESS_MASERS");
+  if (r->sf < 1 || r->sc < 1) return set_error(M3DRT_EINVAL, "m3drt_window_opacity: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::window_opacity(ctx, r, w);
+}
+
 int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
   if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_bf_rates: NULL argument");
   if (r->ncont < 0 || r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)

@@ -90,8 +90,12 @@ __global__ void __launch_bounds__(256) b1d_coef(const int32_t *__restrict__ nd_,
     // ih rounded step by step as bot_setup forms it (HOP/qlg_mod.f90:524-540, no
     // fused multiply-add; IEEE division), and the Doppler scale as two products:
     // the profile's table index follows from them
-    const float dd = __fadd_rn(1.0f, __fmul_rn(ad, 0.60602209212341318f));
-    const float ih = dd / __fadd_rn(__fmul_rn(ad, dd), 0.8325546112f);
+    float ih;
+    {
+#pragma clang fp contract(off)
+      const float dd = 1.0f + ad * 0.60602209212341318f;
+      ih = dd / (ad * dd + 0.8325546112f);
+    }
     const float rr = ad * ih, qq = rr * (1.0f - rr), scl = 1.7724538509f * ih, qs = qq * scl;
     const float x = 2.0f * rr - 1.0f;
     const float bL = (0.3183098862f * rr) * scl;

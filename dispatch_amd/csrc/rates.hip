@@ -16,6 +16,7 @@
 // is what add_largest's order dependence needs; no two lanes touch one point.
 // QLG-bot's tables sit in LDS.  Line-rate sums are FP32 with Kahan compensation.
 #include "m3drt_internal.hpp"
+#include <cfloat>
 #include <cmath>
 #include <algorithm>
 
@@ -252,6 +253,8 @@ __device__ __forceinline__ double opacity_line(const Qlg &Q, const float *TGX, c
                                                float *__restrict__ so1, float *__restrict__ so2,
                                                int32_t *__restrict__ sid, float *__restrict__ sc1,
                                                float *__restrict__ sc2, int32_t *__restrict__ scid) {
+  // the accumulations rounded product and sum apart, as the reference rounds them
+#pragma clang fp contract(off)
   double ws = 0.0;
 #pragma unroll 4
   for (int f = fs; f <= fe; ++f) {
@@ -310,6 +313,136 @@ __global__ void __launch_bounds__(256) bb_opacity_kernel(const OpacArgs a) {
       a.wphi[go] = a.wphi[go] + a.wmu * (float)ws;
     }
   }
+}
+
+// ---------------------------------------------------------------- one window, fused
+
+struct WinArgs {
+  OpacArgs o;
+  const float *cont, *Bnu, *beta;
+  float floor_frac;
+  int suppress;
+  unsigned long long *n_floored;
+};
+
+// convert_to_work_units, the split tracking seeded with the continuum as contribution
+// 0, add_bound_bound_opacities with add_largest, and work_to_transfer_units, for one
+// cell's points of a window, frequency by frequency: each point's working values stay
+// in registers from the continuum to (chi, S), and are written once.  The same
+// operations in the same order as m3drt_convert_to_work_units, m3drt_bb_opacity
+// (lines in index order at each point) and m3drt_work_to_transfer.  A line's per-cell
+// factors are formed when the frequency loop enters its window (lines cover
+// contiguous frequency ranges, so a lane re-forms them only where windows overlap).
+// The frequency loop of one cell; the arrays are distinct (restrict), so the reads of
+// the next points need not wait for this point's writes.
+template <bool SPLIT>
+__device__ __forceinline__ unsigned long long window_cell(
+    const OpacArgs &a, const float *TGX, const float *TH2X, int c, float floor_frac, int suppress,
+    const float *__restrict__ cont, const float *__restrict__ Bnu, const float *__restrict__ beta,
+    float *__restrict__ xhat, float *__restrict__ ehat, float *__restrict__ so1, float *__restrict__ so2,
+    int32_t *__restrict__ sidp, float *__restrict__ sc1, float *__restrict__ sc2, int32_t *__restrict__ scidp) {
+  // every product and sum rounded on its own, as the separate kernels and the
+  // reference round them
+#pragma clang fp contract(off)
+  const int nc = a.ncell;
+  const float idf = a.inv_df[c], vpr = a.vproj[c], ekt = a.e_kT[c];
+  unsigned long long floored = 0;
+  int cur = -1;
+  float chi_fac = 0.0f, eta_fac = 0.0f;
+  Qlg Q(0.0f);
+#pragma unroll 2
+  for (int f = 0; f < a.nf; ++f) {
+    const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
+    const float cc = cont[o], bn = Bnu[o], bt = beta[o];
+    // the continuum (eta = chi B) in working units (convert_to_work_units)
+    const float e0 = cc * bn;
+    float x = cc + e0;
+    float e = e0 / bt;
+    float s1 = e, s2 = 0.0f, c1 = x, c2 = 0.0f;
+    int32_t sid = 0, cid = 0;
+    for (int l = 0; l < a.nline; ++l) {
+      const int fs = max(a.iblue[l] - 1, 0), fe = min(a.ired[l] - 1, a.nf - 1);
+      if (f < fs || f > fe) continue;
+      const int li = a.lev_i[l] - 1, lj = a.lev_j[l] - 1;
+      if (li < 0 || li >= a.nk || lj < 0 || lj >= a.nk) continue;
+      if (l != cur) {
+        const float lower_off = a.lower_off[l];
+        const float ni = (float)(a.nstar[(int64_t)li * nc + c] * a.pop_ratio[c]);
+        float bi = 1.0f, bj = 1.0f;
+        if (!a.lte) {
+          bi = (float)a.b[(int64_t)li * nc + c];
+          bj = (float)a.b[(int64_t)lj * nc + c];
+        }
+        if (lower_off != 0) {
+          const float eoff = (float)exp((double)(-lower_off * ekt));
+          bi = bi * eoff;
+          bj = bj * eoff;
+        }
+        const float Rij = (float)((double)ni * a.Bij[l]);
+        eta_fac = Rij * bj;
+        chi_fac = Rij * bi;
+        Q = Qlg(a.adamp[(int64_t)l * nc + c]);
+        cur = l;
+      }
+      const float vph = Q.profile(TGX, TH2X, dev::ro(a.q)[(int64_t)l * a.nf + f], vpr, idf);
+      const int index = a.split_offset + l + 1;
+      x = x + vph * chi_fac;
+      e = e + vph * eta_fac;
+      if (SPLIT) {
+        add_largest(vph * eta_fac, index, &s1, &s2, &sid);
+        add_largest(vph * chi_fac, index, &c1, &c2, &cid);
+      }
+    }
+    // work_to_transfer_units
+    float eta, chi;
+    if (SPLIT) {
+      const float eta_rest = bt * s2;
+      eta = bt * s1 + eta_rest;
+    } else {
+      eta = e * bt;
+    }
+    chi = x - eta;
+    const float floor = fmaxf(floor_frac * fabsf(x), FLT_MIN);
+    if (suppress) {
+      chi = fmaxf(chi, floor);
+    } else if (fabsf(chi) < floor) {
+      chi = copysignf(floor, chi);
+      ++floored;
+    }
+    xhat[o] = chi;
+    ehat[o] = eta / chi;
+    if (SPLIT) {
+      so1[o] = s1;
+      so2[o] = s2;
+      sidp[o] = sid;
+      sc1[o] = c1;
+      sc2[o] = c2;
+      scidp[o] = cid;
+    }
+  }
+  return floored;
+}
+
+// convert_to_work_units, the split tracking seeded with the continuum as contribution
+// 0, add_bound_bound_opacities with add_largest, and work_to_transfer_units, for one
+// cell's points of a window, frequency by frequency: each point's working values stay
+// in registers from the continuum to (chi, S), and are written once.  The same
+// operations in the same order as m3drt_convert_to_work_units, m3drt_bb_opacity
+// (lines in index order at each point) and m3drt_work_to_transfer.  A line's per-cell
+// factors are formed when the frequency loop enters its window (lines cover
+// contiguous frequency ranges, so a lane re-forms them only where windows overlap).
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) window_opacity_kernel(const WinArgs w) {
+  extern __shared__ float tab[];
+  const OpacArgs &a = w.o;
+  load_tables(tab, a.tab);
+  const float *TGX = tab, *TH2X = tab + QLG_NX + 1;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  const unsigned long long floored =
+      window_cell<SPLIT>(a, TGX, TH2X, c, w.floor_frac, w.suppress, w.cont, w.Bnu, w.beta, a.xhat, a.ehat, a.so1,
+                         a.so2, a.sid, a.sc1, a.sc2, a.scid);
+  if (w.n_floored && floored) atomicAdd(w.n_floored, floored);
 }
 
 // ---------------------------------------------------------------- bound-free rates
@@ -428,6 +561,33 @@ int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
   a.sf = r->sf; a.sc = r->sc;
   a.gij = r->gamma_ij; a.gji = r->gamma_ji;
   return launch_cells(ctx, bf_rates_kernel, a, r->ncell, 0);
+}
+
+int window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r, const m3drt_window_args *wa) {
+  int rc = qlg_tables(ctx);
+  if (rc) return rc;
+  WinArgs w;
+  OpacArgs &a = w.o;
+  a.nline = r->nline; a.nf = r->nf; a.ncell = r->ncell; a.nk = r->nk;
+  a.lte = (r->flags & M3DRT_BBOP_LTE) ? 1 : 0;
+  a.split = (r->flags & M3DRT_BBOP_SPLIT) ? 1 : 0;
+  a.split_offset = r->split_offset;
+  a.iblue = r->iblue; a.ired = r->ired; a.lev_i = r->lev_i; a.lev_j = r->lev_j; a.wphi_line = nullptr;
+  a.Bij = r->Bij; a.wq = nullptr; a.b = r->b; a.nstar = r->nstar; a.pop_ratio = r->pop_ratio;
+  a.lower_off = r->lower_off; a.q = r->q; a.inv_df = r->inv_df; a.vproj = r->vproj; a.e_kT = r->e_kT;
+  a.adamp = r->adamp;
+  a.xhat = r->xhat; a.ehat = r->ehat;
+  a.so1 = r->split_opac1; a.so2 = r->split_opac2; a.sc1 = r->split_chi1; a.sc2 = r->split_chi2;
+  a.sid = r->split_id; a.scid = r->split_chi_id; a.wphi = nullptr;
+  a.sf = r->sf; a.sc = r->sc; a.wmu = r->wmu;
+  a.tab = ctx->qlg_tab;
+  w.cont = wa->chi_cont; w.Bnu = wa->Bnu; w.beta = wa->beta;
+  w.floor_frac = (float)wa->maser_chi_floor;
+  w.suppress = (wa->flags & M3DRT_UNITS_SUPPRESS_MASERS) ? 1 : 0;
+  w.n_floored = reinterpret_cast<unsigned long long *>(wa->n_floored);
+  const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
+  if (a.split) return launch_cells(ctx, window_opacity_kernel<true>, w, r->ncell, lds);
+  return launch_cells(ctx, window_opacity_kernel<false>, w, r->ncell, lds);
 }
 
 }  // namespace m3drt

@@ -552,6 +552,43 @@ def _bb_opacity(self, d, xhat, ehat, wmu=0.0, lte=False, split=None, split_offse
     _check(L.m3drt_bb_opacity(self._h, C.byref(a)), "m3drt_bb_opacity")
 
 
+class WindowArgs(C.Structure):
+    _fields_ = [("flags", C.c_int32), ("maser_chi_floor", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("chi_cont", "Bnu", "beta", "n_floored")]
+
+
+def _window_opacity(self, d, chi_cont, Bnu, beta, xhat, ehat, split=None, split_offset=0, lte=False,
+                    suppress_masers=False, maser_chi_floor=1e-4, n_floored=None, layout="native"):
+    """One direction's opacity pass over a window in one kernel (m3drt_window_opacity):
+    the continuum, convert_to_work_units, the split tracking seeded with the continuum,
+    the lines with add_largest, and work_to_transfer_units.  xhat receives chi and
+    ehat S; split (the six tracking arrays) is overwritten."""
+    L = _ali_lib()
+    if not getattr(L, "_win_ok", False):
+        L.m3drt_window_opacity.argtypes = [C.c_void_p, C.POINTER(BbOpacityArgs), C.POINTER(WindowArgs)]
+        L.m3drt_window_opacity.restype = C.c_int
+        L._win_ok = True
+    sp = split or {}
+    _check_dev(d, BB_DTYPES, "window_opacity")
+    _check_dev(dict(sp, xhat=xhat, ehat=ehat), BB_DTYPES, "window_opacity")
+    _f32_like(xhat, chi_cont, Bnu, beta)
+    nline, nf = d["q"].shape
+    nk, ncell = d["nstar"].shape
+    sf, sc = _strides(layout, nf, ncell)
+    flags = (M3DRT_BBOP_LTE if lte else 0) | (M3DRT_BBOP_SPLIT if split is not None else 0)
+    a = BbOpacityArgs(nline, nf, ncell, nk, flags, split_offset, 0.0, sf, sc,
+                      *[_p(d.get(k)) for k in ("iblue", "ired", "lev_i", "lev_j", "Bij", "lower_off", "q")], None,
+                      None, *[_p(d.get(k)) for k in ("b", "nstar", "pop_ratio", "inv_df", "vproj", "e_kT", "adamp")],
+                      _p(xhat), _p(ehat),
+                      *[_p(sp.get(k)) for k in ("split_opac1", "split_opac2", "split_chi1", "split_chi2", "split_id",
+                                                "split_chi_id")],
+                      None)
+    w = WindowArgs(M3DRT_UNITS_SUPPRESS_MASERS if suppress_masers else 0, maser_chi_floor, _p(chi_cont), _p(Bnu),
+                   _p(beta), _p(n_floored))
+    self._bind_stream()
+    _check(L.m3drt_window_opacity(self._h, C.byref(a), C.byref(w)), "m3drt_window_opacity")
+
+
 def _bf_rates(self, d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0, layout="native"):
     """bound_free_rates on the device; d: BF_DTYPES tensors, spectra (nf, ncell)
     native or (ncell, nf) reference; gamma arrays (nline + ncont, ncell)."""
@@ -570,6 +607,7 @@ def _bf_rates(self, d, nline, gamma_ij, gamma_ji, precond=False, cut_x=70.0, lay
 
 Context.bb_rates = _bb_rates
 Context.bb_opacity = _bb_opacity
+Context.window_opacity = _window_opacity
 Context.bf_rates = _bf_rates
 
 

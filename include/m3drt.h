@@ -409,6 +409,28 @@ int m3drt_background_emission(m3drt_ctx *ctx, const m3drt_background_args *args)
 int m3drt_convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *chi_cont, float *xhat, float *ehat,
                                 const float *beta);
 
+/* ------------------------------------------------------------------ one window, fused */
+
+/* The continuum and conversion arrays of a fused window (every (f, cell) array in the
+ * layout of the m3drt_bb_opacity_args it goes with). */
+typedef struct {
+  int32_t flags;                       /* M3DRT_UNITS_SUPPRESS_MASERS, or 0 */
+  double maser_chi_floor;              /* as m3drt_units_args */
+  const float *chi_cont, *Bnu, *beta;  /* (f, cell): the continuum opacity, B_nu, beta */
+  int64_t *n_floored;                  /* device counter or NULL */
+} m3drt_window_args;
+
+/* One direction's opacity pass over a window in one kernel: foreground's continuum
+ * (eta = chi_cont B_nu), convert_to_work_units (SRC/rt_patch_mod.f90:3518-3536), the
+ * split tracking seeded with the continuum as contribution 0 (SRC/rt_patch_mod.f90:
+ * 1467-1488), add_bound_bound_opacities with add_largest (:2689-2831, 3150-3169) and
+ * work_to_transfer_units (:3571-3651, with M3DRT_BBOP_SPLIT its split form).  Writes
+ * chi into args->xhat, S into args->ehat and, with M3DRT_BBOP_SPLIT, the six tracking
+ * arrays; the same values as that sequence of m3drt_convert_to_work_units,
+ * m3drt_bb_opacity and m3drt_work_to_transfer.  No wphi sums (wphi_line must be NULL)
+ * and no line scattering (M3DRT_UNITS_SCAT). */
+int m3drt_window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *args, const m3drt_window_args *win);
+
 /* ------------------------------------------------------------------ eta-step flags */
 
 #define M3DRT_ETA_GHOST 1  /* the upstream ghost plane fires too (not the top of the domain) */

@@ -58,6 +58,18 @@ class OracleCtx:
         for k in split:
             split[k].copy_(torch.from_numpy(np.asarray(r[k])))
 
+    def window_opacity(self, d, chi_cont, Bnu, beta, xhat, ehat, split=None, split_offset=0):
+        # the fused entry point's definition: the sequence it replaces
+        xhat.zero_()
+        torch.mul(chi_cont, Bnu, out=ehat)
+        self.convert_to_work_units(chi_cont, xhat, ehat, beta)
+        split["split_opac1"].copy_(ehat)
+        split["split_chi1"].copy_(xhat)
+        for k in ("split_opac2", "split_chi2", "split_id", "split_chi_id"):
+            split[k].zero_()
+        self.bb_opacity(d, xhat, ehat, split=split, split_offset=split_offset)
+        self.work_to_transfer(xhat, ehat, beta, split_opac1=split["split_opac1"], split_opac2=split["split_opac2"])
+
     def work_to_transfer(self, xhat, ehat, beta, split_opac1=None, split_opac2=None):
         s1, s2 = np.ascontiguousarray(n(split_opac1)), np.ascontiguousarray(n(split_opac2))
         c, s, _ = O.work_to_transfer(n(xhat), n(ehat), n(beta), split_opac1=s1, split_opac2=s2)

@@ -57,3 +57,48 @@ def test_ali_strong_collisions_give_lte(ctx):
     p.solve()
     b = p.b.cpu().numpy().reshape(p.atom.nk, cfg.nz, -1)
     assert np.abs(b[:, :cfg.nz // 2] - 1).max() < 1e-3, np.abs(b[:, :cfg.nz // 2] - 1).max()
+
+
+def test_window_opacity_equals_the_sequence(ctx):
+    """m3drt_window_opacity against the sequence it fuses (the continuum product,
+    convert_to_work_units, the seeded split tracking, bb_opacity, work_to_transfer) on
+    every window and direction of the small problem at non-LTE populations: chi, S and
+    the six tracking arrays bit for bit (both round every product and sum apart)."""
+    from dispatch_amd.ali import NlteProblem
+    from dispatch_amd import synth
+    p = NlteProblem(ctx, small())
+    g = torch.Generator(device="cuda").manual_seed(3)
+    p.b.mul_(torch.exp(0.3 * torch.randn(p.b.shape, generator=g, dtype=torch.float64, device="cuda")))
+    print("window opacity fused vs sequence", flush=True)
+    for wi, w in enumerate(p.windows):
+        l0, l1 = w.lines[0], w.lines[-1] + 1
+        nf, nc = w.nf, p.ncell
+        for r, v in enumerate(p.vec):
+            vproj = (p.vz * (v[2] / synth.C_LIGHT)).float().contiguous()
+            d = dict(w.const, vproj=vproj, b=p.b, nstar=p.nstar, pop_ratio=p.pop_ratio, inv_df=p.inv_df,
+                     e_kT=p.e_kT32, adamp=p.adamp[l0:l1])
+            out = []
+            for fused in (False, True):
+                xhat = torch.full((nf, nc), 7.0, dtype=torch.float32, device="cuda")
+                ehat = torch.full_like(xhat, 7.0)
+                sp = dict(split_opac1=torch.full_like(xhat, 7.0), split_opac2=torch.full_like(xhat, 7.0),
+                          split_chi1=torch.full_like(xhat, 7.0), split_chi2=torch.full_like(xhat, 7.0),
+                          split_id=torch.full((nf, nc), 7, dtype=torch.int32, device="cuda"),
+                          split_chi_id=torch.full((nf, nc), 7, dtype=torch.int32, device="cuda"))
+                if fused:
+                    ctx.window_opacity(d, w.chi_c, w.Bnu, w.beta, xhat, ehat, split=sp, split_offset=0)
+                else:
+                    xhat.zero_()
+                    torch.mul(w.chi_c, w.Bnu, out=ehat)
+                    ctx.convert_to_work_units(w.chi_c, xhat, ehat, w.beta)
+                    sp["split_opac1"].copy_(ehat)
+                    sp["split_chi1"].copy_(xhat)
+                    for k in ("split_opac2", "split_chi2", "split_id", "split_chi_id"):
+                        sp[k].zero_()
+                    ctx.bb_opacity(d, xhat, ehat, split=sp, split_offset=0)
+                    ctx.work_to_transfer(xhat, ehat, w.beta, split_opac1=sp["split_opac1"],
+                                         split_opac2=sp["split_opac2"])
+                out.append(dict(sp, chi=xhat, S=ehat))
+            torch.cuda.synchronize()
+            for k in out[0]:
+                assert torch.equal(out[0][k], out[1][k]), (wi, r, k, float((out[0][k] - out[1][k]).abs().max()))
