@@ -379,6 +379,20 @@ int m3drt_window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r, const m
   return m3drt::window_opacity(ctx, r, w);
 }
 
+int m3drt_rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *r) {
+  if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_rotate_array: NULL argument");
+  if (r->nf < 1 || r->nx < 1 || r->ny < 1 || r->nz < 1 || r->nk < 1 || r->nz > 65535 || r->nk > 65535)
+    return set_error(M3DRT_EINVAL, "m3drt_rotate_array: bad sizes");
+  if (!r->in || !r->out) return set_error(M3DRT_EINVAL, "m3drt_rotate_array: missing array");
+  if (r->in == r->out) return set_error(M3DRT_EINVAL, "m3drt_rotate_array: out of place only");
+  if (!(r->vec[2] != 0.0) || !(r->ds[0] > 0.0) || !(r->ds[1] > 0.0))
+    return set_error(M3DRT_EINVAL, "m3drt_rotate_array: vec[2] must be nonzero and ds positive");
+  if (r->cs_f < 1 || r->cs_x < 1 || r->cs_y < 1 || r->cs_z < 1 || r->cs_k < 1)
+    return set_error(M3DRT_EINVAL, "m3drt_rotate_array: strides must be >= 1");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::rotate_array(ctx, r);
+}
+
 int m3drt_bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r) {
   if (!ctx || !r) return set_error(M3DRT_EINVAL, "m3drt_bf_rates: NULL argument");
   if (r->ncont < 0 || r->nline < 0 || r->nf < 0 || r->ncell < 0 || r->nk < 1)

@@ -51,6 +51,7 @@ int convert_to_work_units(m3drt_ctx *ctx, int64_t n, const float *cont, float *x
                           const float *beta);
 int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r);
 int window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r, const m3drt_window_args *w);
+int rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *r);
 int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r);
 
 }  // namespace m3drt

@@ -349,6 +349,26 @@ module m3drt
     type(c_ptr) :: n_floored = c_null_ptr
   end type m3drt_window_args
 
+  type, bind(C), public :: m3drt_rotate_args
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: nx
+    integer(c_int32_t) :: ny
+    integer(c_int32_t) :: nz
+    integer(c_int32_t) :: nk
+    integer(c_int32_t) :: reserved
+    integer(c_int64_t) :: cs_f
+    integer(c_int64_t) :: cs_x
+    integer(c_int64_t) :: cs_y
+    integer(c_int64_t) :: cs_z
+    integer(c_int64_t) :: cs_k
+    real(c_double) :: vec(3)
+    real(c_double) :: ds(3)
+    real(c_double) :: bot2pivot
+    real(c_double) :: ray_offset(2)
+    type(c_ptr) :: in = c_null_ptr
+    type(c_ptr) :: out = c_null_ptr
+  end type m3drt_rotate_args
+
   type, bind(C), public :: m3drt_b1d_batch
     integer(c_int32_t) :: natm
     integer(c_int32_t) :: reserved
@@ -388,7 +408,7 @@ module m3drt
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
   public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi, m3drt_line_opacity
-  public :: m3drt_window_opacity
+  public :: m3drt_window_opacity, m3drt_rotate_array
   public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
   public :: m3drt_convert_to_work_units, m3drt_background_emission, m3drt_update_eta_mask
   public :: m3drt_normalize_rates, m3drt_rate_solve_f64
@@ -523,6 +543,11 @@ module m3drt
       type(m3drt_rate_solve_args), intent(in) :: args
       type(m3drt_rate_solve_result), intent(inout) :: res
     end function m3drt_rate_solve_f64
+    integer(c_int) function m3drt_rotate_array(ctx, args) bind(C, name='m3drt_rotate_array')
+      import :: c_int, c_ptr, m3drt_rotate_args
+      type(c_ptr), value :: ctx
+      type(m3drt_rotate_args), intent(in) :: args
+    end function m3drt_rotate_array
     integer(c_int) function m3drt_window_opacity(ctx, args, win) bind(C, name='m3drt_window_opacity')
       import :: c_int, c_ptr, m3drt_bb_opacity_args, m3drt_window_args
       type(c_ptr), value :: ctx

@@ -884,3 +884,42 @@ class B1dPlan:
 
 Context.b1d_synth_plain = _b1d_synth_plain
 Context.b1d_plan = lambda self, arrays, lam_ref, tile=None: B1dPlan(self, arrays, lam_ref, tile)
+
+
+# ------------------------------------------------------------------ rotation for the output pass
+
+class RotateArgs(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nf", "nx", "ny", "nz", "nk", "reserved")] + \
+               [(n, C.c_int64) for n in ("cs_f", "cs_x", "cs_y", "cs_z", "cs_k")] + \
+               [("vec", C.c_double * 3), ("ds", C.c_double * 3), ("bot2pivot", C.c_double),
+                ("ray_offset", C.c_double * 2), ("in", C.c_void_p), ("out", C.c_void_p)]
+
+
+def _rotate_array(self, x, out, vec, ds, bot2pivot, ray_offset=(0.0, 0.0), layout="reference"):
+    """rotate_array of the cubes x into out (distinct float32 device tensors of one
+    shape): layout "reference" is (k, z, y, x, f) in C order -- work(f, x, y, z, k);
+    "native" is (k, z, f, y, x)."""
+    L = lib()
+    if not getattr(L, "_rot_ok", False):
+        L.m3drt_rotate_array.argtypes = [C.c_void_p, C.POINTER(RotateArgs)]
+        L.m3drt_rotate_array.restype = C.c_int
+        L._rot_ok = True
+    for t in (x, out):
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.dim() != 5:
+            raise ValueError("rotate_array: contiguous 5-D float32 device tensors")
+    if x.shape != out.shape or x.data_ptr() == out.data_ptr():
+        raise ValueError("rotate_array: out must be a distinct tensor of x's shape")
+    if layout == "reference":
+        nk, nz, ny, nx, nf = x.shape
+        cs = (1, nf, nf * nx, nf * nx * ny, nf * nx * ny * nz)
+    else:
+        nk, nz, nf, ny, nx = x.shape
+        cs = (nx * ny, 1, nx, nx * ny * nf, nx * ny * nf * nz)
+    a = RotateArgs(nf, nx, ny, nz, nk, 0, *cs, (C.c_double * 3)(*vec), (C.c_double * 3)(*ds), bot2pivot,
+                   (C.c_double * 2)(*ray_offset), x.data_ptr(), out.data_ptr())
+    self._bind_stream()
+    _check(L.m3drt_rotate_array(self._h, C.byref(a)), "m3drt_rotate_array")
+
+
+Context.rotate_array = _rotate_array
+

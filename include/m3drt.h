@@ -452,6 +452,26 @@ typedef struct {
  * compaction of them.  Clearing the three arrays is the reference's eta reset. */
 int m3drt_update_eta_mask(m3drt_ctx *ctx, const m3drt_eta_args *args);
 
+/* ------------------------------------------------------------------ rotation for the output pass */
+
+/* Cubes (f, x, y, z, k) addressed by element strides, k counting the cubes (chi and S
+ * of the work array, or J), every z plane shifted periodically with bilinear
+ * interpolation so that the rays of direction vec become vertical, the plane at the
+ * pivot left in place: rotate_array (SRC/rt_patch_mod.f90:3809-3863), out of place. */
+typedef struct {
+  int32_t nf, nx, ny, nz, nk;
+  int32_t reserved;
+  int64_t cs_f, cs_x, cs_y, cs_z, cs_k;  /* element strides, the same for in and out */
+  double vec[3];                         /* the ray's direction (vec[2] != 0) */
+  double ds[3];                          /* cell size */
+  double bot2pivot;                      /* pivot_z - bottom of the patch, as the caller forms it */
+  double ray_offset[2];                  /* rt_output_params ray_offset [cells] */
+  const float *in;
+  float *out;                            /* must not overlap in */
+} m3drt_rotate_args;
+
+int m3drt_rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *args);
+
 /* ------------------------------------------------------------------ batched 1D synthesis */
 
 /* A batch of plane-parallel LTE atmospheres of uneven depth, frequency and angle

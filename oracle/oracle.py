@@ -789,3 +789,20 @@ def b1d_atmosphere(h, m, lam_ref, want_chi=False):
                          sl("mu", m0, nmu), sl("wflux", m0, nmu), float(lam_ref), I, F,
                          None if chi is None else chi.ctypes.data_as(C.c_void_p))
     return (I, F, chi) if want_chi else (I, F)
+
+
+def rotate_array(x, vec, ds, bot2pivot, ray_offset=(0.0, 0.0)):
+    """rotate_array of cubes x (nk, nz, ny, nx, nf) float32 -- the reference's
+    work(f, x, y, z, k) -- returned as a new array."""
+    L = lib()
+    if not getattr(L, "_rot_ok", False):
+        L.orc_rotate_array.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                       C.POINTER(C.c_double)] + [C.c_int] * 5 + [f32p, f32p]
+        L.orc_rotate_array.restype = None
+        L._rot_ok = True
+    x = np.ascontiguousarray(x, np.float32)
+    nk, nz, ny, nx, nf = x.shape
+    out = np.empty_like(x)
+    d3 = lambda v: (C.c_double * len(v))(*[float(t) for t in v])
+    L.orc_rotate_array(d3(vec), d3(ds), float(bot2pivot), d3(ray_offset), nf, nx, ny, nz, nk, x, out)
+    return out
