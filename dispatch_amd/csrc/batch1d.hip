@@ -14,8 +14,9 @@
 //
 // Mapping: a wavefront takes one item = (atmosphere, 64 FPL consecutive frequencies);
 // lane l holds frequencies f0 + l + 64 j, j < FPL.  Per depth point the wave-uniform
-// values (step, temperature, Doppler speed, continuum) and per line the profile
-// coefficients are scalar loads shared by the 64 lanes.  The persistent kernel's waves
+// values (step, temperature, Doppler speed, continuum) are scalar loads shared by the
+// 64 lanes; the lines that can reach the item are found with one vector load of 64
+// lines' frequency ranges and a ballot, and only their profile records are read.  The persistent kernel's waves
 // take items from a counter until the list (sorted by decreasing cost) is exhausted;
 // the plain kernel gives each atmosphere its own launch over a static grid of items.
 #include "m3drt_internal.hpp"
@@ -153,6 +154,7 @@ __device__ __forceinline__ void b1d_item(const B1dArgs &a, const float *TGX, con
   float I[M3DRT_B1D_MAXMU][FPL], xu[FPL], Su[FPL];
   // opacity ratio and source function at depth point d
   const cptr<float> cont_a = ro(a.cont_a), cont_b = ro(a.cont_b), hckT = ro(a.hckT);
+  const int flo = f0, fhi = f0 + 64 * FPL - 1;
   auto point = [&](int d, float *x, float *S) {
     const int64_t dd = d0 + d;
     const float ca = cont_a[dd], cb = cont_b[dd], hc = hckT[dd];
@@ -162,24 +164,41 @@ __device__ __forceinline__ void b1d_item(const B1dArgs &a, const float *TGX, con
       const float e = __builtin_amdgcn_exp2f(-1.44269504088896341f * hc * rl[j]);
       S[j] = e / (1.0f - e);
     }
-    // the lines whose window can reach the wave's frequencies (a scalar test on the
-    // precomputed range), then the lanes' own test
+    // the lines whose window can reach the wave's frequencies: 64 lines' ranges per
+    // vector load, one lane each, gathered into a wave mask; then only those lines'
+    // records, each read (scalar) while the previous one is summed, and the lanes'
+    // own FP32 test
     const cptr<float4> c = ro(a.coef) + 2 * (r0 + (int64_t)d * nl);
-    const cptr<int2> wn = ro(a.win) + (r0 + (int64_t)d * nl);
-    for (int l = 0; l < nl; ++l) {
-      const int2 w = wn[l];
-      if (w.y < f0 || w.x > f0 + 64 * FPL - 1) continue;
-      const float4 p = c[2 * l], q = c[2 * l + 1];
+    const int2 *wn = a.win + (r0 + (int64_t)d * nl);
+    for (int base = 0; base < nl; base += 64) {
+      bool hit = false;
+      if (base + lane < nl) {
+        const int2 w = wn[base + lane];
+        hit = !(w.y < flo || w.x > fhi);
+      }
+      uint64_t m = __ballot(hit);
+      if (!m) continue;
+      int cur = base + (int)__builtin_ctzll(m);
+      m &= m - 1;
+      float4 p = c[2 * cur], q = c[2 * cur + 1];
+      for (;;) {
+        const int nxt = m ? base + (int)__builtin_ctzll(m) : cur;
+        const float4 pn = c[2 * nxt], qn = c[2 * nxt + 1];
 #pragma unroll
-      for (int j = 0; j < FPL; ++j) {
-        const float dl = fabsf(xf[j] - q.w);
-        if (dl <= q.z) {
-          // qlg_bot_line_eval, HOP/qlg_mod.f90:556-571
-          const float xs = dl * p.x, u = xs * xs, L = dev::rcp(1.0f + u);
-          const int ig = (int)roundf(fminf(xs * 800.0f, (float)QLG_NX));
-          const float K = TGX[ig] * p.z + L * (p.y + L * (q.x + p.w * TH2X[ig]));
-          x[j] += q.y * K;
+        for (int j = 0; j < FPL; ++j) {
+          const float dl = fabsf(xf[j] - q.w);
+          if (dl <= q.z) {
+            // qlg_bot_line_eval, HOP/qlg_mod.f90:556-571
+            const float xs = dl * p.x, u = xs * xs, L = dev::rcp(1.0f + u);
+            const int ig = (int)roundf(fminf(xs * 800.0f, (float)QLG_NX));
+            const float K = TGX[ig] * p.z + L * (p.y + L * (q.x + p.w * TH2X[ig]));
+            x[j] += q.y * K;
+          }
         }
+        if (!m) break;
+        m &= m - 1;
+        p = pn;
+        q = qn;
       }
     }
   };
