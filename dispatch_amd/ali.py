@@ -76,6 +76,7 @@ class AliConfig:
     ng: bool = True           # Ng acceleration (the reference's automatic trigger)
     precond: bool = True      # preconditioned (ALI) rates; False: plain Lambda iteration
     fused: bool = True        # one kernel per (window, direction) for continuum, lines and units
+    graph: bool = False       # replay the rate pass of an iteration as one captured HIP graph
 
 
 @dataclass
@@ -221,6 +222,8 @@ class NlteProblem:
         self.windows = [self._window(lines) for lines in
                         [list(range(w, min(atom.nline, w + cfg.lines_per_window)))
                          for w in range(0, atom.nline, cfg.lines_per_window)]]
+        self._graph = None
+        self._graph_out = None
         self.history = []
         self.ng = NgAcceleration() if cfg.ng else None
         self._oldb = []
@@ -379,7 +382,25 @@ class NlteProblem:
 
     def rates(self):
         """The radiative rate sums of one iteration: (gamma_ij, gamma_ji), (nrad, ncell)
-        float32, normalised (normalize_profiles)."""
+        float32, normalised (normalize_profiles).  With cfg.graph the whole pass (every
+        window's opacity, sweeps and rate sums, the continua, the normalisation) is
+        captured once, after an eager pass that creates the tables and scratch it
+        uses, and replayed from then on: the same kernels on the same buffers, whose
+        contents (departure coefficients) the iteration changes in place."""
+        if not (self.cfg.graph and self.device == "cuda"):
+            return self._rates()
+        if self._graph is None:
+            out = self._rates()
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._graph_out = self._rates()
+            self._graph = g
+            return out
+        self._graph.replay()
+        return self._graph_out
+
+    def _rates(self):
         atom = self.atom
         nrad = len(atom.ti)
         gij = torch.zeros((nrad, self.ncell), dtype=torch.float32, device=self.device)
@@ -447,10 +468,11 @@ def main():
     ap.add_argument("--nz", type=int, default=120)
     ap.add_argument("--max-iter", type=int, default=60)
     ap.add_argument("--fp64-sweeps", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the rate pass eagerly every iteration")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
-    cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter)
+    cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph)
     t0 = time.perf_counter()
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
@@ -462,7 +484,7 @@ def main():
                iterations=n, converged=abs(p.history[-1][0]) < cfg.conv, maxerr=[e for e, _ in p.history],
                s_per_iteration=float(np.median(its)),
                line_updates_per_iteration=p.ncell * p.atom.nline * cfg.nq * len(p.vec),
-               b_min=float(p.b.min()), b_max=float(p.b.max()))
+               b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph)
     out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
     if args.fp64_sweeps:
         b32 = p.b.clone()

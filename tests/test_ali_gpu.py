@@ -102,3 +102,24 @@ def test_window_opacity_equals_the_sequence(ctx):
             torch.cuda.synchronize()
             for k in out[0]:
                 assert torch.equal(out[0][k], out[1][k]), (wi, r, k, float((out[0][k] - out[1][k]).abs().max()))
+
+
+def test_graph_replays_of_the_rate_pass_match_eager(ctx):
+    """AliConfig(graph=True): the rate pass captured once and replayed, with the
+    statistical-equilibrium update (eager work) between the replays, gives the eager
+    pass's rate sums and departure coefficients bit for bit, iteration after
+    iteration."""
+    from dispatch_amd.ali import NlteProblem
+    pe = NlteProblem(ctx, small())
+    pg = NlteProblem(ctx, small(graph=True))
+    print("ali rate pass: graph replays vs eager", flush=True)
+    for it in range(4):
+        ge, je = pe.rates()
+        gg, jg = pg.rates()
+        torch.cuda.synchronize()
+        assert torch.equal(ge, gg) and torch.equal(je, jg), it
+        pe.update(ge, je)
+        pg.update(gg, jg)
+        torch.cuda.synchronize()
+        assert torch.equal(pe.b, pg.b), it
+    assert pg._graph is not None
