@@ -306,7 +306,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sweep shards by frequency window, see DESIGN.md)", file=sys.stderr)
+              "(the sweep shards by frequency window)", file=sys.stderr)
         sys.exit(2)
     import torch
     from dispatch_amd import m3drt, synth
