@@ -350,48 +350,84 @@ __device__ __forceinline__ unsigned long long window_cell(
   int cur = -1;
   float chi_fac = 0.0f, eta_fac = 0.0f;
   Qlg Q(0.0f);
-#pragma unroll 2
+  // The lines covering the current frequency as a mask (windows of up to 64 lines),
+  // re-formed only where some line's window starts or ends: a frequency loop that
+  // otherwise scans every line of the window at every frequency.
+  const bool masked = a.nline <= 64;
+  uint64_t cover = 0;
+  int f_next = 0;
+  // one line's contribution at frequency f, lines in index order
+  auto add_line = [&](int l, int f, float &x, float &e, float &s1, float &s2, int32_t &sid, float &c1, float &c2,
+                      int32_t &cid) {
+    const int li = dev::ro(a.lev_i)[l] - 1, lj = dev::ro(a.lev_j)[l] - 1;
+    if (l != cur) {
+      const float lower_off = dev::ro(a.lower_off)[l];
+      const float ni = (float)(a.nstar[(int64_t)li * nc + c] * a.pop_ratio[c]);
+      float bi = 1.0f, bj = 1.0f;
+      if (!a.lte) {
+        bi = (float)a.b[(int64_t)li * nc + c];
+        bj = (float)a.b[(int64_t)lj * nc + c];
+      }
+      if (lower_off != 0) {
+        const float eoff = (float)exp((double)(-lower_off * ekt));
+        bi = bi * eoff;
+        bj = bj * eoff;
+      }
+      const float Rij = (float)((double)ni * dev::ro(a.Bij)[l]);
+      eta_fac = Rij * bj;
+      chi_fac = Rij * bi;
+      Q = Qlg(a.adamp[(int64_t)l * nc + c]);
+      cur = l;
+    }
+    const float vph = Q.profile(TGX, TH2X, dev::ro(a.q)[(int64_t)l * a.nf + f], vpr, idf);
+    const int index = a.split_offset + l + 1;
+    x = x + vph * chi_fac;
+    e = e + vph * eta_fac;
+    if (SPLIT) {
+      add_largest(vph * eta_fac, index, &s1, &s2, &sid);
+      add_largest(vph * chi_fac, index, &c1, &c2, &cid);
+    }
+  };
+  auto live = [&](int l, int f) {
+    const int fs = max(dev::ro(a.iblue)[l] - 1, 0), fe = min(dev::ro(a.ired)[l] - 1, a.nf - 1);
+    const int li = dev::ro(a.lev_i)[l] - 1, lj = dev::ro(a.lev_j)[l] - 1;
+    return f >= fs && f <= fe && li >= 0 && li < a.nk && lj >= 0 && lj < a.nk;
+  };
+  // the next point's reads are issued before this point's writes
+  int64_t o = (int64_t)c * a.sc;
+  float cc = cont[o], bn = Bnu[o], bt = beta[o];
   for (int f = 0; f < a.nf; ++f) {
-    const int64_t o = (int64_t)f * a.sf + (int64_t)c * a.sc;
-    const float cc = cont[o], bn = Bnu[o], bt = beta[o];
+    const int64_t on = o + a.sf;
+    float ccn = 0.0f, bnn = 0.0f, btn = 0.0f;
+    if (f + 1 < a.nf) {
+      ccn = cont[on];
+      bnn = Bnu[on];
+      btn = beta[on];
+    }
+    if (masked && f >= f_next) {
+      cover = 0;
+      f_next = a.nf;
+      for (int l = 0; l < a.nline; ++l) {
+        const int fs = max(dev::ro(a.iblue)[l] - 1, 0), fe = min(dev::ro(a.ired)[l] - 1, a.nf - 1);
+        if (fs > f) {
+          f_next = min(f_next, fs);
+        } else if (fe >= f) {
+          f_next = min(f_next, fe + 1);
+          if (live(l, f)) cover |= 1ull << l;
+        }
+      }
+    }
     // the continuum (eta = chi B) in working units (convert_to_work_units)
     const float e0 = cc * bn;
     float x = cc + e0;
     float e = e0 / bt;
     float s1 = e, s2 = 0.0f, c1 = x, c2 = 0.0f;
     int32_t sid = 0, cid = 0;
-    for (int l = 0; l < a.nline; ++l) {
-      const int fs = max(a.iblue[l] - 1, 0), fe = min(a.ired[l] - 1, a.nf - 1);
-      if (f < fs || f > fe) continue;
-      const int li = a.lev_i[l] - 1, lj = a.lev_j[l] - 1;
-      if (li < 0 || li >= a.nk || lj < 0 || lj >= a.nk) continue;
-      if (l != cur) {
-        const float lower_off = a.lower_off[l];
-        const float ni = (float)(a.nstar[(int64_t)li * nc + c] * a.pop_ratio[c]);
-        float bi = 1.0f, bj = 1.0f;
-        if (!a.lte) {
-          bi = (float)a.b[(int64_t)li * nc + c];
-          bj = (float)a.b[(int64_t)lj * nc + c];
-        }
-        if (lower_off != 0) {
-          const float eoff = (float)exp((double)(-lower_off * ekt));
-          bi = bi * eoff;
-          bj = bj * eoff;
-        }
-        const float Rij = (float)((double)ni * a.Bij[l]);
-        eta_fac = Rij * bj;
-        chi_fac = Rij * bi;
-        Q = Qlg(a.adamp[(int64_t)l * nc + c]);
-        cur = l;
-      }
-      const float vph = Q.profile(TGX, TH2X, dev::ro(a.q)[(int64_t)l * a.nf + f], vpr, idf);
-      const int index = a.split_offset + l + 1;
-      x = x + vph * chi_fac;
-      e = e + vph * eta_fac;
-      if (SPLIT) {
-        add_largest(vph * eta_fac, index, &s1, &s2, &sid);
-        add_largest(vph * chi_fac, index, &c1, &c2, &cid);
-      }
+    if (masked) {
+      for (uint64_t m = cover; m; m &= m - 1) add_line((int)__builtin_ctzll(m), f, x, e, s1, s2, sid, c1, c2, cid);
+    } else {
+      for (int l = 0; l < a.nline; ++l)
+        if (live(l, f)) add_line(l, f, x, e, s1, s2, sid, c1, c2, cid);
     }
     // work_to_transfer_units
     float eta, chi;
@@ -419,6 +455,10 @@ __device__ __forceinline__ unsigned long long window_cell(
       sc2[o] = c2;
       scidp[o] = cid;
     }
+    o = on;
+    cc = ccn;
+    bn = bnn;
+    bt = btn;
   }
   return floored;
 }
