@@ -401,7 +401,12 @@ static int exact_lds(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
 }
 
 template <typename T, int L>
-static int exact_out(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes) {
+static int exact_out(m3drt_ctx *ctx, const Sc2Args<T> &a, int nt, int nplanes, bool any) {
+  // any other set of outputs (FP32, strips of at most 12 rows: at 15 and more the
+  // run-time outputs need more registers than a lane has): every output checked at
+  // run time
+  if constexpr (sizeof(T) == 4 && L <= 12)
+    if (any) return exact_lds<T, L, OUT_ANY>(ctx, a, nt, nplanes);
   if (!a.J) return exact_lds<T, L, OUT_I>(ctx, a, nt, nplanes);
   if (a.PsiJ) return exact_lds<T, L, OUT_J | OUT_PJ>(ctx, a, nt, nplanes);
   return exact_lds<T, L, OUT_J>(ctx, a, nt, nplanes);
@@ -411,15 +416,15 @@ static const int kExactL32[] = {4, 12, 15};
 static const int kExactL64[] = {12, 24, 30};
 
 template <typename T>
-static int exact_dispatch(m3drt_ctx *ctx, const Sc2Args<T> &a, int L, int nt, int nplanes) {
+static int exact_dispatch(m3drt_ctx *ctx, const Sc2Args<T> &a, int L, int nt, int nplanes, bool any) {
   if (sizeof(T) == 4) {
-    if (L == 4) return exact_out<T, 4>(ctx, a, nt, nplanes);
-    if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes);
-    return exact_out<T, 15>(ctx, a, nt, nplanes);
+    if (L == 4) return exact_out<T, 4>(ctx, a, nt, nplanes, any);
+    if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes, any);
+    return exact_out<T, 15>(ctx, a, nt, nplanes, any);
   }
-  if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes);
-  if (L == 24) return exact_out<T, 24>(ctx, a, nt, nplanes);
-  return exact_out<T, 30>(ctx, a, nt, nplanes);
+  if (L == 12) return exact_out<T, 12>(ctx, a, nt, nplanes, false);
+  if (L == 24) return exact_out<T, 24>(ctx, a, nt, nplanes, false);
+  return exact_out<T, 30>(ctx, a, nt, nplanes, false);
 }
 
 template <typename T>
@@ -444,9 +449,12 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
   const bool plain_out = !b->eta_row && !(d->flags & M3DRT_SPLIT_RATES) &&
                          ((!b->I_out && b->J && !((d->flags & M3DRT_GET_RATES) && b->Psi_out)) ||
                           (b->I_out && !b->J && !psi_out));
-  if (plain_out && !getenv("M3DRT_NO_EXACT")) {
+  // FP32 with any other outputs but the eta step (the ALI's I, Psi, Irem, escape): the
+  // exact strips with run-time output checks
+  const bool any_out = !plain_out && !b->eta_row && sizeof(T) == 4;
+  if ((plain_out || any_out) && !getenv("M3DRT_NO_EXACT")) {
     for (int c : (sizeof(T) == 4 ? kExactL32 : kExactL64))
-      if (c >= L && ny % c == 0 && !Lexact) Lexact = c;
+      if (c >= L && ny % c == 0 && !Lexact && !(any_out && c > 12)) Lexact = c;
   }
   if (Lexact) L = Lexact;
   const int nstrips = (ny + L - 1) / L;
@@ -532,7 +540,7 @@ static int run_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
     fprintf(stderr, "[m3drt] sc2 %s nf=%d plane=%dx%d nz=%d: %d threads, strip %d%s, %d LDS planes of %zu B (lds_max %d)%s\n",
             sizeof(T) == 8 ? "f64" : "f32", d->nf, nx, ny, nz, nt, L, Lexact ? " exact" : "", nplanes, plane,
             ctx->lds_max, b->eta_row ? ", eta" : "");
-  if (Lexact) return exact_dispatch<T>(ctx, a, L, nt, nplanes);
+  if (Lexact) return exact_dispatch<T>(ctx, a, L, nt, nplanes, any_out);
   if (L <= 4) return dispatch_lds<T, 4>(ctx, a, nt, nplanes);
   if (L <= 8) return dispatch_lds<T, 8>(ctx, a, nt, nplanes);
   if (L <= 16) return dispatch_lds<T, 16>(ctx, a, nt, nplanes);
