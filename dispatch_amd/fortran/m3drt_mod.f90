@@ -30,6 +30,7 @@ module m3drt
                                        M3DRT_UNITS_SUPPRESS_MASERS = 4
   integer(c_int), parameter, public :: M3DRT_BG_SCAT_ALI = 1, M3DRT_BG_J_TO_B = 2
   integer(c_int), parameter, public :: M3DRT_ETA_GHOST = 1
+  integer(c_int), parameter, public :: M3DRT_LCH_TOP = 0, M3DRT_LCH_BOTTOM = 1, M3DRT_LCH_UPSTREAM = 2
 
   type, bind(C), public :: m3drt_sweep_desc
     integer(c_int32_t) :: nf, nx, ny, nz
@@ -170,6 +171,14 @@ module m3drt
     type(c_ptr) :: dlnt_dz = c_null_ptr
     type(c_ptr) :: hknu = c_null_ptr
   end type m3drt_hermite_fields
+
+  type, bind(C), public :: m3drt_lc_ghost
+    integer(c_int32_t) :: kind = 0
+    type(c_ptr) :: pe = c_null_ptr
+    type(c_ptr) :: dpe_dz = c_null_ptr
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: dlnt_dz = c_null_ptr
+  end type m3drt_lc_ghost
 
   type, bind(C), public :: m3drt_bb_rates_args
     integer(c_int32_t) :: nline
@@ -407,6 +416,7 @@ module m3drt
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
+  public :: m3drt_lc_sweep_hermite
   public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi, m3drt_line_opacity
   public :: m3drt_window_opacity, m3drt_rotate_array
   public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
@@ -476,6 +486,15 @@ module m3drt
       type(m3drt_sweep_bufs), intent(in) :: bufs
       type(m3drt_hermite_fields), intent(in) :: fields
     end function m3drt_sc_sweep_hermite
+    integer(c_int) function m3drt_lc_sweep_hermite(ctx, desc, bufs, fields, ghost) &
+        bind(C, name='m3drt_lc_sweep_hermite')
+      import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_sweep_bufs, m3drt_hermite_fields, m3drt_lc_ghost
+      type(c_ptr), value :: ctx
+      type(m3drt_sweep_desc), intent(in) :: desc
+      type(m3drt_sweep_bufs), intent(in) :: bufs
+      type(m3drt_hermite_fields), intent(in) :: fields
+      type(m3drt_lc_ghost), intent(in) :: ghost
+    end function m3drt_lc_sweep_hermite
     integer(c_int) function m3drt_sc_adjoint(ctx, desc, bufs) bind(C, name='m3drt_sc_adjoint')
       import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_adjoint_bufs
       type(c_ptr), value :: ctx

@@ -62,6 +62,13 @@ class HermiteFields(C.Structure):
                                           "dlnt_dz", "hknu")]
 
 
+M3DRT_LCH_GHOST = {"top": 0, "bottom": 1, "upstream": 2}
+
+
+class LcGhost(C.Structure):
+    _fields_ = [("kind", C.c_int32)] + [(n, C.c_void_p) for n in ("pe", "dpe_dz", "temp", "dlnt_dz")]
+
+
 _lib = None
 
 
@@ -222,6 +229,58 @@ class Context:
             _check(L.m3drt_sc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h)), "m3drt_sc_sweep_hermite")
             return
         _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")
+
+    def lc_sweep_hermite(self, chi, S, ghost_chi, ghost_S, ghost_I, *, dir, ds, vec, fields, ghost="top",
+                         ghost_fields=None, layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None,
+                         exit_I=None, get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30):
+        """long_char_hermite along every column (m3drt_lc_sweep_hermite) with J += wj*I
+        and PsiJ += wj*Psi fused.  fields: dict of m3drt_hermite_fields (beta a cube
+        like chi; pe4, temp, dpe_dz, dlnt_dz per cell; hknu); ghost "top", "bottom" or
+        "upstream", the last two with ghost_fields = dict(pe, dpe_dz, temp, dlnt_dz) of
+        (ny, nx) planes."""
+        if ghost not in M3DRT_LCH_GHOST:
+            raise M3drtError(f"unknown ghost kind {ghost!r}")
+        if get_rates is None:
+            get_rates = PsiJ is not None or Psi_out is not None
+        d, (nf, nz, ny, nx) = _sweep_desc(chi, layout, [chi, S, J, PsiJ, I_out, Psi_out],
+                                          [ghost_chi, ghost_S, ghost_I, exit_I], None, None, None,
+                                          dir, ds, vec, wj, maser_dtau_max, maser_Inu_max)
+        d.solver = SOLVERS["quadratic"]
+        d.flags = M3DRT_GET_RATES if get_rates else 0
+        tensors = dict(fields)
+        if ghost != "top":
+            if ghost_fields is None:
+                raise M3drtError("the bottom and upstream ghosts need ghost_fields")
+            tensors.update({"ghost_" + k: v for k, v in ghost_fields.items()})
+        for k, t in tensors.items():
+            if t is not None and (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()):
+                raise M3drtError(f"{k} must be a contiguous float32 device tensor")
+        if tuple(fields["beta"].shape) != tuple(chi.shape):
+            raise M3drtError("beta is a cube shaped like chi")
+        for k in ("pe4", "temp", "dpe_dz", "dlnt_dz"):
+            if fields[k].numel() != nx * ny * nz:
+                raise M3drtError(f"{k} holds nx*ny*nz cells")
+        if fields["hknu"].numel() != nf:
+            raise M3drtError("hknu holds nf values")
+        if ghost != "top":
+            for k in ("pe", "dpe_dz", "temp", "dlnt_dz"):
+                if ghost_fields[k].numel() != nx * ny:
+                    raise M3drtError(f"ghost {k} holds nx*ny cells")
+        p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        b = SweepBufs(p(chi), p(S), p(ghost_chi), p(ghost_S), p(ghost_I), None, None, None, None, None, p(J), p(PsiJ),
+                      p(I_out), p(Psi_out), None, None, p(exit_I))
+        h = HermiteFields(*[p(fields.get(k)) for k, _ in HermiteFields._fields_])
+        gf = ghost_fields or {}
+        g = LcGhost(M3DRT_LCH_GHOST[ghost], *[p(gf.get(k)) for k in ("pe", "dpe_dz", "temp", "dlnt_dz")])
+        self._bind_stream()
+        L = lib()
+        if not getattr(L, "_lch_ok", False):
+            L.m3drt_lc_sweep_hermite.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(SweepBufs),
+                                                 C.POINTER(HermiteFields), C.POINTER(LcGhost)]
+            L.m3drt_lc_sweep_hermite.restype = C.c_int
+            L._lch_ok = True
+        _check(L.m3drt_lc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h), C.byref(g)),
+               "m3drt_lc_sweep_hermite")
 
     def sc_adjoint(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, I, *, dir, ds, vec,
                    chibar, Sbar, layout="native", wj=0.0, Jbar=None, Ibar=None, exit_bar=None,

@@ -267,6 +267,31 @@ typedef struct {
 int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs,
                            const m3drt_hermite_fields *fields);
 
+/* ------------------------------------------------------------------ cubic-Bezier column sweep */
+
+/* The boundary below the first plane of long_char_hermite (SRC/rt_solvers_mod.f90:1100-1130). */
+#define M3DRT_LCH_TOP 0      /* the zeroed top ghost (chi = 0): no thermodynamic state there; the
+                                first interval takes secant slopes at its upwind end (herm_first) */
+#define M3DRT_LCH_BOTTOM 1   /* the LTE ghost below the domain with the model's Pe and T there
+                                (bottom_ghost_at, ghost_thermo) */
+#define M3DRT_LCH_UPSTREAM 2 /* the last plane of an upstream neighbour (comp_up) */
+
+/* Thermodynamics of the ghost plane, (ny, nx) x fastest; unused for M3DRT_LCH_TOP. */
+typedef struct {
+  int32_t kind;                             /* M3DRT_LCH_TOP, _BOTTOM or _UPSTREAM */
+  const float *pe, *dpe_dz, *temp, *dlnt_dz;
+} m3drt_lc_ghost;
+
+/* long_char_hermite (SRC/rt_solvers_mod.f90:1052-1504) + integrate_intensity
+ * (SRC/rt_patch_mod.f90:3399-3431): the cubic-Bezier long-characteristics solver along
+ * every column (the output pass; inclined rays after m3drt_rotate_array), chi's
+ * slopes by the product rule through Pe, the source's through B_nu and S/B_nu, the
+ * control points clamped to [min, max] of the end values.  FP32, as the reference.
+ * Of fields, beta, pe4, temp, dpe_dz, dlnt_dz and hknu are read (the x, y gradients
+ * may be NULL).  Needs nz >= 2; desc->solver is ignored; no split rates or eta flags. */
+int m3drt_lc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs,
+                           const m3drt_hermite_fields *fields, const m3drt_lc_ghost *ghost);
+
 /* ------------------------------------------------------------------ line rates */
 
 #define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T */

@@ -731,6 +731,56 @@ def short_char_hermite(chi, S, I, beta, pe4, temp, dpe, dlnt, hknu, dir, ds, vec
     return I, psi, irem, esc
 
 
+LCH_GHOST = {"top": 0, "bottom": 1, "upstream": 2}
+
+
+def bezier3_weights(t):
+    """bezier3_weights (SRC/rt_solvers_mod.f90:1301-1323): (E, w0, w1, w2, w3)."""
+    L = lib()
+    if not getattr(L, "_b3_ok", False):
+        L.orc_bezier3_weights_n.argtypes = [C.c_int] + [C.c_void_p] * 6
+        L.orc_bezier3_weights_n.restype = None
+        L._b3_ok = True
+    t = np.ascontiguousarray(t, np.float32)
+    out = [np.empty_like(t) for _ in range(5)]
+    L.orc_bezier3_weights_n(t.size, _p(t), *[_p(o) for o in out])
+    return tuple(out)
+
+
+def long_char_hermite(chi, S, I, beta, pe4, temp, dpe_dz, dlnt_dz, hknu, dir, ds, vec, ghost="top",
+                      gpe=None, gdpe=None, gtemp=None, gdlnt=None, get_rates=True, maser_dtau_max=2.0,
+                      maser_Inu_max=1e30):
+    """long_char_hermite (SRC/rt_solvers_mod.f90:1052-1504) along every column: work
+    arrays (nz+2, ny, nx, nf) with ghosts; beta (nz, ny, nx, nf); pe4, temp, dpe_dz,
+    dlnt_dz (nz, ny, nx), gradients per cm along +z; hknu (nf); ghost "top" (zeroed
+    top ghost), "bottom" (LTE ghost, the model's Pe/T below) or "upstream" (a
+    neighbour's last plane) with its thermodynamics gpe, gdpe, gtemp, gdlnt (ny, nx).
+    Returns (I, Psi)."""
+    L = lib()
+    if not getattr(L, "_lch_ok", False):
+        L.orc_long_char_hermite_f32.argtypes = [C.POINTER(ScParams)] + [C.c_void_p] * 9 + [C.c_int] + \
+            [C.c_void_p] * 5
+        L.orc_long_char_hermite_f32.restype = None
+        L._lch_ok = True
+    keep = _Keep()
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    if nz < 2:
+        raise ValueError("long_char_hermite needs nz >= 2")
+    kind = LCH_GHOST[ghost]
+    if kind and any(a is None for a in (gpe, gdpe, gtemp, gdlnt)):
+        raise ValueError("the bottom and upstream ghosts need gpe, gdpe, gtemp and gdlnt")
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, False, get_rates)
+    f32 = lambda a: keep.p(a, np.float32)
+    L.orc_long_char_hermite_f32(C.byref(p), _p(chi), _p(S), _p(I), f32(beta), f32(pe4), f32(temp), f32(dpe_dz),
+                                f32(dlnt_dz), f32(hknu), kind, f32(gpe), f32(gdpe), f32(gtemp), f32(gdlnt),
+                                _p(psi))
+    return I, psi
+
+
 def short_char2_adjoint(chi, S, I, chi500, chi500_up, g500, Ibar, dir, ds, vec, maser_dtau_max=2.0,
                         maser_Inu_max=1e30, chibar=None, Sbar=None):
     """Reverse of short_char2's S step in FP64 (oracle/m3d_adjoint.c): work arrays

@@ -47,7 +47,7 @@ MIRRORS = {  # ctypes mirror in dispatch_amd/m3drt.py -> header typedef
     "UnitsArgs": "m3drt_units_args", "EtaArgs": "m3drt_eta_args", "BackgroundArgs": "m3drt_background_args",
     "HermiteFields": "m3drt_hermite_fields", "AdjointBufs": "m3drt_adjoint_bufs",
     "B1dBatch": "m3drt_b1d_batch", "B1dTile": "m3drt_b1d_tile", "WindowArgs": "m3drt_window_args",
-    "RotateArgs": "m3drt_rotate_args",
+    "RotateArgs": "m3drt_rotate_args", "LcGhost": "m3drt_lc_ghost",
 }
 
 
