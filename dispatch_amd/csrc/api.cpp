@@ -156,8 +156,10 @@ int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
     return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: empty grid");
   if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must be +1 or -1");
   if (d->prec != M3DRT_F32 && d->prec != M3DRT_F64) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: bad prec");
-  if (d->solver != M3DRT_LINEAR2 && d->solver != M3DRT_QUADRATIC && d->solver != M3DRT_LINEAR)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: solver must be M3DRT_LINEAR2, M3DRT_QUADRATIC or M3DRT_LINEAR");
+  if (d->solver != M3DRT_LINEAR2 && d->solver != M3DRT_QUADRATIC && d->solver != M3DRT_LINEAR &&
+      d->solver != M3DRT_LONG_LINEAR)
+    return set_error(M3DRT_EINVAL,
+                     "m3drt_sc_sweep: solver must be M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR or M3DRT_LONG_LINEAR");
   if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0)) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: ds must be > 0");
   if (d->vec[2] == 0.0) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: horizontal ray (vec(3) = 0)");
   if ((d->vec[2] > 0) != (d->dir > 0))
@@ -171,6 +173,7 @@ int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_
       return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
     M3DRT_HIP(hipSetDevice(ctx->device));
     if (d->solver == M3DRT_QUADRATIC) return m3drt::sweep_lcq(ctx, d, b);
+    if (d->solver == M3DRT_LONG_LINEAR) return m3drt::sweep_lcl(ctx, d, b);
     return m3drt::sweep_sc1(ctx, d, b);
   }
   if (!b->chi500 || !b->g500) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing chi500 / g500");

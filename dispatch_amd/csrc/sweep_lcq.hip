@@ -4,7 +4,8 @@
 // characteristic (the reference applies it along columns, after rotate_array for
 // inclined rays).  One thread per (frequency, column) walks the column holding the
 // three-plane window of chi and S in registers; lanes run along the unit-stride
-// axis of the layout.  FP32, as the reference.
+// axis of the layout.  FP32, as the reference.  The same file holds the column form
+// of long_char_linear (SRC/rt_solvers_mod.f90:666-757).
 #include "m3drt_internal.hpp"
 #include <cmath>
 #include <algorithm>
@@ -132,7 +133,96 @@ __global__ void __launch_bounds__(256) lcq_sweep(const LcqArgs a) {
   if (a.exitI) a.exitI[go] = Inu;
 }
 
+// long_char_linear (SRC/rt_solvers_mod.f90:666-757): the trapezoid step with the linear
+// weights along each column, one lane per (frequency, column) carrying the upwind
+// chi, S and I in registers; with split rates the remainder and the escape factor
+struct LclArgs {
+  const float *chi, *S, *gchi, *gS, *gI;
+  float *J, *PsiJ, *Iout, *Psiout, *Irem, *esc, *exitI;
+  int64_t cs_f, cs_x, cs_y, cs_z, gs_f, gs_x, gs_y;
+  int nf, nx, ny, nz, dir, f_fast, split;
+  float dsup1, mdtau, imax, wj;
+};
+
+__global__ void __launch_bounds__(256) lcl_sweep(const LclArgs a) {
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= nxy * a.nf) return;
+  int f, col;
+  if (a.f_fast) {
+    f = (int)(id % a.nf);
+    col = (int)(id / a.nf);
+  } else {
+    col = (int)(id % nxy);
+    f = (int)(id / nxy);
+  }
+  const int x = col % a.nx, y = col / a.nx;
+  const int64_t o = f * a.cs_f + x * a.cs_x + y * a.cs_y;
+  const int64_t go = f * a.gs_f + x * a.gs_x + y * a.gs_y;
+  float xxup = a.gchi[go], ssup = a.gS[go], Iup = a.gI[go];
+  for (int s = 0; s < a.nz; ++s) {
+    const int64_t oz = o + (int64_t)(a.dir > 0 ? s : a.nz - 1 - s) * a.cs_z;
+    const float xx = a.chi[oz], ss = a.S[oz];
+    const float dt = fmaxf(a.dsup1 * (xxup + xx), a.mdtau);
+    float E, W, Ps, Esc;
+    dev::lin_weights(dt, E, W, Ps, Esc);
+    float Inu = Iup * E + W * ssup + Ps * ss;
+    const bool off = dt <= 0.0f || xx <= 0.0f || fabsf(Inu) > a.imax;
+    const float rem = Iup * E + W * ssup;
+    Inu = fmaxf(-a.imax, fminf(a.imax, Inu));
+    if (a.split) {
+      Ps = off ? 0.0f : Ps;
+      a.Irem[oz] = off ? Inu : rem;
+      a.esc[oz] = off ? 1.0f : Esc;
+    }
+    const float plam = (dt > 0.0f && xx > 0.0f) ? Ps / xx : 0.0f;
+    if (a.J) a.J[oz] = a.J[oz] + a.wj * Inu;
+    if (a.PsiJ) a.PsiJ[oz] = a.PsiJ[oz] + a.wj * plam;
+    if (a.Iout) a.Iout[oz] = Inu;
+    if (a.Psiout) a.Psiout[oz] = plam;
+    xxup = xx;
+    ssup = ss;
+    Iup = Inu;
+  }
+  if (a.exitI) a.exitI[go] = Iup;
+}
+
 }  // namespace
+
+int sweep_lcl(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  if (d->prec != M3DRT_F32)
+    return set_error(M3DRT_ELIMIT, "m3drt_sc_sweep: the long-characteristics linear solver is FP32 (as the reference)");
+  if (b->eta_row) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: the long-characteristics linear solver takes no eta flags");
+  LclArgs a;
+  a.chi = static_cast<const float *>(b->chi);
+  a.S = static_cast<const float *>(b->S);
+  a.gchi = static_cast<const float *>(b->ghost_chi);
+  a.gS = static_cast<const float *>(b->ghost_S);
+  a.gI = static_cast<const float *>(b->ghost_I);
+  const bool rates = (d->flags & M3DRT_GET_RATES) != 0;
+  a.split = (d->flags & M3DRT_SPLIT_RATES) ? 1 : 0;
+  a.J = static_cast<float *>(b->J);
+  a.PsiJ = rates ? static_cast<float *>(b->PsiJ) : nullptr;
+  a.Iout = static_cast<float *>(b->I_out);
+  a.Psiout = rates ? static_cast<float *>(b->Psi_out) : nullptr;
+  a.Irem = static_cast<float *>(b->Irem_out);
+  a.esc = static_cast<float *>(b->escape_out);
+  a.exitI = static_cast<float *>(b->exit_I);
+  a.cs_f = d->cs_f; a.cs_x = d->cs_x; a.cs_y = d->cs_y; a.cs_z = d->cs_z;
+  a.gs_f = d->gs_f; a.gs_x = d->gs_x; a.gs_y = d->gs_y;
+  a.nf = d->nf; a.nx = d->nx; a.ny = d->ny; a.nz = d->nz; a.dir = d->dir > 0 ? 1 : -1;
+  a.f_fast = d->cs_f <= d->cs_x ? 1 : 0;
+  // SRC/rt_solvers_mod.f90:681-682: half the path per plane, 0.5*ds in real(4) over |mu|
+  const float ds = (float)d->ds[2];
+  a.dsup1 = (float)((double)(0.5f * ds) / fabs(d->vec[2]));
+  a.mdtau = -(float)d->maser_dtau_max;
+  a.imax = (float)d->maser_Inu_max;
+  a.wj = (float)d->wj;
+  const int64_t n = (int64_t)d->nx * d->ny * d->nf;
+  hipLaunchKernelGGL(lcl_sweep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a);
+  M3DRT_HIP(hipGetLastError());
+  return M3DRT_OK;
+}
 
 int sweep_lcq(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
   if (d->prec != M3DRT_F32)

@@ -20,7 +20,8 @@ module m3drt
   private
 
   integer(c_int), parameter, public :: M3DRT_OK = 0, M3DRT_F32 = 0, M3DRT_F64 = 1
-  integer(c_int), parameter, public :: M3DRT_LINEAR2 = 0, M3DRT_QUADRATIC = 1, M3DRT_LINEAR = 2
+  integer(c_int), parameter, public :: M3DRT_LINEAR2 = 0, M3DRT_QUADRATIC = 1, M3DRT_LINEAR = 2, &
+                                        M3DRT_LONG_LINEAR = 3
   integer(c_int), parameter, public :: M3DRT_SPLIT_RATES = 1, M3DRT_GET_RATES = 2
   integer(c_int), parameter, public :: M3DRT_OPAC_NLTE = 1, M3DRT_OPAC_SCAT = 2
   integer(c_int), parameter, public :: M3DRT_RATES_PRECOND = 1, M3DRT_RATES_OWN_NORM = 2, &

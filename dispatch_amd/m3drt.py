@@ -27,7 +27,8 @@ LIB_PATH = os.path.join(_HERE, "libm3drt.so")
 M3DRT_OK = 0
 M3DRT_F32, M3DRT_F64 = 0, 1
 M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR = 0, 1, 2
-SOLVERS = dict(linear2=M3DRT_LINEAR2, quadratic=M3DRT_QUADRATIC, linear=M3DRT_LINEAR)
+M3DRT_LONG_LINEAR = 3
+SOLVERS = dict(linear2=M3DRT_LINEAR2, quadratic=M3DRT_QUADRATIC, linear=M3DRT_LINEAR, long_linear=M3DRT_LONG_LINEAR)
 M3DRT_SPLIT_RATES, M3DRT_GET_RATES = 1, 2
 
 
@@ -189,7 +190,8 @@ class Context:
                  get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2", hermite=None):
         """One directional formal solution (short_char2 in 3D, long_char2 when
         nx = ny = 1; solver="quadratic": long_char_quadratic along every column;
-        solver="linear": short_char, the Multi3D-comparison preset; hermite=dict of
+        solver="linear": short_char, the Multi3D-comparison preset; solver="long_linear":
+        long_char_linear along every column; hermite=dict of
         the composition fields of m3drt_hermite_fields: short_char_hermite)
         with J += wj*I and PsiJ += wj*Psi fused."""
         if solver not in SOLVERS:

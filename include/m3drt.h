@@ -41,6 +41,9 @@ extern "C" {
                              rotation for inclined rays; SRC/rt_solvers_mod.f90:891-1020), FP32 */
 #define M3DRT_LINEAR 2    /* short_char, the 'linear' Multi3D-comparison preset: chi, S, I all
                              from the 4x4 stencil (SRC/rt_solvers_mod.f90:19-189), FP32 */
+#define M3DRT_LONG_LINEAR 3 /* long_char_linear along every column: the trapezoid step with the
+                               linear weights (SRC/rt_solvers_mod.f90:666-757; the 'linear'
+                               output solver and 1D iteration solver), FP32, split rates */
 
 /* sweep flags */
 #define M3DRT_SPLIT_RATES 1  /* fill Irem and escape ('largest' rate formulation) */

@@ -11,6 +11,8 @@
  *   herm_first       :1465-1520 (the zeroed top ghost: secant slope at the upwind end)
  *   herm_plane       :1171-1250 and the plane order of :1100-1160
  *   ghost_thermo     :1257-1266
+ * and long_char_linear (:666-757), the trapezoid step with linear_weights along each
+ * column, with the split parts of the 'largest' rate formulation.
  *
  * Pinning (tests/test_oracle_lch.py): the four weights against numerical
  * quadrature of the Bernstein integrals over -3 <= t <= 87, and their sum against
@@ -21,6 +23,9 @@
  * cubic chi).  The Lambda diagonal is the reference's heuristic (a bounded
  * surrogate of dI/dS, not the derivative): it is pinned only by 0 <= chi Psi <= 1
  * and by its isothermal, constant-chi value from the quadrature weights.
+ * long_char_linear is pinned by test_oracle_lch.py too: along a vertical ray it
+ * equals short_char (whose 4x4 stencil then reads the column itself), a source linear
+ * in tau with the ghost on the particular solution is exact, and Irem + Psi S = I.
  */
 #include <math.h>
 #include <stdint.h>
@@ -281,4 +286,45 @@ void orc_long_char_hermite_f32(const orc_sc_params *p, const float *chi, const f
 #undef G_
 #undef B_
 #undef P_
+}
+
+void orc_long_char_linear_f32(const orc_sc_params *p, const float *chi, const float *S, float *I, float *Psi,
+                              float *Irem, float *escape)
+{
+  const int nf = p->nf, nx = p->nx, ny = p->ny, nz = p->nz, dir = p->dir;
+  const size_t sx = (size_t)nf, sy = sx * nx, sz = sy * ny;
+  const float ds = (float)p->ds[2];
+  const float dsup1 = (float)((double)(0.5f * ds) / fabs(p->vec[2]));
+  const float mdt = -(float)p->maser_dtau_max, imax = (float)p->maser_Inu_max;
+  const int zs = dir > 0 ? 1 : nz, ze = dir > 0 ? nz : 1;
+#define W_(a, f, x, y, z) a[(f) + sx * (x) + sy * (y) + sz * (size_t)(z)]
+#define O_(a, f, x, y, z) a[(f) + sx * (x) + sy * (y) + sz * (size_t)((z) - 1)]
+  for (int z = zs; dir > 0 ? z <= ze : z >= ze; z += dir) {
+    const int zup = z - dir;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++)
+        for (int f = 0; f < nf; f++) {
+          const float xxup = W_(chi, f, x, y, zup), ssup = W_(S, f, x, y, zup), Inuup = W_(I, f, x, y, zup);
+          const float xx = W_(chi, f, x, y, z), ss = W_(S, f, x, y, z);
+          float dt = dsup1 * (xxup + xx);
+          dt = MAX_(dt, mdt);
+          float E, W, Ps, Esc;
+          orc_linear_weights(dt, &E, &W, &Ps, &Esc);
+          float Inu = Inuup * E + W * ssup + Ps * ss;
+          if (p->split_rates) {
+            const int off = dt <= 0 || xx <= 0 || fabsf(Inu) > imax;
+            const float Ir = Inuup * E + W * ssup;
+            Inu = MAX_(-imax, MIN_(imax, Inu));
+            Ps = off ? 0.0f : Ps;
+            if (Irem) O_(Irem, f, x, y, z) = off ? Inu : Ir;
+            if (escape) O_(escape, f, x, y, z) = off ? 1.0f : Esc;
+          } else {
+            Inu = MAX_(-imax, MIN_(imax, Inu));
+          }
+          W_(I, f, x, y, z) = Inu;
+          if (p->get_rates && Psi) O_(Psi, f, x, y, z) = (dt > 0 && xx > 0) ? Ps / xx : 0.0f;
+        }
+  }
+#undef W_
+#undef O_
 }

@@ -27,6 +27,11 @@ void orc_long_char_hermite_f32(const orc_sc_params *p, const float *chi, const f
                                const float *dlnt_dz, const float *hknu, int ghost_kind, const float *gpe,
                                const float *gdpe, const float *gtemp, const float *gdlnt, float *Psi);
 
+/* long_char_linear (SRC/rt_solvers_mod.f90:666-757): work arrays as above; Psi, Irem,
+ * escape (nz, ny, nx, nf) or NULL (Irem, escape filled with p->split_rates). */
+void orc_long_char_linear_f32(const orc_sc_params *p, const float *chi, const float *S, float *I, float *Psi,
+                              float *Irem, float *escape);
+
 #ifdef __cplusplus
 }
 #endif

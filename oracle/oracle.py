@@ -781,6 +781,27 @@ def long_char_hermite(chi, S, I, beta, pe4, temp, dpe_dz, dlnt_dz, hknu, dir, ds
     return I, psi
 
 
+def long_char_linear(chi, S, I, dir, ds, vec, split_rates=False, get_rates=True, maser_dtau_max=2.0,
+                     maser_Inu_max=1e30):
+    """long_char_linear (SRC/rt_solvers_mod.f90:666-757) along every column: work
+    arrays (nz+2, ny, nx, nf) with ghosts.  Returns (I, Psi, Irem, escape)."""
+    L = lib()
+    if not getattr(L, "_lcl_ok", False):
+        L.orc_long_char_linear_f32.argtypes = [C.POINTER(ScParams)] + [C.c_void_p] * 6
+        L.orc_long_char_linear_f32.restype = None
+        L._lcl_ok = True
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    irem = np.zeros_like(psi) if split_rates else None
+    esc = np.zeros_like(psi) if split_rates else None
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, split_rates, get_rates)
+    L.orc_long_char_linear_f32(C.byref(p), _p(chi), _p(S), _p(I), _p(psi), _p(irem), _p(esc))
+    return I, psi, irem, esc
+
+
 def short_char2_adjoint(chi, S, I, chi500, chi500_up, g500, Ibar, dir, ds, vec, maser_dtau_max=2.0,
                         maser_Inu_max=1e30, chibar=None, Sbar=None):
     """Reverse of short_char2's S step in FP64 (oracle/m3d_adjoint.c): work arrays

@@ -99,3 +99,46 @@ def test_lc_sweep_hermite_rejects_bad_input(ctx):
     with pytest.raises(M3drtError, match="ghost_fields"):
         ctx.lc_sweep_hermite(c, cub(interior(case["S"])), pla(case["chi"]), pla(case["S"]), pla(case["I"]), dir=1,
                              ds=case["ds"], vec=case["vec"], fields=fields, ghost="bottom")
+
+
+@pytest.mark.parametrize("layout", ["native", "reference"])
+@pytest.mark.parametrize("vec,nz,split", [((0.0, 0.0, 0.78867513), 12, True), ((0.0, 0.0, -0.5), 9, False),
+                                          ((0.0, 0.0, -1.0), 1, True)])
+def test_long_char_linear_columns(ctx, layout, vec, nz, split):
+    """long_char_linear along every column (M3DRT_LONG_LINEAR), masers included: I,
+    Psi, the split parts, J, PsiJ and the exit plane against the oracle."""
+    from oracle import oracle as O
+    case = make_case(nf=24, nx=5, ny=4, nz=nz, seed=90 + nz, vec=vec, maser=True)
+    print(f"long_char_linear {vec} nz={nz} split={split} {layout}", flush=True)
+    I, psi, irem, esc = O.long_char_linear(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"],
+                                           split_rates=split)
+    gz = case["gz"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(a))
+        pla = lambda a: dev(plane_to_native(a[gz]))
+        back = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pback = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        cub = lambda a: dev(a)
+        pla = lambda a: dev(a[gz])
+        back = pback = lambda t: t.double().cpu().numpy()
+    c = cub(interior(case["chi"]))
+    J, PsiJ = torch.full_like(c, 0.125), torch.full_like(c, 0.5)
+    Iout, Psi = torch.zeros_like(c), torch.zeros_like(c)
+    Irem = torch.zeros_like(c) if split else None
+    Esc = torch.zeros_like(c) if split else None
+    exitI = torch.zeros_like(pla(case["chi"]))
+    ctx.sc_sweep(c, cub(interior(case["S"])), pla(case["chi"]), pla(case["S"]), pla(case["I"]), None, None, None,
+                 dir=case["dir"], ds=case["ds"], vec=case["vec"], layout=layout, wj=0.25, J=J, PsiJ=PsiJ,
+                 I_out=Iout, Psi_out=Psi, Irem_out=Irem, escape_out=Esc, exit_I=exitI, split_rates=split,
+                 solver="long_linear")
+    torch.cuda.synchronize()
+    check32(back(Iout), interior(I), "I")
+    check32(back(Psi), psi, "Psi")
+    check32(back(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J")
+    check32(back(PsiJ), 0.5 + 0.25 * psi.astype(np.float64), "PsiJ")
+    if split:
+        check32(back(Irem), irem, "Irem")
+        check32(back(Esc), esc, "escape")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(pback(exitI), I[last], "exit plane")

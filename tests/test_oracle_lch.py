@@ -173,3 +173,38 @@ def test_lambda_diagonal_is_bounded(kind, vec):
     assert np.all(np.isfinite(I)) and np.all(lam >= 0)
     assert np.all(lam <= 1.0)
     assert lam.max() > 0.05
+
+
+# ---------------------------------------------------------------- long_char_linear
+
+@pytest.mark.parametrize("vec", [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0)])
+def test_long_char_linear_is_short_char_on_a_vertical_ray(vec):
+    """A vertical ray's 4x4 stencil reads the column itself, so short_char's step is
+    long_char_linear's: I, Psi and the split parts agree."""
+    case = make_case(nf=5, nx=6, ny=5, nz=7, seed=11, vec=vec, maser=True)
+    a = O.short_char(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"], split_rates=True)
+    b = O.long_char_linear(case["chi"], case["S"], case["I"], case["dir"], case["ds"], case["vec"], split_rates=True)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(y, x, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("vec", [(0.3, -0.4, 0.866), (-0.2, 0.1, -0.5)])
+def test_long_char_linear_source_is_exact(vec):
+    nf, nx, ny, nz = 4, 3, 2, 9
+    c = column_case(nf, nx, ny, nz, vec, seed=5)
+    mu = c["vec"][2]
+    chi0 = np.float32(8e-7)
+    chi = np.full(c["shape"], chi0, np.float32)
+    zc = (np.arange(nz + 2, dtype=np.float64) - 1) * DS[2]
+    S = np.broadcast_to((0.3 + 1e-7 * zc)[:, None, None, None], c["shape"])
+    dSdtau = 1e-7 * mu / float(chi0)
+    Iex = S - dSdtau
+    I0 = np.zeros(c["shape"], np.float32)
+    I0[c["gz"]] = Iex[c["gz"]]
+    I, psi, irem, esc = O.long_char_linear(chi, S.astype(np.float32), I0, c["dir"], DS, c["vec"], split_rates=True)
+    assert abs(dSdtau) / 0.3 > 1e-2
+    np.testing.assert_allclose(I[1:-1], Iex[1:-1], rtol=2e-6)
+    # the split parts: I = Irem + (chi Psi) S, escape = 1 - chi Psi
+    lam = psi.astype(np.float64) * chi0
+    np.testing.assert_allclose(irem + lam * S[1:-1], I[1:-1], rtol=2e-6)
+    np.testing.assert_allclose(esc, 1 - lam, rtol=2e-6)
