@@ -212,12 +212,12 @@ int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3dr
   return m3drt::sweep_sch(ctx, d, b, h);
 }
 
-int m3drt_lc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
-                           const m3drt_hermite_fields *h, const m3drt_lc_ghost *g) {
-  const std::string w = "m3drt_lc_sweep_hermite: ";
+static int lc_hermite_entry(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
+                            const m3drt_hermite_fields *h, const m3drt_lc_ghost *g, bool hybrid) {
+  const std::string w = hybrid ? "m3drt_lc_sweep_hermite2: " : "m3drt_lc_sweep_hermite: ";
   if (!ctx || !d || !b || !h || !g) return set_error(M3DRT_EINVAL, w + "NULL argument");
   if (d->nf < 1 || d->nx < 1 || d->ny < 1) return set_error(M3DRT_EINVAL, w + "empty grid");
-  // SRC/rt_solvers_mod.f90:1076-1077
+  // SRC/rt_solvers_mod.f90:1076-1077, 1552-1553
   if (d->nz < 2) return set_error(M3DRT_EINVAL, w + "needs nz >= 2");
   if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, w + "dir must be +1 or -1");
   if (!(d->ds[2] > 0) || d->vec[2] == 0.0 || (d->vec[2] > 0) != (d->dir > 0))
@@ -227,18 +227,28 @@ int m3drt_lc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3dr
     return set_error(M3DRT_EINVAL, w + "takes neither split rates nor eta flags");
   if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I)
     return set_error(M3DRT_EINVAL, w + "missing input array");
-  if (!h->beta || !h->pe4 || !h->temp || !h->dpe_dz || !h->dlnt_dz || !h->hknu)
+  if (!h->pe4 || !h->dpe_dz || (!hybrid && (!h->beta || !h->temp || !h->dlnt_dz || !h->hknu)))
     return set_error(M3DRT_EINVAL, w + "missing composition field");
   if (g->kind != M3DRT_LCH_TOP && g->kind != M3DRT_LCH_BOTTOM && g->kind != M3DRT_LCH_UPSTREAM)
     return set_error(M3DRT_EINVAL, w + "unknown ghost kind");
-  if (g->kind != M3DRT_LCH_TOP && (!g->pe || !g->dpe_dz || !g->temp || !g->dlnt_dz))
-    return set_error(M3DRT_EINVAL, w + "the bottom and upstream ghosts need pe, dpe_dz, temp and dlnt_dz");
+  if (g->kind != M3DRT_LCH_TOP && (!g->pe || !g->dpe_dz || (!hybrid && (!g->temp || !g->dlnt_dz))))
+    return set_error(M3DRT_EINVAL, w + "the bottom and upstream ghosts need their Pe, dPe/dz (and T, dlnT/dz)");
   if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
     return set_error(M3DRT_EINVAL, w + "strides must be >= 1");
   if ((int64_t)d->nf * d->nx * d->ny > (int64_t)0xffffffffLL * 256 - 255)
     return set_error(M3DRT_ELIMIT, w + "more columns times frequencies than one launch covers");
   M3DRT_HIP(hipSetDevice(ctx->device));
-  return m3drt::sweep_lch(ctx, d, b, h, g);
+  return m3drt::sweep_lch(ctx, d, b, h, g, hybrid);
+}
+
+int m3drt_lc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
+                           const m3drt_hermite_fields *h, const m3drt_lc_ghost *g) {
+  return lc_hermite_entry(ctx, d, b, h, g, false);
+}
+
+int m3drt_lc_sweep_hermite2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
+                            const m3drt_hermite_fields *h, const m3drt_lc_ghost *g) {
+  return lc_hermite_entry(ctx, d, b, h, g, true);
 }
 
 int m3drt_sc_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint_bufs *b) {

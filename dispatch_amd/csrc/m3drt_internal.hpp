@@ -40,7 +40,7 @@ int sweep_lcl(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs 
 int sweep_sc1(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_sch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const m3drt_hermite_fields *h);
 int sweep_lch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const m3drt_hermite_fields *h,
-              const m3drt_lc_ghost *g);
+              const m3drt_lc_ghost *g, bool hermite2);
 int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint_bufs *b);
 int b1d_synth_plain(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile);
 int b1d_plan_create(m3drt_ctx *ctx, const m3drt_b1d_batch *b, const m3drt_b1d_tile *tile, m3drt_b1d_plan **out);
@@ -112,6 +112,25 @@ __device__ __forceinline__ void lin_weights(float t, float &E, float &W, float &
 }
 
 // FP64 form: exact weights, the plain FP64 reference of the sweep.
+// parab_moments (SRC/rt_solvers_mod.f90:862-877): E = e^-t, U0 = 1 - E, U1 = t - U0
+// (t * Psi(t) below |t| = 1, Psi the degree-7 series of linear_weights), q = U2/t
+// (degree-8 minimax below |t| = 2).  E from the hardware exponential, as the linear
+// weights.
+__device__ __forceinline__ void parab_moments(float t, float &E, float &U0, float &U1, float &q) {
+#pragma clang fp contract(off)
+  const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
+                   t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
+                   t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
+  E = __builtin_amdgcn_exp2f(fminf(t, 87.0f) * -1.44269504088896341f);
+  const bool thin = fabsf(t) <= 1.0f;
+  U0 = thin ? t * (1.0f - ps) : 1.0f - E;
+  U1 = thin ? t * ps : t - U0;
+  const float qs = t * t * (3.333333433e-01f + t * (-8.333329856e-02f + t * (1.666668802e-02f +
+                   t * (-2.777901944e-03f + t * (3.968053206e-04f + t * (-4.948913192e-05f +
+                   t * (5.514686109e-06f + t * (-5.894584660e-07f + t * 5.153443894e-08f))))))));
+  q = fabsf(t) <= 2.0f ? qs : t - 2.0f * (U1 / t);
+}
+
 __device__ __forceinline__ void lin_weights(double t, double &E, double &W, double &Psi, double &Esc) {
   E = exp(-t);
   const double u0 = -expm1(-t);

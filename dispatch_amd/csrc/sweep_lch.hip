@@ -1,5 +1,6 @@
-// Column sweep with the cubic-Bezier source integration of long_char_hermite
-// (SRC/rt_solvers_mod.f90:1052-1504), fused with integrate_intensity
+// Column sweeps with the cubic-Bezier source integration of long_char_hermite
+// (SRC/rt_solvers_mod.f90:1052-1504) and the hybrid long_char_hermite2 (:1536-1895),
+// fused with integrate_intensity
 // (SRC/rt_patch_mod.f90:3399-3431).  Each column is its own long characteristic (the
 // output pass applies it along columns, after rotate_array for inclined rays).
 //
@@ -27,6 +28,7 @@ struct LchArgs {
   int64_t cs_f, cs_x, cs_y, cs_z, gs_f, gs_x, gs_y;
   int nf, nx, ny, nz, dir, f_fast, kind, get_rates;
   float rdsup, dsup1, dsup2, mdtau, imax, wj, gfac, third;
+  double mdtau64;
 };
 
 // one sample of the column: the frequency's chi, S, beta; the cell's Pe, 1/Pe,
@@ -240,10 +242,158 @@ __global__ void __launch_bounds__(256) lch_sweep(const LchArgs a) {
   if (a.exitI) a.exitI[go] = Inu;
 }
 
+// long_char_hermite2 (SRC/rt_solvers_mod.f90:1536-1895): long_char_hermite's optical
+// depth (product-rule chi slopes, here clamped to 3x the secant) with
+// long_char_quadratic's source treatment, the MC-limited dS/dtau at the target from
+// the secants of the two intervals beside it (the second interval's optical depth
+// from the same parabola of chi/Pe) and the parab_moments weights, A, B, C in FP64
+// as the reference keeps them.  onesided: the last plane's third sample is real
+// (backward one-sided source slope) or the ghost (secant).
+__device__ __forceinline__ double tau_floor64(double dt, double mdtau) {
+  return fmax(copysign(fmax(fabs(dt), (double)1e-20f), dt), mdtau);
+}
+
+__device__ __forceinline__ void herm2_step(const LchArgs &a, const Samp &u, const Samp &c, const Samp &t3,
+                                           bool center, bool onesided, float Iup, float &Inu, float &plam) {
+  const float hh = 0.5f * a.rdsup;
+  float t1 = (c.chi - u.chi) * a.rdsup;
+  float sgn = copysignf(1.0f, t1);
+  const float rxpu = u.chi * u.rpe, rxpc = c.chi * c.rpe, rxp3 = t3.chi * t3.rpe;
+  float drxpu, drxpc;
+  if (center) {
+    drxpc = (rxp3 - rxpu) * hh;
+    drxpu = (4.0f * rxpc - 3.0f * rxpu - rxp3) * hh;
+  } else {
+    drxpu = (rxpc - rxp3) * hh;
+    drxpc = (rxp3 - 4.0f * rxpu + 3.0f * rxpc) * hh;
+  }
+  float mup = rxpu * u.gp + u.pe * drxpu;
+  float mc = rxpc * c.gp + c.pe * drxpc;
+  mup = sgn * fmaxf(0.0f, fminf(sgn * mup, 3.0f * fabsf(t1)));
+  mc = sgn * fmaxf(0.0f, fminf(sgn * mc, 3.0f * fabsf(t1)));
+  const double mdt = a.mdtau64;
+  const double dt = tau_floor64((double)(a.dsup1 * (u.chi + c.chi) + a.dsup2 * (mup - mc)), mdt);
+  const double rdt = 1.0 / dt;
+  float ex, U0, U1, q;
+  dev::parab_moments((float)dt, ex, U0, U1, q);
+  const float dss = c.S - u.S;
+  float dsdtau;
+  if (center) {
+    float t3s = (t3.chi - c.chi) * a.rdsup;
+    const float sgn3 = copysignf(1.0f, t3s);
+    const float drxp3 = (3.0f * rxp3 - 4.0f * rxpc + rxpu) * hh;
+    float m3 = rxp3 * t3.gp + t3.pe * drxp3;
+    m3 = sgn3 * fmaxf(0.0f, fminf(sgn3 * m3, 3.0f * fabsf(t3s)));
+    const double dt3 = tau_floor64((double)(a.dsup1 * (c.chi + t3.chi) + a.dsup2 * (mc - m3)), mdt);
+    t1 = dss * (float)rdt;
+    t3s = (t3.S - c.S) / (float)dt3;
+    dsdtau = 0.5f * (t1 + t3s);
+    dsdtau = (copysignf(1.0f, t1) + copysignf(1.0f, t3s)) * fminf(fminf(fabsf(t1), fabsf(t3s)), fabsf(0.5f * dsdtau));
+  } else {
+    t1 = dss * (float)rdt;
+    dsdtau = t1;
+    if (onesided && c.chi > 0.0f) {
+      dsdtau = (3.0f * c.S - 4.0f * u.S + t3.S) * hh / c.chi;
+      sgn = copysignf(1.0f, t1);
+      dsdtau = sgn * fmaxf(0.0f, fminf(sgn * dsdtau, 2.0f * fabsf(t1)));
+    }
+  }
+  const double B = (2.0 * (double)U1 - (double)q) * rdt, A = (double)U0 - B, C = (double)q - (double)U1;
+  const float I = Iup * ex + (float)A * u.S + (float)B * c.S + (float)C * dsdtau;
+  Inu = fmaxf(-a.imax, fminf(a.imax, I));
+  plam = (a.get_rates && dt > 0.0 && c.chi > 0.0f) ? (float)B / c.chi : 0.0f;
+}
+
+// herm2_first (SRC/rt_solvers_mod.f90:1860-1893): below the zeroed top ghost
+__device__ __forceinline__ void herm2_first(const LchArgs &a, float xxup, float ssup, const Samp &c, const Samp &d,
+                                            float Iup, float &Inu, float &plam) {
+  const float t1 = (c.chi - xxup) * a.rdsup;
+  const float sgn = copysignf(1.0f, t1);
+  const float rxpc = c.chi * c.rpe;
+  float mc = rxpc * c.gp + c.pe * (d.chi * d.rpe - rxpc) * a.rdsup;
+  mc = sgn * fmaxf(0.0f, fminf(sgn * mc, 3.0f * fabsf(t1)));
+  const double dt = tau_floor64((double)(a.dsup1 * (xxup + c.chi) + a.dsup2 * (t1 - mc)), a.mdtau64);
+  const double rdt = 1.0 / dt;
+  float ex, U0, U1, q;
+  dev::parab_moments((float)dt, ex, U0, U1, q);
+  const float dsdtau = (c.S - ssup) * (float)rdt;
+  const double B = (2.0 * (double)U1 - (double)q) * rdt, A = (double)U0 - B, C = (double)q - (double)U1;
+  const float I = Iup * ex + (float)A * ssup + (float)B * c.S + (float)C * dsdtau;
+  Inu = fmaxf(-a.imax, fminf(a.imax, I));
+  plam = (a.get_rates && dt > 0.0 && c.chi > 0.0f) ? (float)B / c.chi : 0.0f;
+}
+
+__global__ void __launch_bounds__(256) lch2_sweep(const LchArgs a) {
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= nxy * a.nf) return;
+  int f, col;
+  if (a.f_fast) {
+    f = (int)(id % a.nf);
+    col = (int)(id / a.nf);
+  } else {
+    col = (int)(id % nxy);
+    f = (int)(id / nxy);
+  }
+  const int x = col % a.nx, y = col / a.nx;
+  const int64_t o = f * a.cs_f + x * a.cs_x + y * a.cs_y;
+  const int64_t go = f * a.gs_f + x * a.gs_x + y * a.gs_y;
+  auto zof = [&](int s) { return a.dir > 0 ? s : a.nz - 1 - s; };
+  auto load = [&](int s) {
+    const int z = zof(s);
+    const int64_t oz = o + (int64_t)z * a.cs_z;
+    const int64_t cz = col + (int64_t)z * nxy;
+    Samp p;
+    p.chi = a.chi[oz];
+    p.S = a.S[oz];
+    p.beta = p.aT = 0.0f;
+    p.pe = a.pe4[cz];
+    p.rpe = 1.0f / p.pe;
+    p.gp = a.gfac * a.dpe_dz[cz];
+    return p;
+  };
+  auto emit = [&](int s, float Inu, float plam) {
+    const int64_t oz = o + (int64_t)zof(s) * a.cs_z;
+    if (a.J) a.J[oz] = a.J[oz] + a.wj * Inu;
+    if (a.PsiJ) a.PsiJ[oz] = a.PsiJ[oz] + a.wj * plam;
+    if (a.Iout) a.Iout[oz] = Inu;
+    if (a.Psiout) a.Psiout[oz] = plam;
+  };
+  Samp g;
+  g.chi = a.gchi[go];
+  g.S = a.gS[go];
+  g.beta = g.aT = g.pe = g.rpe = g.gp = 0.0f;
+  if (a.kind != M3DRT_LCH_TOP) {
+    g.pe = a.gpe[col];
+    g.rpe = 1.0f / g.pe;
+    g.gp = a.gfac * a.gdpe[col];
+  }
+  Samp pm = g, p0 = load(0), p1 = load(1);
+  float Inu, plam;
+  if (a.kind == M3DRT_LCH_TOP)
+    herm2_first(a, g.chi, g.S, p0, p1, a.gI[go], Inu, plam);
+  else
+    herm2_step(a, g, p0, p1, true, false, a.gI[go], Inu, plam);
+  emit(0, Inu, plam);
+  for (int s = 1; s < a.nz - 1; ++s) {
+    pm = p0;
+    p0 = p1;
+    p1 = load(s + 1);
+    herm2_step(a, pm, p0, p1, true, false, Inu, Inu, plam);
+    emit(s, Inu, plam);
+  }
+  // the last plane: third sample two planes upwind; at nz = 2 the ghost, with the
+  // neighbour's Pe or else the target's
+  if (a.nz == 2 && a.kind != M3DRT_LCH_UPSTREAM) pm.rpe = p1.rpe;
+  herm2_step(a, p0, p1, pm, false, a.nz >= 3, Inu, Inu, plam);
+  emit(a.nz - 1, Inu, plam);
+  if (a.exitI) a.exitI[go] = Inu;
+}
+
 }  // namespace
 
 int sweep_lch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const m3drt_hermite_fields *h,
-              const m3drt_lc_ghost *g) {
+              const m3drt_lc_ghost *g, bool hermite2) {
   LchArgs a;
   a.chi = static_cast<const float *>(b->chi);
   a.S = static_cast<const float *>(b->S);
@@ -281,10 +431,14 @@ int sweep_lch(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs 
   a.gfac = (float)(a.dir * fabs(d->vec[2]));
   a.third = (float)(1.0 / 3.0);
   a.mdtau = -(float)d->maser_dtau_max;
+  a.mdtau64 = -d->maser_dtau_max;
   a.imax = (float)d->maser_Inu_max;
   a.wj = (float)d->wj;
   const int64_t n = (int64_t)d->nx * d->ny * d->nf;
-  hipLaunchKernelGGL(lch_sweep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a);
+  if (hermite2)
+    hipLaunchKernelGGL(lch2_sweep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(lch_sweep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
   return M3DRT_OK;
 }

@@ -24,25 +24,6 @@ struct LcqArgs {
   float rdsup, dsup1, dsup2, mdtau, imax, wj;
 };
 
-// parab_moments (SRC/rt_solvers_mod.f90:862-877): E = e^-t, U0 = 1 - E, U1 = t - U0
-// (t * Psi(t) below |t| = 1, Psi the degree-7 series of linear_weights), q = U2/t
-// (degree-8 minimax below |t| = 2).  E from the hardware exponential, as the
-// linear sweep's weights.
-__device__ __forceinline__ void parab_moments(float t, float &E, float &U0, float &U1, float &q) {
-  const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
-                   t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
-                   t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
-  const float tc = fminf(t, 87.0f);
-  E = __builtin_amdgcn_exp2f(tc * -1.44269504088896341f);
-  const bool thin = fabsf(t) <= 1.0f;
-  U0 = thin ? t * (1.0f - ps) : 1.0f - E;
-  U1 = thin ? t * ps : t - U0;
-  const float qs = t * t * (3.333333433e-01f + t * (-8.333329856e-02f + t * (1.666668802e-02f +
-                   t * (-2.777901944e-03f + t * (3.968053206e-04f + t * (-4.948913192e-05f +
-                   t * (5.514686109e-06f + t * (-5.894584660e-07f + t * 5.153443894e-08f))))))));
-  q = fabsf(t) <= 2.0f ? qs : t - 2.0f * (U1 / t);
-}
-
 // the monotonised (MC-limited) centred slope of two one-sided differences
 __device__ __forceinline__ float mc_slope(float t1, float t3) {
   const float t2 = 0.5f * (t1 + t3);
@@ -64,7 +45,7 @@ __device__ __forceinline__ void quad_step(float xxup, float xxc, float xxdn, flo
   const float dt = at_target ? dtup : dtdn;
   const float rdt = 1.0f / dt;
   float ex, U0, U1, q;
-  parab_moments(dt, ex, U0, U1, q);
+  dev::parab_moments(dt, ex, U0, U1, q);
   float s1, s3;
   if (at_target) {
     s1 = (ssc - ssup) * rdt;

@@ -35,21 +35,6 @@ struct SchArgs {
   float wj;
 };
 
-// parab_moments (SRC/rt_solvers_mod.f90:862-877), E from the hardware exponential
-__device__ __forceinline__ void parab_moments(float t, float &E, float &U0, float &U1, float &q) {
-  const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
-                   t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
-                   t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
-  E = __builtin_amdgcn_exp2f(fminf(t, 87.0f) * -1.44269504088896341f);
-  const bool thin = fabsf(t) <= 1.0f;
-  U0 = thin ? t * (1.0f - ps) : 1.0f - E;
-  U1 = thin ? t * ps : t - U0;
-  const float qs = t * t * (3.333333433e-01f + t * (-8.333329856e-02f + t * (1.666668802e-02f +
-                   t * (-2.777901944e-03f + t * (3.968053206e-04f + t * (-4.948913192e-05f +
-                   t * (5.514686109e-06f + t * (-5.894584660e-07f + t * 5.153443894e-08f))))))));
-  q = fabsf(t) <= 2.0f ? qs : t - 2.0f * (U1 / t);
-}
-
 __global__ void __launch_bounds__(1024) sch_sweep(const SchArgs a) {
   using namespace dev;
   extern __shared__ float sI[];  // two planes
@@ -127,7 +112,7 @@ __global__ void __launch_bounds__(1024) sch_sweep(const SchArgs a) {
         const double rdt = 1.0 / dt;
         const float rdt4 = (float)rdt;
         float ex, U0, U1, q;
-        parab_moments((float)dt, ex, U0, U1, q);
+        dev::parab_moments((float)dt, ex, U0, U1, q);
         const float r1mbc = 1.0f / (1.0f - betc), Bc = betc * r1mbc;
         const float gc = hk * r1mbc * aT / xx;
         const float rhoc = ss / Bc, t1s = (ss - ssup) * rdt4, sgns = copysignf(1.0f, t1s);

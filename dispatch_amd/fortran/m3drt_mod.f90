@@ -417,7 +417,7 @@ module m3drt
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
-  public :: m3drt_lc_sweep_hermite
+  public :: m3drt_lc_sweep_hermite, m3drt_lc_sweep_hermite2
   public :: m3drt_sc_sweep_hermite, m3drt_sc_adjoint, m3drt_line_opacity_equi, m3drt_line_opacity
   public :: m3drt_window_opacity, m3drt_rotate_array
   public :: m3drt_bb_opacity, m3drt_bb_rates, m3drt_bf_rates, m3drt_work_to_transfer
@@ -496,6 +496,15 @@ module m3drt
       type(m3drt_hermite_fields), intent(in) :: fields
       type(m3drt_lc_ghost), intent(in) :: ghost
     end function m3drt_lc_sweep_hermite
+    integer(c_int) function m3drt_lc_sweep_hermite2(ctx, desc, bufs, fields, ghost) &
+        bind(C, name='m3drt_lc_sweep_hermite2')
+      import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_sweep_bufs, m3drt_hermite_fields, m3drt_lc_ghost
+      type(c_ptr), value :: ctx
+      type(m3drt_sweep_desc), intent(in) :: desc
+      type(m3drt_sweep_bufs), intent(in) :: bufs
+      type(m3drt_hermite_fields), intent(in) :: fields
+      type(m3drt_lc_ghost), intent(in) :: ghost
+    end function m3drt_lc_sweep_hermite2
     integer(c_int) function m3drt_sc_adjoint(ctx, desc, bufs) bind(C, name='m3drt_sc_adjoint')
       import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_adjoint_bufs
       type(c_ptr), value :: ctx

@@ -234,8 +234,10 @@ class Context:
 
     def lc_sweep_hermite(self, chi, S, ghost_chi, ghost_S, ghost_I, *, dir, ds, vec, fields, ghost="top",
                          ghost_fields=None, layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None,
-                         exit_I=None, get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30):
-        """long_char_hermite along every column (m3drt_lc_sweep_hermite) with J += wj*I
+                         exit_I=None, get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, hybrid=False):
+        """long_char_hermite along every column (m3drt_lc_sweep_hermite; hybrid=True:
+        long_char_hermite2, m3drt_lc_sweep_hermite2, which reads only pe4 and dpe_dz of
+        fields and pe, dpe_dz of ghost_fields) with J += wj*I
         and PsiJ += wj*Psi fused.  fields: dict of m3drt_hermite_fields (beta a cube
         like chi; pe4, temp, dpe_dz, dlnt_dz per cell; hknu); ghost "top", "bottom" or
         "upstream", the last two with ghost_fields = dict(pe, dpe_dz, temp, dlnt_dz) of
@@ -257,15 +259,18 @@ class Context:
         for k, t in tensors.items():
             if t is not None and (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()):
                 raise M3drtError(f"{k} must be a contiguous float32 device tensor")
-        if tuple(fields["beta"].shape) != tuple(chi.shape):
-            raise M3drtError("beta is a cube shaped like chi")
-        for k in ("pe4", "temp", "dpe_dz", "dlnt_dz"):
+        cellk = ("pe4", "dpe_dz") if hybrid else ("pe4", "temp", "dpe_dz", "dlnt_dz")
+        ghostk = ("pe", "dpe_dz") if hybrid else ("pe", "dpe_dz", "temp", "dlnt_dz")
+        if not hybrid:
+            if tuple(fields["beta"].shape) != tuple(chi.shape):
+                raise M3drtError("beta is a cube shaped like chi")
+            if fields["hknu"].numel() != nf:
+                raise M3drtError("hknu holds nf values")
+        for k in cellk:
             if fields[k].numel() != nx * ny * nz:
                 raise M3drtError(f"{k} holds nx*ny*nz cells")
-        if fields["hknu"].numel() != nf:
-            raise M3drtError("hknu holds nf values")
         if ghost != "top":
-            for k in ("pe", "dpe_dz", "temp", "dlnt_dz"):
+            for k in ghostk:
                 if ghost_fields[k].numel() != nx * ny:
                     raise M3drtError(f"ghost {k} holds nx*ny cells")
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
@@ -277,12 +282,13 @@ class Context:
         self._bind_stream()
         L = lib()
         if not getattr(L, "_lch_ok", False):
-            L.m3drt_lc_sweep_hermite.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(SweepBufs),
-                                                 C.POINTER(HermiteFields), C.POINTER(LcGhost)]
-            L.m3drt_lc_sweep_hermite.restype = C.c_int
+            for fn in (L.m3drt_lc_sweep_hermite, L.m3drt_lc_sweep_hermite2):
+                fn.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(SweepBufs), C.POINTER(HermiteFields),
+                               C.POINTER(LcGhost)]
+                fn.restype = C.c_int
             L._lch_ok = True
-        _check(L.m3drt_lc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h), C.byref(g)),
-               "m3drt_lc_sweep_hermite")
+        name = "m3drt_lc_sweep_hermite2" if hybrid else "m3drt_lc_sweep_hermite"
+        _check(getattr(L, name)(self._h, C.byref(d), C.byref(b), C.byref(h), C.byref(g)), name)
 
     def sc_adjoint(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, I, *, dir, ds, vec,
                    chibar, Sbar, layout="native", wj=0.0, Jbar=None, Ibar=None, exit_bar=None,

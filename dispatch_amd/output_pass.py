@@ -4,8 +4,9 @@
 For each upward output ray the work arrays are rotated so that the rays become
 columns (rotate_array, SRC/rt_patch_mod.f90:3809-3863, pivoting about the top
 plane), a column solver runs from the LTE ghost plane below the bottom
-(long_char_linear, long_char_quadratic or long_char_hermite,
-SRC/rt_solvers_mod.f90:666-757, 891-1020, 1052-1504), and the exit planes give
+(long_char_linear, long_char_quadratic, long_char_hermite or
+long_char_hermite2, SRC/rt_solvers_mod.f90:666-757, 891-1020, 1052-1504,
+1536-1895), and the exit planes give
 save_intensity's emergent products ie and flux (SRC/rt_patch_mod.f90:4151-4195,
 dispatch_amd/save.py).  long_char_hermite needs the thermodynamic state along the
 ray (the reference admits it only with ray-aligned compositions): T and Pe are
@@ -30,7 +31,7 @@ import torch
 from . import m3drt, synth
 from .save import emergent_products
 
-SOLVERS = ("long_linear", "quadratic", "hermite")
+SOLVERS = ("long_linear", "quadratic", "hermite", "hermite2")
 
 
 def rotate(ctx, x, vec, ds, bot2pivot):
@@ -76,8 +77,9 @@ def ray_state(ctx, box, chi, S, pe, vec):
 
 def solve(ctx, solver, chi, S, gh, fields, ghost, box, vec, exit_I):
     kw = dict(dir=1, ds=box.ds, vec=vec, layout="native", exit_I=exit_I)
-    if solver == "hermite":
-        ctx.lc_sweep_hermite(chi, S, *gh, fields=fields, ghost="bottom", ghost_fields=ghost, **kw)
+    if solver in ("hermite", "hermite2"):
+        ctx.lc_sweep_hermite(chi, S, *gh, fields=fields, ghost="bottom", ghost_fields=ghost,
+                             hybrid=solver == "hermite2", **kw)
     else:
         ctx.sc_sweep(chi, S, *gh, None, None, None, solver=solver, **kw)
 
@@ -136,7 +138,7 @@ def run(nf=256, windows=1, solvers=SOLVERS, rays=None, device=0, save_dir=None, 
     ctx.close()
     # per (f, cell): chi and S read (8 B; the Hermite sweep also beta, 12 B); the exit
     # plane written once per column
-    rd = dict(long_linear=8, quadratic=8, hermite=12)
+    rd = dict(long_linear=8, quadratic=8, hermite=12, hermite2=8)
     out = dict(workload=f"output pass: 120x120x230 box, {len(up)} upward rays, {windows} window(s) of {nf} f",
                ms_per_ray={k: float(np.median(v)) for k, v in tim.items()})
     out["GB/s"] = {s: rd[s] * nbytes_cube / 4 / (out["ms_per_ray"][s] * 1e-3) / 1e9 for s in solvers}

@@ -295,6 +295,14 @@ typedef struct {
 int m3drt_lc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs,
                            const m3drt_hermite_fields *fields, const m3drt_lc_ghost *ghost);
 
+/* long_char_hermite2 (SRC/rt_solvers_mod.f90:1536-1895), the 'hermite2' output solver:
+ * long_char_hermite's product-rule optical depth (slopes clamped to 3x the secant)
+ * with long_char_quadratic's MC-limited source slope and parab_moments weights (A, B, C
+ * in FP64, as the reference).  Of fields only pe4 and dpe_dz are read, of ghost only
+ * pe and dpe_dz.  Otherwise as m3drt_lc_sweep_hermite. */
+int m3drt_lc_sweep_hermite2(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs,
+                            const m3drt_hermite_fields *fields, const m3drt_lc_ghost *ghost);
+
 /* ------------------------------------------------------------------ line rates */
 
 #define M3DRT_RATES_PRECOND 1   /* &nlte_params preconditioning=T */

@@ -12,7 +12,10 @@
  *   herm_plane       :1171-1250 and the plane order of :1100-1160
  *   ghost_thermo     :1257-1266
  * and long_char_linear (:666-757), the trapezoid step with linear_weights along each
- * column, with the split parts of the 'largest' rate formulation.
+ * column, with the split parts of the 'largest' rate formulation, and
+ * long_char_hermite2 (:1536-1895): long_char_hermite's product-rule optical depth
+ * (slopes clamped to 3x the secant) with long_char_quadratic's MC-limited source
+ * slope and parab_moments weights (A, B, C in FP64).
  *
  * Pinning (tests/test_oracle_lch.py): the four weights against numerical
  * quadrature of the Bernstein integrals over -3 <= t <= 87, and their sum against
@@ -26,6 +29,9 @@
  * long_char_linear is pinned by test_oracle_lch.py too: along a vertical ray it
  * equals short_char (whose 4x4 stencil then reads the column itself), a source linear
  * in tau with the ghost on the particular solution is exact, and Irem + Psi S = I.
+ * long_char_hermite2 likewise: a source linear in tau is exact on every plane in the
+ * three boundary kinds (the parabolic step reproduces it), and a cubic chi with
+ * S = 0 attenuates by the exact optical depth.
  */
 #include <math.h>
 #include <stdint.h>
@@ -327,4 +333,187 @@ void orc_long_char_linear_f32(const orc_sc_params *p, const float *chi, const fl
   }
 #undef W_
 #undef O_
+}
+
+typedef struct {
+  float rdsup, dsup1, dsup2, imax;
+  double mdtau;
+  int get_rates;
+} lch2_c;
+
+static double tau_floor64(double dt, double mdtau)
+{
+  return MAX_(copysign(MAX_(fabs(dt), (double)1e-20f), dt), mdtau);
+}
+
+/* herm2_step (SRC/rt_solvers_mod.f90:1768-1853); onesided: 1 the third sample is
+ * real (last plane, nz >= 3), 0 it is the ghost, -1 not the last plane */
+static void herm2_step(const lch2_c *k, float xxup, float xxc, float xx3, float ssup, float ssc, float ss3,
+                       float Inuup, float peu, float rpeu, float gpu, float pec, float rpec, float gpc, float pe3,
+                       float rpe3, float gp3, int ct, int onesided, float *Inu, float *plam)
+{
+  const float hh = 0.5f * k->rdsup;
+  const float dchi = xxc - xxup;
+  float t1 = dchi * k->rdsup;
+  float sgn = copysignf(1.0f, t1);
+  const float rxpu = xxup * rpeu, rxpc = xxc * rpec, rxp3 = xx3 * rpe3;
+  float drxpu, drxpc;
+  if (ct) {
+    drxpc = (rxp3 - rxpu) * hh;
+    drxpu = (4.0f * rxpc - 3.0f * rxpu - rxp3) * hh;
+  } else {
+    drxpu = (rxpc - rxp3) * hh;
+    drxpc = (rxp3 - 4.0f * rxpu + 3.0f * rxpc) * hh;
+  }
+  float mup = rxpu * gpu + peu * drxpu;
+  float mc = rxpc * gpc + pec * drxpc;
+  mup = sgn * MAX_(0.0f, MIN_(sgn * mup, 3.0f * fabsf(t1)));
+  mc = sgn * MAX_(0.0f, MIN_(sgn * mc, 3.0f * fabsf(t1)));
+  const double dt = tau_floor64((double)(k->dsup1 * (xxup + xxc) + k->dsup2 * (mup - mc)), k->mdtau);
+  const double rdt = 1.0 / dt;
+  float ex, U0, U1, q;
+  orc_parab_moments((float)dt, &ex, &U0, &U1, &q);
+  const float dss = ssc - ssup;
+  float dsdtau;
+  if (ct) {
+    float t3 = (xx3 - xxc) * k->rdsup;
+    const float sgn3 = copysignf(1.0f, t3);
+    const float drxp3 = (3.0f * rxp3 - 4.0f * rxpc + rxpu) * hh;
+    float m3 = rxp3 * gp3 + pe3 * drxp3;
+    m3 = sgn3 * MAX_(0.0f, MIN_(sgn3 * m3, 3.0f * fabsf(t3)));
+    const double dt3 = tau_floor64((double)(k->dsup1 * (xxc + xx3) + k->dsup2 * (mc - m3)), k->mdtau);
+    t1 = dss * (float)rdt;
+    t3 = (ss3 - ssc) / (float)dt3;
+    dsdtau = 0.5f * (t1 + t3);
+    dsdtau = (copysignf(1.0f, t1) + copysignf(1.0f, t3)) * MIN_(MIN_(fabsf(t1), fabsf(t3)), fabsf(0.5f * dsdtau));
+  } else {
+    t1 = dss * (float)rdt;
+    dsdtau = t1;
+    if (onesided == 1 && xxc > 0.0f) {
+      dsdtau = (3.0f * ssc - 4.0f * ssup + ss3) * hh / xxc;
+      sgn = copysignf(1.0f, t1);
+      dsdtau = sgn * MAX_(0.0f, MIN_(sgn * dsdtau, 2.0f * fabsf(t1)));
+    }
+  }
+  const double B = (2.0 * U1 - q) * rdt, A = U0 - B, C = q - U1;
+  float I = Inuup * ex + (float)A * ssup + (float)B * ssc + (float)C * dsdtau;
+  *Inu = MAX_(-k->imax, MIN_(k->imax, I));
+  *plam = (k->get_rates && dt > 0 && xxc > 0) ? (float)B / xxc : 0.0f;
+}
+
+/* herm2_first (SRC/rt_solvers_mod.f90:1860-1893) */
+static void herm2_first(const lch2_c *k, float xxup, float xxc, float xxdn, float ssup, float ssc, float Inuup,
+                        float pec, float rpec, float rpedn, float gpc, float *Inu, float *plam)
+{
+  const float t1 = (xxc - xxup) * k->rdsup;
+  const float sgn = copysignf(1.0f, t1);
+  const float rxpc = xxc * rpec;
+  float mc = rxpc * gpc + pec * (xxdn * rpedn - rxpc) * k->rdsup;
+  mc = sgn * MAX_(0.0f, MIN_(sgn * mc, 3.0f * fabsf(t1)));
+  const double dt = tau_floor64((double)(k->dsup1 * (xxup + xxc) + k->dsup2 * (t1 - mc)), k->mdtau);
+  const double rdt = 1.0 / dt;
+  float ex, U0, U1, q;
+  orc_parab_moments((float)dt, &ex, &U0, &U1, &q);
+  const float dsdtau = (ssc - ssup) * (float)rdt;
+  const double B = (2.0 * U1 - q) * rdt, A = U0 - B, C = q - U1;
+  float I = Inuup * ex + (float)A * ssup + (float)B * ssc + (float)C * dsdtau;
+  *Inu = MAX_(-k->imax, MIN_(k->imax, I));
+  *plam = (k->get_rates && dt > 0 && xxc > 0) ? (float)B / xxc : 0.0f;
+}
+
+void orc_long_char_hermite2_f32(const orc_sc_params *p, const float *chi, const float *S, float *I,
+                                const float *pe4, const float *dpe_dz, int ghost_kind, const float *gpe,
+                                const float *gdpe, float *Psi)
+{
+  const int nf = p->nf, nx = p->nx, ny = p->ny, nz = p->nz, dir = p->dir;
+  const size_t sx = (size_t)nf, sy = sx * nx, sz = sy * ny, nxy = (size_t)nx * ny;
+  lch2_c k;
+  const float ds = (float)p->ds[2];
+  const float dsup = (float)((double)ds / fabs(p->vec[2]));
+  k.rdsup = 1.0f / dsup;
+  k.dsup1 = 0.5f * dsup;
+  k.dsup2 = 1.0f / 12.0f * dsup * dsup;
+  k.mdtau = -p->maser_dtau_max;
+  k.imax = (float)p->maser_Inu_max;
+  k.get_rates = p->get_rates;
+  const float gfac = (float)(dir * fabs(p->vec[2]));
+  const int zs = dir > 0 ? 1 : nz, ze = dir > 0 ? nz : 1;
+  const int ghosted = ghost_kind == ORC_LCH_UPSTREAM || ghost_kind == ORC_LCH_BOTTOM;
+#define W_(a, f, x, y, z) a[(f) + sx * (x) + sy * (y) + sz * (size_t)(z)]
+#define C_(a, x, y, z) a[(x) + (size_t)nx * (y) + nxy * (size_t)((z) - 1)]
+#define G_(a, x, y) a[(x) + (size_t)nx * (y)]
+#define P_(f, x, y, z) Psi[(f) + sx * (x) + sy * (y) + sz * (size_t)((z) - 1)]
+#define STORE(Z)                                                                                               \
+  do {                                                                                                         \
+    W_(I, f, x, y, (Z)) = Inu;                                                                                 \
+    if (p->get_rates && Psi) P_(f, x, y, (Z)) = pl;                                                            \
+  } while (0)
+  /* the first plane */
+  {
+    const int z = zs, zup = z - dir, zdn = z + dir;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const float pec = C_(pe4, x, y, z), rpec = 1.0f / pec, gpc = gfac * C_(dpe_dz, x, y, z);
+        const float pedn = C_(pe4, x, y, zdn), rpedn = 1.0f / pedn;
+        for (int f = 0; f < nf; f++) {
+          float Inu, pl;
+          if (ghosted) {
+            const float peu = G_(gpe, x, y), rpeu = 1.0f / peu, gpu = gfac * G_(gdpe, x, y);
+            const float gpdn = gfac * C_(dpe_dz, x, y, zdn);
+            herm2_step(&k, W_(chi, f, x, y, zup), W_(chi, f, x, y, z), W_(chi, f, x, y, zdn), W_(S, f, x, y, zup),
+                       W_(S, f, x, y, z), W_(S, f, x, y, zdn), W_(I, f, x, y, zup), peu, rpeu, gpu, pec, rpec, gpc,
+                       pedn, rpedn, gpdn, 1, -1, &Inu, &pl);
+          } else {
+            herm2_first(&k, W_(chi, f, x, y, zup), W_(chi, f, x, y, z), W_(chi, f, x, y, zdn), W_(S, f, x, y, zup),
+                        W_(S, f, x, y, z), W_(I, f, x, y, zup), pec, rpec, rpedn, gpc, &Inu, &pl);
+          }
+          STORE(z);
+        }
+      }
+  }
+  /* interior planes */
+  for (int z = zs + dir; z != ze; z += dir) {
+    const int zup = z - dir, zdn = z + dir;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const float peu = C_(pe4, x, y, zup), pec = C_(pe4, x, y, z), pedn = C_(pe4, x, y, zdn);
+        const float rpeu = 1.0f / peu, rpec = 1.0f / pec, rpedn = 1.0f / pedn;
+        const float gpu = gfac * C_(dpe_dz, x, y, zup), gpc = gfac * C_(dpe_dz, x, y, z),
+                    gpdn = gfac * C_(dpe_dz, x, y, zdn);
+        for (int f = 0; f < nf; f++) {
+          float Inu, pl;
+          herm2_step(&k, W_(chi, f, x, y, zup), W_(chi, f, x, y, z), W_(chi, f, x, y, zdn), W_(S, f, x, y, zup),
+                     W_(S, f, x, y, z), W_(S, f, x, y, zdn), W_(I, f, x, y, zup), peu, rpeu, gpu, pec, rpec, gpc,
+                     pedn, rpedn, gpdn, 1, -1, &Inu, &pl);
+          STORE(z);
+        }
+      }
+  }
+  /* the last plane: third sample two planes upwind, or the ghost when nz = 2 */
+  {
+    const int z = ze, zup = z - dir, zff = z - 2 * dir;
+    const int real3 = zff >= 1 && zff <= nz;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const float peu = C_(pe4, x, y, zup), pec = C_(pe4, x, y, z);
+        const float rpeu = 1.0f / peu, rpec = 1.0f / pec;
+        const float gpu = gfac * C_(dpe_dz, x, y, zup), gpc = gfac * C_(dpe_dz, x, y, z);
+        float rpe3;
+        if (real3) rpe3 = 1.0f / C_(pe4, x, y, zff);
+        else if (ghost_kind == ORC_LCH_UPSTREAM) rpe3 = 1.0f / G_(gpe, x, y);
+        else rpe3 = rpec;
+        for (int f = 0; f < nf; f++) {
+          float Inu, pl;
+          herm2_step(&k, W_(chi, f, x, y, zup), W_(chi, f, x, y, z), W_(chi, f, x, y, zff), W_(S, f, x, y, zup),
+                     W_(S, f, x, y, z), W_(S, f, x, y, zff), W_(I, f, x, y, zup), peu, rpeu, gpu, pec, rpec, gpc,
+                     0.0f, rpe3, 0.0f, 0, real3 ? 1 : 0, &Inu, &pl);
+          STORE(z);
+        }
+      }
+  }
+#undef STORE
+#undef W_
+#undef C_
+#undef G_
+#undef P_
 }

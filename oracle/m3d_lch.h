@@ -32,6 +32,13 @@ void orc_long_char_hermite_f32(const orc_sc_params *p, const float *chi, const f
 void orc_long_char_linear_f32(const orc_sc_params *p, const float *chi, const float *S, float *I, float *Psi,
                               float *Irem, float *escape);
 
+/* long_char_hermite2 (SRC/rt_solvers_mod.f90:1536-1895): as orc_long_char_hermite_f32
+ * without beta, T and their gradients (its source treatment does not use them);
+ * gpe, gdpe (ny, nx) the ghost's Pe for ORC_LCH_BOTTOM / ORC_LCH_UPSTREAM. */
+void orc_long_char_hermite2_f32(const orc_sc_params *p, const float *chi, const float *S, float *I,
+                                const float *pe4, const float *dpe_dz, int ghost_kind, const float *gpe,
+                                const float *gdpe, float *Psi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -802,6 +802,35 @@ def long_char_linear(chi, S, I, dir, ds, vec, split_rates=False, get_rates=True,
     return I, psi, irem, esc
 
 
+def long_char_hermite2(chi, S, I, pe4, dpe_dz, dir, ds, vec, ghost="top", gpe=None, gdpe=None, get_rates=True,
+                       maser_dtau_max=2.0, maser_Inu_max=1e30):
+    """long_char_hermite2 (SRC/rt_solvers_mod.f90:1536-1895) along every column: work
+    arrays (nz+2, ny, nx, nf); pe4, dpe_dz (nz, ny, nx); the ghost's gpe, gdpe (ny, nx)
+    for "bottom" / "upstream".  Returns (I, Psi)."""
+    L = lib()
+    if not getattr(L, "_lch2_ok", False):
+        L.orc_long_char_hermite2_f32.argtypes = [C.POINTER(ScParams)] + [C.c_void_p] * 5 + [C.c_int] + \
+            [C.c_void_p] * 3
+        L.orc_long_char_hermite2_f32.restype = None
+        L._lch2_ok = True
+    keep = _Keep()
+    chi, S = np.ascontiguousarray(chi, np.float32), np.ascontiguousarray(S, np.float32)
+    I = np.array(I, np.float32, copy=True, order="C")
+    nz2, ny, nx, nf = chi.shape
+    nz = nz2 - 2
+    if nz < 2:
+        raise ValueError("long_char_hermite2 needs nz >= 2")
+    kind = LCH_GHOST[ghost]
+    if kind and (gpe is None or gdpe is None):
+        raise ValueError("the bottom and upstream ghosts need gpe and gdpe")
+    psi = np.zeros((nz, ny, nx, nf), np.float32)
+    p = _params(nf, nx, ny, nz, dir, ds, vec, maser_dtau_max, maser_Inu_max, False, get_rates)
+    f32 = lambda a: keep.p(a, np.float32)
+    L.orc_long_char_hermite2_f32(C.byref(p), _p(chi), _p(S), _p(I), f32(pe4), f32(dpe_dz), kind, f32(gpe), f32(gdpe),
+                                 _p(psi))
+    return I, psi
+
+
 def short_char2_adjoint(chi, S, I, chi500, chi500_up, g500, Ibar, dir, ds, vec, maser_dtau_max=2.0,
                         maser_Inu_max=1e30, chibar=None, Sbar=None):
     """Reverse of short_char2's S step in FP64 (oracle/m3d_adjoint.c): work arrays

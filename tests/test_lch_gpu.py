@@ -142,3 +142,48 @@ def test_long_char_linear_columns(ctx, layout, vec, nz, split):
         check32(back(Esc), esc, "escape")
     last = case["nz"] if case["dir"] > 0 else 1
     check32(pback(exitI), I[last], "exit plane")
+
+
+@pytest.mark.parametrize("layout", ["native", "reference"])
+@pytest.mark.parametrize("kind", ["top", "bottom", "upstream"])
+@pytest.mark.parametrize("vec,nz", [((0.0, 0.0, 0.78867513), 12), ((0.0, 0.0, -0.5), 3), ((0.0, 0.0, 1.0), 2),
+                                    ((0.0, 0.0, -0.21132487), 2)])
+def test_lc_sweep_hermite2(ctx, layout, kind, vec, nz):
+    """long_char_hermite2 (m3drt_lc_sweep_hermite2) against the oracle: I, Psi, J,
+    PsiJ, the exit plane; masers included (the FP64 step floor and the clamp)."""
+    from oracle import oracle as O
+    case = make_case(nf=20, nx=5, ny=4, nz=nz, seed=80 + nz, vec=vec, top_ghost_zero=(kind == "top"),
+                     maser=(nz == 12))
+    h = hermite_inputs(case, seed=nz + 2)
+    g = ghost_thermo(h, case, kind)
+    print(f"long_char_hermite2 {kind} {vec} nz={nz} {layout}", flush=True)
+    og = {} if g is None else dict(gpe=g["pe"], gdpe=g["dpe_dz"])
+    I, psi = O.long_char_hermite2(case["chi"], case["S"], case["I"], h["pe4"], h["dpe"][2], case["dir"], case["ds"],
+                                  case["vec"], ghost=kind, **og)
+    assert np.all(np.isfinite(I)) and np.all(np.isfinite(psi))
+    gz = case["gz"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(a))
+        pla = lambda a: dev(plane_to_native(a[gz]))
+        back = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pback = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        cub = lambda a: dev(a)
+        pla = lambda a: dev(a[gz])
+        back = pback = lambda t: t.double().cpu().numpy()
+    c = cub(interior(case["chi"]))
+    J, PsiJ = torch.full_like(c, 0.125), torch.full_like(c, 0.5)
+    Iout, Psi = torch.zeros_like(c), torch.zeros_like(c)
+    exitI = torch.zeros_like(pla(case["chi"]))
+    fields = dict(pe4=dev(h["pe4"]), dpe_dz=dev(h["dpe"][2]))
+    gf = None if g is None else dict(pe=dev(g["pe"]), dpe_dz=dev(g["dpe_dz"]))
+    ctx.lc_sweep_hermite(c, cub(interior(case["S"])), pla(case["chi"]), pla(case["S"]), pla(case["I"]),
+                         dir=case["dir"], ds=case["ds"], vec=case["vec"], fields=fields, ghost=kind, ghost_fields=gf,
+                         layout=layout, wj=0.25, J=J, PsiJ=PsiJ, I_out=Iout, Psi_out=Psi, exit_I=exitI, hybrid=True)
+    torch.cuda.synchronize()
+    check32(back(Iout), interior(I), "I")
+    check32(back(Psi), psi, "Psi")
+    check32(back(J), 0.125 + 0.25 * interior(I).astype(np.float64), "J")
+    check32(back(PsiJ), 0.5 + 0.25 * psi.astype(np.float64), "PsiJ")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(pback(exitI), I[last], "exit plane")

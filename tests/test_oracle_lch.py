@@ -208,3 +208,65 @@ def test_long_char_linear_source_is_exact(vec):
     lam = psi.astype(np.float64) * chi0
     np.testing.assert_allclose(irem + lam * S[1:-1], I[1:-1], rtol=2e-6)
     np.testing.assert_allclose(esc, 1 - lam, rtol=2e-6)
+
+
+# ---------------------------------------------------------------- long_char_hermite2
+
+@pytest.mark.parametrize("kind", ["top", "bottom", "upstream"])
+@pytest.mark.parametrize("vec", [(0.3, -0.4, 0.866), (-0.2, 0.1, -0.5)])
+@pytest.mark.parametrize("nz", [2, 3, 7])
+def test_hermite2_linear_source_is_exact(kind, vec, nz):
+    """The parabolic step with MC-limited (interior), one-sided (last plane, nz >= 3)
+    or secant (first plane under the top ghost, last plane at nz = 2) source slopes
+    reproduces a source linear in tau on every plane."""
+    nf, nx, ny = 4, 3, 2
+    c = column_case(nf, nx, ny, nz, vec, seed=nz)
+    mu = c["vec"][2]
+    chi0 = np.float32(1.1e-6)
+    chi = np.full(c["shape"], chi0, np.float32)
+    zc = (np.arange(nz + 2, dtype=np.float64) - 1) * DS[2]
+    S = np.broadcast_to((0.4 + 1.5e-7 * zc)[:, None, None, None], c["shape"])
+    dSdtau = 1.5e-7 * mu / float(chi0)
+    Iex = S - dSdtau
+    I0 = np.zeros(c["shape"], np.float32)
+    I0[c["gz"]] = Iex[c["gz"]]
+    pe = np.full((nz, ny, nx), 9.0, np.float32)
+    zero = np.zeros((nz, ny, nx), np.float32)
+    gh = {} if kind == "top" else dict(gpe=np.full((ny, nx), 9.0, np.float32), gdpe=np.zeros((ny, nx), np.float32))
+    I, psi = O.long_char_hermite2(chi, S.astype(np.float32), I0, pe, zero, c["dir"], DS, c["vec"], ghost=kind, **gh)
+    assert abs(dSdtau) / 0.4 > 1e-2
+    np.testing.assert_allclose(I[1:-1], Iex[1:-1], rtol=2e-6)
+    # the Lambda diagonal is the parabolic weight of the target, B / chi, in (0, 1/chi)
+    lam = psi * chi0
+    assert np.all(lam > 0) and np.all(lam < 1)
+
+
+@pytest.mark.parametrize("kind", ["bottom", "upstream", "top"])
+@pytest.mark.parametrize("vec", [(0.5, 0.2, 0.84), (0.1, -0.3, -0.6)])
+def test_hermite2_attenuation_of_a_cubic_opacity(kind, vec):
+    nf, nx, ny, nz = 3, 2, 2, 9
+    c = column_case(nf, nx, ny, nz, vec, seed=3)
+    mu = c["vec"][2]
+    dz = DS[2]
+    zc = (np.arange(nz + 2, dtype=np.float64) - 1) * dz
+    H = nz * dz
+    a0 = 1.5e-8
+    a1, a2 = -0.5 * a0 / H, (0.0 if kind == "top" else 0.3 * a0 / H ** 2)
+    p0, p1 = 20.0, (0.0 if kind == "top" else -8.0 / H)
+    q = a0 + a1 * zc + a2 * zc ** 2
+    pe_all = p0 + p1 * zc
+    chi = np.broadcast_to((q * pe_all)[:, None, None, None], c["shape"]).astype(np.float32)
+    S = np.zeros(c["shape"], np.float32)
+    I0 = np.zeros(c["shape"], np.float32)
+    I0[c["gz"]] = 1.0
+    pe = np.broadcast_to(pe_all[1:-1, None, None], (nz, ny, nx)).astype(np.float32)
+    dpe = np.full((nz, ny, nx), p1, np.float32)
+    g = c["gz"]
+    gh = {} if kind == "top" else dict(gpe=np.full((ny, nx), pe_all[g], np.float32),
+                                       gdpe=np.full((ny, nx), p1, np.float32))
+    I, _ = O.long_char_hermite2(chi, S, I0, pe, dpe, c["dir"], DS, c["vec"], ghost=kind, **gh)
+    coef = np.polynomial.polynomial.polymul([a0, a1, a2], [p0, p1])
+    prim = np.polynomial.polynomial.polyint(coef)
+    cum = np.abs(np.polynomial.polynomial.polyval(zc, prim) - np.polynomial.polynomial.polyval(zc[g], prim)) / abs(mu)
+    assert cum[1:-1].max() > 1.0
+    np.testing.assert_allclose(I[1:-1, 0, 0, 0].astype(np.float64), np.exp(-cum)[1:-1], rtol=1e-5)

@@ -41,6 +41,10 @@ def test_output_pass_columns_match_oracle(ctx):
                                        h(fields["dpe_dz"]), h(fields["dlnt_dz"]), h(fields["hknu"]), 1, box.ds, vec,
                                        ghost="bottom", gpe=h(ghost["pe"]), gdpe=h(ghost["dpe_dz"]),
                                        gtemp=h(ghost["temp"]), gdlnt=h(ghost["dlnt_dz"]))
+        elif solver == "hermite2":
+            ctx.lc_sweep_hermite(c, s, *gh, fields=fields, ghost="bottom", ghost_fields=ghost, hybrid=True, **kw)
+            I, _ = O.long_char_hermite2(wchi, wS, wI, h(fields["pe4"]), h(fields["dpe_dz"]), 1, box.ds, vec,
+                                        ghost="bottom", gpe=h(ghost["pe"]), gdpe=h(ghost["dpe_dz"]))
         elif solver == "quadratic":
             ctx.sc_sweep(c, s, *gh, None, None, None, solver=solver, **kw)
             I, _ = O.long_char_quadratic(wchi, wS, wI, 1, box.ds, vec)
@@ -60,4 +64,4 @@ def test_output_pass_fluxes_agree():
     f = out["flux_mean"]
     assert all(np.isfinite(v) and v > 0 for v in f.values())
     # the three source integrations of one atmosphere: within a few per cent
-    assert abs(f["hermite"] / f["quadratic"] - 1) < 0.05 and abs(f["long_linear"] / f["quadratic"] - 1) < 0.05
+    assert all(abs(f[s] / f["quadratic"] - 1) < 0.05 for s in ("hermite", "hermite2", "long_linear"))
