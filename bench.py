@@ -56,6 +56,7 @@ def parse():
                    help="run a plain 4 GiB device copy first (HBM-counter calibration)")
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--no-batch1d", action="store_true")
+    p.add_argument("--no-output-pass", action="store_true")
     p.add_argument("--adjoint-fd", action="store_true",
                    help="check the adjoint's abundance gradients against FP64 central differences")
     p.add_argument("--nf", type=int, default=1024)
@@ -381,6 +382,14 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_batch1d:
         res["batch1d"] = batch1d_leg(ctx, args)
+        torch.cuda.empty_cache()
+    if not args.no_output_pass:
+        # the output pass on the same box: rotation to the 12 upward rays and the four
+        # column solvers over one window of 256 frequencies, the emergent fluxes
+        from dispatch_amd import output_pass
+        t0 = time.time()
+        res["output_pass"] = output_pass.run(nf=256, windows=1)
+        log(f"[bench] output pass in {time.time() - t0:.1f}s: {res['output_pass']['ms_per_ray']}")
         torch.cuda.empty_cache()
     if not args.no_cpu:
         chi32, S32 = synth.build_cubes(box, torch.float32)
