@@ -32,10 +32,21 @@ struct LchArgs {
 };
 
 // one sample of the column: the frequency's chi, S, beta; the cell's Pe, 1/Pe,
-// dPe/ds and dlnT/ds / T (ds along the ray)
+// dPe/ds and dlnT/ds / T (ds along the ray); and the quotients every step that
+// touches the sample forms from them (1/(1-beta), B, rho = S/B, 1/chi, chi/Pe), formed
+// once when the sample enters the window
 struct Samp {
   float chi, S, beta, pe, rpe, gp, aT;
+  float r1mb, B, rho, rx, rxp;
 };
+
+__device__ __forceinline__ void derive(Samp &p) {
+  p.r1mb = 1.0f / (1.0f - p.beta);
+  p.B = p.beta * p.r1mb;
+  p.rho = p.S / p.B;
+  p.rx = 1.0f / p.chi;
+  p.rxp = p.chi * p.rpe;
+}
 
 // P2 = w2/t and P3 = w3/t of the Bernstein weights on |t| <= 3, minimax of degree
 // 10 and 9 (the reference's fits, SRC/rt_solvers_mod.f90:1327-1352)
@@ -65,12 +76,18 @@ __device__ __forceinline__ void bezier3(float t, float &E, float &w0, float &w1,
   const float b0 = M3 * r3;
   const float b1 = 3.0f * (M2 * tb - M3) * r3;
   const float b2 = 3.0f * (M1 * tb2 - 2.0f * M2 * tb + M3) * r3;
-  const bool ser = t <= 3.0f;
-  const float tE = t * E;
-  w0 = ser ? tE * b3_p3(-t) : b0;
-  w1 = ser ? tE * b3_p2(-t) : b1;
-  w2 = ser ? t * b3_p2(t) : b2;
-  w3 = ser ? t * b3_p3(t) : M0 - b0 - b1 - b2;
+  if (t <= 3.0f) {
+    const float tE = t * E;
+    w0 = tE * b3_p3(-t);
+    w1 = tE * b3_p2(-t);
+    w2 = t * b3_p2(t);
+    w3 = t * b3_p3(t);
+  } else {
+    w0 = b0;
+    w1 = b1;
+    w2 = b2;
+    w3 = M0 - b0 - b1 - b2;
+  }
 }
 
 __device__ __forceinline__ float tau_floor(float dt, float mdtau) {
@@ -85,7 +102,7 @@ __device__ __forceinline__ void herm_step(const LchArgs &a, const Samp &u, const
   const float hh = 0.5f * a.rdsup;
   const float t1 = (c.chi - u.chi) * a.rdsup;
   const float sgn = copysignf(1.0f, t1);
-  const float rxpu = u.chi * u.rpe, rxpc = c.chi * c.rpe, rxp3 = t3.chi * t3.rpe;
+  const float rxpu = u.rxp, rxpc = c.rxp, rxp3 = t3.rxp;
   float drxpu, drxpc;
   if (center) {
     drxpc = (rxp3 - rxpu) * hh;
@@ -102,9 +119,9 @@ __device__ __forceinline__ void herm_step(const LchArgs &a, const Samp &u, const
   float ex, w0, w1, w2, w3;
   bezier3(dt, ex, w0, w1, w2, w3);
   // rho = S/B at the three samples and its parabola's slopes at both ends
-  const float r1mbu = 1.0f / (1.0f - u.beta), r1mbc = 1.0f / (1.0f - c.beta), r1mb3 = 1.0f / (1.0f - t3.beta);
-  const float Bu = u.beta * r1mbu, Bc = c.beta * r1mbc, B3 = t3.beta * r1mb3;
-  const float rhou = u.S / Bu, rhoc = c.S / Bc, rho3 = t3.S / B3;
+  const float r1mbu = u.r1mb, r1mbc = c.r1mb;
+  const float Bu = u.B, Bc = c.B;
+  const float rhou = u.rho, rhoc = c.rho, rho3 = t3.rho;
   float drhu, drhc;
   if (center) {
     drhc = (rho3 - rhou) * hh;
@@ -113,7 +130,7 @@ __device__ __forceinline__ void herm_step(const LchArgs &a, const Samp &u, const
     drhu = (rhoc - rho3) * hh;
     drhc = (rho3 - 4.0f * rhou + 3.0f * rhoc) * hh;
   }
-  const float rxxu = 1.0f / u.chi, rxxc = 1.0f / c.chi;
+  const float rxxu = u.rx, rxxc = c.rx;
   const float gu = hk * r1mbu * u.aT * rxxu;   // (dB/dtau)/B at up
   const float gc = hk * r1mbc * c.aT * rxxc;   // and at c
   const float dt3 = dt * a.third;
@@ -135,16 +152,16 @@ __device__ __forceinline__ void herm_first(const LchArgs &a, float xxup, float s
                                            float hk, float Iup, float &Inu, float &plam) {
   const float t1 = (c.chi - xxup) * a.rdsup;
   const float sgn = copysignf(1.0f, t1);
-  const float rxpc = c.chi * c.rpe;
-  float mc = rxpc * c.gp + c.pe * (d.chi * d.rpe - rxpc) * a.rdsup;
+  const float rxpc = c.rxp;
+  float mc = rxpc * c.gp + c.pe * (d.rxp - rxpc) * a.rdsup;
   mc = sgn * fmaxf(0.0f, fminf(sgn * mc, 2.0f * fabsf(t1)));
   const float dt = tau_floor(a.dsup1 * (xxup + c.chi) + a.dsup2 * (t1 - mc), a.mdtau);
   float ex, w0, w1, w2, w3;
   bezier3(dt, ex, w0, w1, w2, w3);
-  const float r1mbc = 1.0f / (1.0f - c.beta), r1mbdn = 1.0f / (1.0f - d.beta);
-  const float Bc = c.beta * r1mbc, Bdn = d.beta * r1mbdn;
+  const float r1mbc = c.r1mb;
+  const float Bc = c.B;
   const float Gc = hk * r1mbc * c.aT / c.chi;
-  const float rhoc = c.S / Bc, rhodn = d.S / Bdn;
+  const float rhoc = c.rho, rhodn = d.rho;
   float taudn = a.dsup1 * (c.chi + d.chi);
   taudn = copysignf(fmaxf(fabsf(taudn), 1e-20f), taudn);
   const float dSc = c.S * Gc + Bc * (rhodn - rhoc) / taudn;
@@ -190,6 +207,7 @@ __global__ void __launch_bounds__(256) lch_sweep(const LchArgs a) {
     p.rpe = 1.0f / p.pe;
     p.gp = a.gfac * a.dpe_dz[cz];
     p.aT = a.gfac * a.dlnt_dz[cz] / a.temp[cz];
+    derive(p);
     return p;
   };
   auto emit = [&](int s, float Inu, float plam) {
@@ -204,6 +222,7 @@ __global__ void __launch_bounds__(256) lch_sweep(const LchArgs a) {
   g.chi = a.gchi[go];
   g.S = a.gS[go];
   g.beta = g.pe = g.rpe = g.gp = g.aT = 0.0f;
+  g.r1mb = g.B = g.rho = g.rx = g.rxp = 0.0f;
   if (a.kind != M3DRT_LCH_TOP) {
     g.pe = a.gpe[col];
     g.rpe = 1.0f / g.pe;
@@ -211,6 +230,7 @@ __global__ void __launch_bounds__(256) lch_sweep(const LchArgs a) {
     const float rT = 1.0f / a.gtemp[col];
     g.aT = a.gfac * a.gdlnt[col] * rT;
     g.beta = fmaxf(expf(-hk * rT), FLT_MIN);
+    derive(g);
   }
   Samp pm = g, p0 = load(0), p1 = load(1);
   float Inu, plam;
@@ -236,6 +256,7 @@ __global__ void __launch_bounds__(256) lch_sweep(const LchArgs a) {
       pm.rpe = p1.rpe;
       pm.beta = p1.beta;
     }
+    derive(pm);
   }
   herm_step(a, p0, p1, pm, false, hk, Inu, Inu, plam);
   emit(a.nz - 1, Inu, plam);
