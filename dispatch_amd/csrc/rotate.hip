@@ -7,6 +7,7 @@
 // reference's order of the weighted sum -- HBM-bound, one read of each input element
 // per output element from the cache and one write.
 #include "m3drt_internal.hpp"
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -29,28 +30,32 @@ struct RotArgs {
   int fast_f;  // the unit-stride axis runs along f (else along x)
 };
 
+// I: the index type of the element number within a plane (32 bits where the plane
+// fits, as integer division is far cheaper there); the shifts are reduced to [0, n)
+// on the host, so a shifted index wraps at most once
+template <typename I>
 __global__ void __launch_bounds__(256) rotate_kernel(const RotArgs a) {
-  const int64_t n = (int64_t)a.nf * a.nx * a.ny;
+  const I n = (I)a.nf * a.nx * a.ny;
   const int z = blockIdx.y, k = blockIdx.z;
   const PlaneShift s = a.sh[z];
   const int64_t base = (int64_t)z * a.cs_z + (int64_t)k * a.cs_k;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (I i = blockIdx.x * (I)blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
     int f, x, y;
     if (a.fast_f) {
       f = (int)(i % a.nf);
-      const int64_t r = i / a.nf;
+      const I r = i / a.nf;
       x = (int)(r % a.nx);
       y = (int)(r / a.nx);
     } else {
       x = (int)(i % a.nx);
-      const int64_t r = i / a.nx;
+      const I r = i / a.nx;
       y = (int)(r % a.ny);
       f = (int)(r / a.ny);
     }
     // modulo(x + up - 1, n) + 1 in 1-based indices: (x + up) mod n here
-    int x1 = (x + s.up1) % a.nx, y1 = (y + s.up2) % a.ny;
-    x1 += x1 < 0 ? a.nx : 0;
-    y1 += y1 < 0 ? a.ny : 0;
+    int x1 = x + s.up1, y1 = y + s.up2;
+    x1 -= x1 >= a.nx ? a.nx : 0;
+    y1 -= y1 >= a.ny ? a.ny : 0;
     const int x2 = x1 + 1 == a.nx ? 0 : x1 + 1, y2 = y1 + 1 == a.ny ? 0 : y1 + 1;
     const float *p = a.in + base + (int64_t)f * a.cs_f;
     float v = s.a * p[x1 * a.cs_x + y1 * a.cs_y];
@@ -112,7 +117,10 @@ int rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *r) {
   a.fast_f = r->cs_f <= r->cs_x ? 1 : 0;
   const int64_t n = (int64_t)r->nf * r->nx * r->ny;
   const int bx = (int)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(rotate_kernel, dim3(bx, r->nz, r->nk), dim3(256), 0, ctx->stream, a);
+  if (n + (int64_t)bx * 256 <= INT32_MAX)
+    hipLaunchKernelGGL(rotate_kernel<int>, dim3(bx, r->nz, r->nk), dim3(256), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(rotate_kernel<int64_t>, dim3(bx, r->nz, r->nk), dim3(256), 0, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
   // the shifts stay in the scratch buffer until the kernel has read them
   M3DRT_HIP(hipStreamSynchronize(ctx->stream));

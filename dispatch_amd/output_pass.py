@@ -54,25 +54,26 @@ def ray_state(ctx, box, chi, S, pe, vec):
     nz = box.nz
     dz = box.ds[2]
     top = nz * dz
-    cubes = rotate(ctx, torch.stack([chi, S]), vec, box.ds, top)
+    chi_r = rotate(ctx, chi[None], vec, box.ds, top)[0]
+    S_r = rotate(ctx, S[None], vec, box.ds, top)[0]
     cell = torch.stack([box.T.float(), pe])[:, :, None]                  # (2, nz, 1, ny, nx)
     cell = rotate(ctx, cell.contiguous(), vec, box.ds, top)[:, :, 0]
     T, pe_r = cell[0].contiguous(), cell[1].contiguous()
     lam = torch.as_tensor(box.lam, dtype=torch.float64, device=chi.device)
-    hk_lam = synth.H_PLANCK * synth.C_LIGHT / (lam * synth.K_BOLTZ)     # h nu / k per frequency
-    beta = torch.clamp(torch.exp(-hk_lam[None, :, None, None] / T.double()[:, None]),
-                       min=np.finfo(np.float32).tiny).float().contiguous()
+    hk_lam = (synth.H_PLANCK * synth.C_LIGHT / (lam * synth.K_BOLTZ)).float()   # h nu / k per frequency
+    # beta = max(exp(-h nu / k T), tiny) in FP32, as lte_background_opacities keeps it
+    beta = torch.clamp(torch.exp(-hk_lam[None, :, None, None] / T[:, None]), min=np.finfo(np.float32).tiny)
     dpe = dz_gradient(pe_r, dz)
     dlnt = dz_gradient(torch.log(T), dz)
     # the LTE ghost plane below the bottom (lte_ghost): chi scaled by the chi500 ratio,
     # S = I = B_nu(T_ghost); its Pe and gradients extrapolated from the bottom plane
-    gh = synth.ghost_planes(box, cubes[0], torch.float32)["up"]
+    gh = synth.ghost_planes(box, chi_r, torch.float32)["up"]
     Tg = box.T_ghost_bot.float().contiguous()
     ghost = dict(pe=torch.clamp(pe_r[0] - dz * dpe[0], min=1e-3).contiguous(), dpe_dz=dpe[0].contiguous(),
                  temp=Tg, dlnt_dz=dlnt[0].contiguous())
-    fields = dict(beta=beta, pe4=pe_r, temp=T, dpe_dz=dpe.contiguous(), dlnt_dz=dlnt.contiguous(),
-                  hknu=hk_lam.float().contiguous())
-    return cubes[0].contiguous(), cubes[1].contiguous(), gh, fields, ghost
+    fields = dict(beta=beta.contiguous(), pe4=pe_r, temp=T, dpe_dz=dpe.contiguous(), dlnt_dz=dlnt.contiguous(),
+                  hknu=hk_lam.contiguous())
+    return chi_r, S_r, gh, fields, ghost
 
 
 def solve(ctx, solver, chi, S, gh, fields, ghost, box, vec, exit_I):
