@@ -74,6 +74,8 @@ class AliConfig:
     max_iter: int = 40
     conv: float = 1e-4
     ng: bool = True           # Ng acceleration (the reference's automatic trigger)
+    ng_step: int = 0          # > 0: the reference's fixed schedule instead (&nlte_params ng_step,
+    ng_start: int = 8         # ng_start; SRC/atom_mod.f90:278-287, 2500-2502)
     precond: bool = True      # preconditioned (ALI) rates; False: plain Lambda iteration
     fused: bool = True        # one kernel per (window, direction) for continuum, lines and units
     graph: bool = False       # replay the rate pass of an iteration as one captured HIP graph
@@ -225,6 +227,7 @@ class NlteProblem:
         self._graph = None
         self._graph_out = None
         self.history = []
+        self.ng_history = []
         self.ng = NgAcceleration() if cfg.ng else None
         self._oldb = []
         self._iter = 0
@@ -429,7 +432,14 @@ class NlteProblem:
             ng = self.ng
             ng.promote()
             fired = False
-            if ng.should_fire(self._iter, res["maxerr"], self.cfg.conv):
+            if self.cfg.ng_step > 0:
+                # mod(iter - ng_start, ng_step) == 1 with Fortran's mod (the sign of the
+                # dividend: no firing before ng_start), and a promoted Gram matrix
+                d = self._iter - self.cfg.ng_start
+                fire = d - self.cfg.ng_step * int(d / self.cfg.ng_step) == 1 and ng.s_done > 0
+            else:
+                fire = ng.should_fire(self._iter, res["maxerr"], self.cfg.conv)
+            if fire:
                 change = ng.extrapolate(self.b, self._oldb, self.nstar, self.totn)
                 if change is not None:
                     ng.iter_fired = self._iter
@@ -448,6 +458,7 @@ class NlteProblem:
                 torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             self.history.append((res["maxerr"], dt))
+            self.ng_history.append("ng" in res)
             if log:
                 log(f"[ali] iteration {it}: max|dn/n| = {res['maxerr']:.3e}  ({dt:.2f} s)")
             if abs(res["maxerr"]) < self.cfg.conv:
@@ -469,10 +480,13 @@ def main():
     ap.add_argument("--max-iter", type=int, default=60)
     ap.add_argument("--fp64-sweeps", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the rate pass eagerly every iteration")
+    ap.add_argument("--ng-step", type=int, default=0, help="> 0: Ng on the reference's fixed schedule")
+    ap.add_argument("--ng-start", type=int, default=8)
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
-    cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph)
+    cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph,
+                    ng_step=args.ng_step, ng_start=args.ng_start)
     t0 = time.perf_counter()
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
@@ -484,7 +498,10 @@ def main():
                iterations=n, converged=abs(p.history[-1][0]) < cfg.conv, maxerr=[e for e, _ in p.history],
                s_per_iteration=float(np.median(its)),
                line_updates_per_iteration=p.ncell * p.atom.nline * cfg.nq * len(p.vec),
-               b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph)
+               b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph,
+               ng=("fixed schedule" if cfg.ng_step > 0 else "automatic") if cfg.ng else "off",
+               ng_step=cfg.ng_step, ng_start=cfg.ng_start,
+               ng_fired=[i + 1 for i, h in enumerate(p.ng_history) if h])
     out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
     if args.fp64_sweeps:
         b32 = p.b.clone()
