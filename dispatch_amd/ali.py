@@ -92,6 +92,14 @@ class _Window:
     const: dict = field(default_factory=dict)
 
 
+def ng_fixed_fires(it: int, step: int, start: int) -> bool:
+    """The reference's fixed Ng schedule, mod(iter - ng_start, ng_step) == 1
+    (SRC/atom_mod.f90:2500-2502), with Fortran's mod: the result takes the dividend's
+    sign, so nothing fires before ng_start."""
+    d = it - start
+    return d - step * int(d / step) == 1
+
+
 class NgAcceleration:
     """Ng's three-difference extrapolation in ln b (SRC/ng_mod.f90, applied as
     atom_type%ng_acceleration does, SRC/atom_mod.f90:2469-2554): the Gram matrix of
@@ -433,10 +441,8 @@ class NlteProblem:
             ng.promote()
             fired = False
             if self.cfg.ng_step > 0:
-                # mod(iter - ng_start, ng_step) == 1 with Fortran's mod (the sign of the
-                # dividend: no firing before ng_start), and a promoted Gram matrix
-                d = self._iter - self.cfg.ng_start
-                fire = d - self.cfg.ng_step * int(d / self.cfg.ng_step) == 1 and ng.s_done > 0
+                # the fixed schedule, once a Gram matrix has been promoted
+                fire = ng_fixed_fires(self._iter, self.cfg.ng_step, self.cfg.ng_start) and ng.s_done > 0
             else:
                 fire = ng.should_fire(self._iter, res["maxerr"], self.cfg.conv)
             if fire:

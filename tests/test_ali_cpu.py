@@ -37,3 +37,13 @@ def test_strong_collisions_give_lte():
     p.solve()
     b = p.b.numpy().reshape(p.atom.nk, cfg.nz, -1)
     assert np.abs(b[:, :cfg.nz // 2] - 1).max() < 1e-3
+
+
+def test_fixed_ng_schedule_follows_fortran_mod():
+    """mod(iter - ng_start, ng_step) == 1 with Fortran's mod: with ng_step 4 and
+    ng_start 8 the extrapolation fires at 9, 13, 17, ...; before ng_start the
+    remainder is negative or zero and nothing fires (Python's % would fire at 1, 5)."""
+    from dispatch_amd.ali import ng_fixed_fires
+    assert [i for i in range(1, 30) if ng_fixed_fires(i, 4, 8)] == [9, 13, 17, 21, 25, 29]
+    assert [i for i in range(1, 20) if ng_fixed_fires(i, 3, 6)] == [7, 10, 13, 16, 19]
+    assert not any(ng_fixed_fires(i, 1, 8) for i in range(1, 40))   # mod(., 1) is never 1
