@@ -74,7 +74,7 @@ typedef struct {
   int32_t nf, nx, ny, nz;  /* frequencies, plane size, planes swept */
   int32_t dir;             /* +1: z increasing (up), -1: down (rt_patch_t%dir) */
   int32_t prec;            /* M3DRT_F32 or M3DRT_F64 for every float cube */
-  int32_t solver;          /* M3DRT_LINEAR2, M3DRT_QUADRATIC or M3DRT_LINEAR */
+  int32_t solver;          /* M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR or M3DRT_LONG_LINEAR */
   int32_t flags;           /* M3DRT_SPLIT_RATES | M3DRT_GET_RATES */
   double ds[3];            /* cell size in cm (self%ds * scaling%l) */
   double vec[3];           /* ray direction (rt_iter%vec(:,i_mu)) */
@@ -109,6 +109,8 @@ typedef struct {
  * angle integration of J and PsiJ.  Replaces, per direction:
  *   rt_patch_t%short_char2   SRC/rt_solvers_mod.f90:224-507 (nx*ny > 1)
  *   rt_patch_t%long_char2    SRC/rt_solvers_mod.f90:527-663 (nx = ny = 1)
+ *   and with desc->solver: short_char (M3DRT_LINEAR), long_char_quadratic
+ *   (M3DRT_QUADRATIC) and long_char_linear (M3DRT_LONG_LINEAR, :666-757) along columns
  *   rt_patch_t%integrate_intensity  SRC/rt_patch_mod.f90:3399-3431 (J, PsiJ)
  * over a whole column of slabs (rt_column_t%execute, SRC/rt_column_mod.f90:252-321
  * hands the exit plane downstream; here the sweep continues through it).
