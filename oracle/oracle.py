@@ -906,3 +906,52 @@ def rotate_array(x, vec, ds, bot2pivot, ray_offset=(0.0, 0.0)):
     d3 = lambda v: (C.c_double * len(v))(*[float(t) for t in v])
     L.orc_rotate_array(d3(vec), d3(ds), float(bot2pivot), d3(ray_offset), nf, nx, ny, nz, nk, x, out)
     return out
+
+
+# ------------------------------------------------------------- boundaries, background
+
+def lte_background(nu, temp):
+    """lte_background_opacities' (beta, Bnu), each (nf, ncell) float32."""
+    L = lib()
+    if not getattr(L, "_bg2_ok", False):
+        L.orc_lte_background.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_lte_background.restype = None
+        L._bg2_ok = True
+    nu = np.ascontiguousarray(nu, np.float64)
+    temp = np.ascontiguousarray(temp, np.float64)
+    beta = np.empty((nu.size, temp.size), np.float32)
+    B = np.empty_like(beta)
+    L.orc_lte_background(nu.size, temp.size, _p(nu), _p(temp), _p(beta), _p(B))
+    return beta, B
+
+
+def lte_ghost(nu, chi1, g500, chi500_1, T_ghost):
+    """The LTE ghost plane: (chi, S, I), each (nf, nxy) float32, from the first
+    plane's chi (nf, nxy) and the per-cell g500, chi500 of the first plane and the
+    ghost temperature (nxy)."""
+    L = lib()
+    if not getattr(L, "_lg_ok", False):
+        L.orc_lte_ghost.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 8
+        L.orc_lte_ghost.restype = None
+        L._lg_ok = True
+    nu = np.ascontiguousarray(nu, np.float64)
+    chi1 = np.ascontiguousarray(chi1, np.float32)
+    f = lambda a: np.ascontiguousarray(a, np.float32).ravel()
+    g, c1, T = f(g500), f(chi500_1), f(T_ghost)
+    out = [np.empty_like(chi1) for _ in range(3)]
+    L.orc_lte_ghost(nu.size, g.size, _p(nu), _p(chi1), _p(g), _p(c1), _p(T), *[_p(o) for o in out])
+    return tuple(out)
+
+
+def pchip_ghost(y1, y2, y3, ds3):
+    """set_upstream_boundary's PCHIP value one plane below the first three (before
+    the clamp at 0)."""
+    L = lib()
+    if not getattr(L, "_pg_ok", False):
+        L.orc_pchip_ghost.argtypes = [C.c_int, C.c_double] + [C.c_void_p] * 4
+        L.orc_pchip_ghost.restype = None
+        L._pg_ok = True
+    ys = [np.ascontiguousarray(y, np.float32).ravel() for y in (y1, y2, y3)]
+    out = np.empty_like(ys[0])
+    L.orc_pchip_ghost(out.size, float(ds3), *[_p(y) for y in ys], _p(out))
+    return out

@@ -23,30 +23,36 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 EXE = os.path.join(ROOT, "oracle", "_ref", "hop_golden.x")
+EXE_RT = os.path.join(ROOT, "oracle", "_ref", "rt_golden.x")
 
 
-def provenance():
+def provenance(rt=False):
     fc = subprocess.run(["amdflang", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    if rt:
+        return dict(compiler=fc, flags="-cpp -DHOP_STANDALONE -O2 -fPIC -fopenmp",
+                    sources="experiments/OpenMULTI3D/SRC/ray_mod.f90, solvers/rt/radau_mod.f90, "
+                            "experiments/OpenMULTI3D/HOP/{interpol_mod,optab_mod,...}.f90",
+                    harness="oracle/ref_harness/rt_golden.f90")
     return dict(compiler=fc, flags="-cpp -DHOP_STANDALONE -O2 -fPIC -fopenmp",
                 sources="experiments/OpenMULTI3D/HOP/{qlg_mod,op_math_mod,linelist_mod,...}.f90, exp_neg.inc",
                 harness="oracle/ref_harness/hop_golden.f90")
 
 
-def run(kind, payload: bytes) -> bytes:
+def run(kind, payload: bytes, exe=None) -> bytes:
     with tempfile.TemporaryDirectory() as d:
         fi, fo = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         open(fi, "wb").write(payload)
-        subprocess.run([EXE, kind, fi, fo], check=True)
+        subprocess.run([exe or EXE, kind, fi, fo], check=True)
         return open(fo, "rb").read()
 
 
 ONLY: set = set()
 
 
-def save(name, **arrays):
+def save(name, rt=False, **arrays):
     if ONLY and name not in ONLY:
         return
-    prov = provenance()
+    prov = provenance(rt)
     h = hashlib.sha256()
     for k in sorted(arrays):
         if k.startswith("in_"):
@@ -212,11 +218,116 @@ def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4, gene
          in_dep=dep, chi=chi_o, eta=eta_o, used=used, strength=strength, reach=reach, **extra)
 
 
+QUAD_DS = ((50e5, 50e5, 10e5), (20e5, 30e5, 10e5))
+
+
+def quad_requests():
+    """(scheme, ntheta, nphi, Jnu_weights, one_dimensional, ds, mu) of every set the
+    passes can ask setup_quadrature for."""
+    req = []
+    for jnu in (1, 0):
+        for n in range(1, 11):
+            for nphi in (1, 4):
+                req.append(("radau", n, nphi, jnu, 0, QUAD_DS[0], ()))
+        req.append(("radau", 5, 4, jnu, 1, QUAD_DS[0], ()))
+        for n in range(1, 7):
+            req.append(("lobatto", n, 4, jnu, 0, QUAD_DS[0], ()))
+            req.append(("gauss", n, 6, jnu, 0, QUAD_DS[0], ()))
+        req.append(("gauss", 3, 6, jnu, 1, QUAD_DS[0], ()))
+        for c in ("disk_center", "set_a2", "set_a4", "set_a6", "set_a8", "set_b4", "set_b6", "set_b8"):
+            req.append((c, 0, 0, jnu, 0, QUAD_DS[0], ()))
+    for mu in ((0.7,), (1.0, 0.5, 0.2), (0.9, 0.6, 0.3, 0.1), (0.2, 1.0, 0.55, 0.8, 0.35)):
+        req.append(("custom", len(mu), 4, 0, 0, QUAD_DS[0], mu))
+    for g in ("grid+", "gridx", "grid*"):
+        for ds in QUAD_DS:
+            req.append((g, 0, 4, 0, 0, ds, ()))
+    return req
+
+
+def gen_quad():
+    req = quad_requests()
+    payload = i32(len(req))
+    for (sch, nt, nphi, jnu, one_d, ds, mu) in req:
+        payload += sch.ljust(16).encode() + i32(nt, nphi, jnu, one_d) + np.array(ds, np.float64).tobytes()
+        payload += i32(len(mu)) + np.array(mu, np.float64).tobytes()
+    out = run("quad", payload, EXE_RT)
+    p, nray, vecs, wts = 0, [], [], []
+    for _ in req:
+        n = int(np.frombuffer(out[p:p + 4], np.int32)[0]); p += 4
+        v = np.frombuffer(out[p:p + 24 * n], np.float64).reshape(n, 3); p += 24 * n
+        w = np.frombuffer(out[p:p + 8 * n], np.float64); p += 8 * n
+        nray.append(n)
+        vecs.append(v)
+        wts.append(w)
+    mus = np.zeros((len(req), 5))
+    for i, r in enumerate(req):
+        mus[i, :len(r[6])] = r[6]
+    save("rt_quadrature.npz", rt=True, in_scheme=np.array([r[0] for r in req]),
+         in_int=np.array([r[1:5] for r in req], np.int32), in_ds=np.array([r[5] for r in req]),
+         in_nmu=np.array([len(r[6]) for r in req], np.int32), in_mu=mus, nray=np.array(nray, np.int32),
+         vec=np.concatenate(vecs), wts=np.concatenate(wts))
+
+
+def gen_pchip(rng):
+    n = 4000
+    y = (10 ** rng.uniform(-12, -2, (n, 3))).astype(np.float32)
+    y[:500] = np.sort(y[:500], axis=1)                  # rising with depth index
+    y[500:1000] = np.sort(y[500:1000], axis=1)[:, ::-1]  # falling
+    y[1000:1100, 1] = y[1000:1100, 0]                   # flat first interval
+    y[1100:1200, 2] = y[1100:1200, 1]                   # flat second interval
+    y[1200:1300] = y[1200:1300, :1]                     # constant
+    y[1300:1400] = 0.0
+    y[1400:1500] *= rng.choice([-1.0, 1.0], (100, 3)).astype(np.float32)   # negative (masers)
+    ds3 = 10e5
+    out = np.frombuffer(run("pchip", i32(n) + np.float64(ds3).tobytes() + y.tobytes(), EXE_RT), np.float32)
+    save("rt_pchip_ghost.npz", rt=True, in_y=y, in_ds3=np.float64(ds3), ynew=out)
+
+
+def gen_optab(rng):
+    nnu, nt, ne_ = 40, 12, 9
+    lam = np.sort(rng.uniform(3000.0, 9000.0, nnu))
+    ang = 0.3
+    rot = np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+    mu = np.array([8.6, 30.5])
+    c0 = np.array([-1.1, -3.2])
+    dc = np.array([0.2, 0.75])
+    ti = np.arange(nt)[None, :, None]
+    ei = np.arange(ne_)[None, None, :]
+    li = np.log(lam)[:, None, None]
+    val = (np.exp(0.3 * ti - 0.2 * ei + 0.5 * li - 4.0) * (1 + 0.3 * rng.random((nnu, nt, ne_))))
+    scat = 0.05 * val * (1 + rng.random((nnu, nt, ne_)))
+    val, scat = val.astype(np.float32), scat.astype(np.float32)
+    nc, nf = 600, 70
+    # query points in PC coordinates over the table and a little beyond, back to (T, ne)
+    p1 = c0[0] + rng.uniform(-0.3, nt - 0.7, nc) * dc[0]
+    p2 = c0[1] + rng.uniform(-0.3, ne_ - 0.7, nc) * dc[1]
+    d = rot @ np.vstack([p1, p2])
+    temp = np.exp(d[0] + mu[0])
+    ne = np.exp(d[1] + mu[1])
+    chi500 = (10 ** rng.uniform(-9, -5, nc)).astype(np.float32)
+    lam_q = np.concatenate([[lam[0] - 10.0, lam[0], lam[-1], lam[-1] + 5.0],
+                            np.sort(rng.uniform(lam[0], lam[-1], nf - 4))])
+    F = lambda a: np.asfortranarray(a).tobytes(order="F")
+    payload = (i32(nnu, nt, ne_) + mu.tobytes() + F(rot) + c0.tobytes() + dc.tobytes() + lam.tobytes()
+               + F(val) + F(scat) + i32(nc, nf) + temp.tobytes() + ne.tobytes() + chi500.tobytes()
+               + lam_q.tobytes())
+    out = np.frombuffer(run("optab", payload, EXE_RT), np.float32)
+    n3 = nnu * nt * ne_
+    sl = [out[i * n3:(i + 1) * n3].reshape(ne_, nt, nnu).transpose(2, 1, 0) for i in range(6)]
+    o = 6 * n3
+    back = out[o:o + nf * nc].reshape(nc, nf)
+    sc = out[o + nf * nc:o + 2 * nf * nc].reshape(nc, nf)
+    save("hop_optab_sample.npz", rt=True, in_lam=lam, in_val=val, in_scat=scat, in_mu=mu, in_rot=rot, in_c0=c0,
+         in_dc=dc, in_temp=temp, in_ne=ne, in_chi500=chi500, in_lam_q=lam_q, dl=sl[0], dt=sl[1], de=sl[2],
+         sl=sl[3], st=sl[4], se=sl[5], back=back, scat=sc)
+
+
 def main():
     """python make_golden.py [name ...]: every fixture, or only the named ones (the
     seeded draws are the same either way: each fixture has its own generator)."""
-    if not os.path.exists(EXE):
-        sys.exit(f"{EXE} missing: run oracle/build_ref.sh first")
+    for exe in (EXE, EXE_RT):
+        if not os.path.exists(exe):
+            sys.exit(f"{exe} missing: run oracle/build_ref.sh first")
     only = set(sys.argv[1:])
     ONLY.update(only)
     rng = np.random.default_rng(20261202)
@@ -231,6 +342,9 @@ def main():
                                     ("hop_opac_gen_scat.npz", dict(flags=4, nl=30, nf=200, m=(3, 2, 2)))]):
         if not only or name in only:
             gen_opac(np.random.default_rng(20261210 + i), name, general=True, **kw)
+    gen_quad()
+    gen_pchip(np.random.default_rng(20261220))
+    gen_optab(np.random.default_rng(20261221))
 
 
 if __name__ == "__main__":
