@@ -22,12 +22,26 @@ int hip_check(hipError_t e, const char *what) {
 }
 
 int ensure_scratch(m3drt_ctx *ctx, size_t bytes) {
-  if (bytes <= ctx->scratch_bytes) return M3DRT_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  M3DRT_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  if (bytes <= ctx->scratch_bytes) {
+    if (capturing) ctx->scratch_captured = true;
+    return M3DRT_OK;
+  }
+  if (capturing)
+    return set_error(M3DRT_ELIMIT, "the scratch buffer would grow inside a stream capture: run the call once "
+                                   "before capturing it");
   if (ctx->scratch) {
-    M3DRT_HIP(hipStreamSynchronize(ctx->stream));
-    M3DRT_HIP(hipFree(ctx->scratch));
+    if (ctx->scratch_captured) {
+      ctx->retired.push_back(ctx->scratch);  // captured graphs still replay into it
+    } else {
+      M3DRT_HIP(hipStreamSynchronize(ctx->stream));
+      M3DRT_HIP(hipFree(ctx->scratch));
+    }
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
+    ctx->scratch_captured = false;
   }
   const size_t rounded = (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
   M3DRT_HIP(hipMalloc(&ctx->scratch, rounded));
@@ -72,8 +86,9 @@ int m3drt_create(m3drt_ctx **ctx, int device, int flags) {
 int m3drt_destroy(m3drt_ctx *ctx) {
   if (!ctx) return M3DRT_OK;
   (void)hipSetDevice(ctx->device);
-  if (ctx->scratch || ctx->qlg_tab) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch || ctx->qlg_tab || !ctx->retired.empty()) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  for (void *p : ctx->retired) (void)hipFree(p);
   if (ctx->qlg_tab) (void)hipFree(ctx->qlg_tab);
   delete ctx;
   return M3DRT_OK;
@@ -150,45 +165,58 @@ int m3drt_reference_strides(m3drt_sweep_desc *d) {
   return M3DRT_OK;
 }
 
-int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
-  if (!ctx || !d || !b) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: NULL argument");
+}  // extern "C"
+
+namespace m3drt {
+
+int sc_sweep_checked(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const char *who) {
+  const std::string w(who);
+  if (!ctx || !d || !b) return set_error(M3DRT_EINVAL, w + ": NULL argument");
   if (d->nf < 1 || d->nx < 1 || d->ny < 1 || d->nz < 1)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: empty grid");
-  if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must be +1 or -1");
-  if (d->prec != M3DRT_F32 && d->prec != M3DRT_F64) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: bad prec");
+    return set_error(M3DRT_EINVAL, w + ": empty grid");
+  if (d->dir != 1 && d->dir != -1) return set_error(M3DRT_EINVAL, w + ": dir must be +1 or -1");
+  if (d->prec != M3DRT_F32 && d->prec != M3DRT_F64) return set_error(M3DRT_EINVAL, w + ": bad prec");
   if (d->solver != M3DRT_LINEAR2 && d->solver != M3DRT_QUADRATIC && d->solver != M3DRT_LINEAR &&
       d->solver != M3DRT_LONG_LINEAR)
     return set_error(M3DRT_EINVAL,
-                     "m3drt_sc_sweep: solver must be M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR or M3DRT_LONG_LINEAR");
-  if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0)) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: ds must be > 0");
-  if (d->vec[2] == 0.0) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: horizontal ray (vec(3) = 0)");
+                     w + ": solver must be M3DRT_LINEAR2, M3DRT_QUADRATIC, M3DRT_LINEAR or M3DRT_LONG_LINEAR");
+  if (!(d->ds[0] > 0 && d->ds[1] > 0 && d->ds[2] > 0)) return set_error(M3DRT_EINVAL, w + ": ds must be > 0");
+  if (d->vec[2] == 0.0) return set_error(M3DRT_EINVAL, w + ": horizontal ray (vec(3) = 0)");
   if ((d->vec[2] > 0) != (d->dir > 0))
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: dir must follow the sign of vec(3)");
+    return set_error(M3DRT_EINVAL, w + ": dir must follow the sign of vec(3)");
   if (!b->chi || !b->S || !b->ghost_chi || !b->ghost_S || !b->ghost_I)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing input array");
+    return set_error(M3DRT_EINVAL, w + ": missing input array");
   if (d->solver != M3DRT_LINEAR2) {
     if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
-      return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: strides must be >= 1 (see m3drt_native_strides)");
+      return set_error(M3DRT_EINVAL, w + ": strides must be >= 1 (see m3drt_native_strides)");
     if ((d->flags & M3DRT_SPLIT_RATES) && (!b->Irem_out || !b->escape_out))
-      return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
+      return set_error(M3DRT_EINVAL, w + ": M3DRT_SPLIT_RATES needs Irem_out and escape_out");
     M3DRT_HIP(hipSetDevice(ctx->device));
-    if (d->solver == M3DRT_QUADRATIC) return m3drt::sweep_lcq(ctx, d, b);
-    if (d->solver == M3DRT_LONG_LINEAR) return m3drt::sweep_lcl(ctx, d, b);
-    return m3drt::sweep_sc1(ctx, d, b);
+    if (d->solver == M3DRT_QUADRATIC) return sweep_lcq(ctx, d, b);
+    if (d->solver == M3DRT_LONG_LINEAR) return sweep_lcl(ctx, d, b);
+    return sweep_sc1(ctx, d, b);
   }
-  if (!b->chi500 || !b->g500) return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: missing chi500 / g500");
+  if (!b->chi500 || !b->g500) return set_error(M3DRT_EINVAL, w + ": missing chi500 / g500");
   if (d->nx * (int64_t)d->ny > 1 && !b->chi500_up)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: short_char2: chi500 at the upwind points is missing");
+    return set_error(M3DRT_EINVAL, w + ": short_char2: chi500 at the upwind points is missing");
   if (b->eta_row && (!b->eta_mask || d->eta_ld < d->nf))
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: eta_row without eta_mask / eta_ld < nf");
+    return set_error(M3DRT_EINVAL, w + ": eta_row without eta_mask / eta_ld < nf");
   if ((d->flags & M3DRT_SPLIT_RATES) && (!b->Irem_out || !b->escape_out))
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: M3DRT_SPLIT_RATES needs Irem_out and escape_out");
+    return set_error(M3DRT_EINVAL, w + ": M3DRT_SPLIT_RATES needs Irem_out and escape_out");
   if (d->cs_f < 1 || d->cs_x < 1 || d->cs_y < 1 || d->cs_z < 1 || d->gs_f < 1 || d->gs_x < 1 || d->gs_y < 1)
-    return set_error(M3DRT_EINVAL, "m3drt_sc_sweep: strides must be >= 1 (see m3drt_native_strides)");
+    return set_error(M3DRT_EINVAL, w + ": strides must be >= 1 (see m3drt_native_strides)");
   // every in-plane offset of a cube is formed in 64 bits; the planes themselves must not overlap
   M3DRT_HIP(hipSetDevice(ctx->device));
-  if (d->nx == 1 && d->ny == 1) return m3drt::sweep_lc2(ctx, d, b);
-  return m3drt::sweep_sc2(ctx, d, b);
+  if (d->nx == 1 && d->ny == 1) return sweep_lc2(ctx, d, b);
+  return sweep_sc2(ctx, d, b);
+}
+
+}  // namespace m3drt
+
+extern "C" {
+
+int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b) {
+  return m3drt::sc_sweep_checked(ctx, d, b, "m3drt_sc_sweep");
 }
 
 int m3drt_sc_sweep_hermite(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b,
@@ -555,4 +583,17 @@ int m3drt_b1d_plan_destroy(m3drt_b1d_plan *plan) {
   return m3drt::b1d_plan_destroy(plan);
 }
 
+int m3drt_linear_weights(m3drt_ctx *ctx, int64_t n, const float *t, float *E, float *W, float *Psi, float *Esc) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_linear_weights: ctx is NULL");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::step_weights(ctx, 0, n, t, E, W, Psi, Esc);
+}
+
+int m3drt_parab_moments(m3drt_ctx *ctx, int64_t n, const float *t, float *E, float *U0, float *U1, float *q) {
+  if (!ctx) return set_error(M3DRT_EINVAL, "m3drt_parab_moments: ctx is NULL");
+  M3DRT_HIP(hipSetDevice(ctx->device));
+  return m3drt::step_weights(ctx, 1, n, t, E, U0, U1, q);
+}
+
 }  // extern "C"
+

@@ -4,13 +4,16 @@
 #include <cstdint>
 #include <cstddef>
 #include <string>
+#include <vector>
 #include "m3drt.h"
 
 struct m3drt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  void *scratch = nullptr;   // per-sweep scratch (dt500, rc per cell); grown on demand
+  void *scratch = nullptr;   // per-call scratch (dt500, rc per cell, ...); grown on demand
   size_t scratch_bytes = 0;
+  bool scratch_captured = false;  // a stream capture recorded launches that use `scratch`
+  std::vector<void *> retired;    // outgrown scratch buffers captured graphs may still use
   int lds_max = 0;           // dynamic LDS bytes available per workgroup
   float *qlg_tab = nullptr;  // QLG-bot's two tables on the device (2 x 9020 floats)
 };
@@ -20,7 +23,13 @@ namespace m3drt {
 // error state of the calling thread (m3drt_last_error)
 int set_error(int code, const std::string &msg);
 int hip_check(hipError_t e, const char *what);
+// Makes ctx->scratch hold at least `bytes`.  Launches recorded by a stream capture
+// keep the scratch address, so a buffer that has been captured is never freed when
+// it is outgrown (it is retired and freed by m3drt_destroy), and growth inside a
+// capture is refused.
 int ensure_scratch(m3drt_ctx *ctx, size_t bytes);
+// the sweep entry point's checks and dispatch
+int sc_sweep_checked(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const char *who);
 
 #define M3DRT_HIP(call)                                         \
   do {                                                          \
@@ -56,6 +65,7 @@ int bb_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r);
 int window_opacity(m3drt_ctx *ctx, const m3drt_bb_opacity_args *r, const m3drt_window_args *w);
 int rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *r);
 int bf_rates(m3drt_ctx *ctx, const m3drt_bf_rates_args *r);
+int step_weights(m3drt_ctx *ctx, int kind, int64_t n, const float *t, float *o1, float *o2, float *o3, float *o4);
 
 }  // namespace m3drt
 
@@ -84,46 +94,61 @@ __device__ __forceinline__ cptr<T> ro(const T *p) {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 
+// E = exp(-t) and u0 = 1 - exp(-t) in FP32, neither formed from the other (the
+// quantities of exp_neg, HOP/exp_neg.inc:20-37), with t clamped at 87 as there.
+// Cody-Waite reduction t = n ln2 + r, |r| <= ln2/2, ln2 split so that n*LN2_HI is
+// exact for |n| < 2^15; m = 1 - e^-r = r*M(r) by the Taylor series of
+// (1 - e^-r)/r to r^7 (truncation below 1e-9 relative on |r| <= ln2/2); s = 2^-n by
+// ldexp.  Then E = s*(1 - m) and u0 = (1 - s) + s*m: within 1.2 and 1.6 ulp of the
+// exact values over -2 <= t <= 87, where a single hardware exponential of
+// -t*log2 e would carry the rounding of that product, |t| ulp and more.
+__device__ __forceinline__ void exp_neg(float t, float &E, float &u0) {
+  const float tc = fminf(t, 87.0f);
+  const float n = __builtin_rintf(tc * 1.44269504088896341f);
+  float r = __builtin_fmaf(-n, 0.693359375f, tc);
+  r = __builtin_fmaf(-n, -2.12194440e-4f, r);
+  const float M = 1.0f + r * (-0.5f + r * (1.66666672e-01f + r * (-4.16666679e-02f + r * (8.33333377e-03f +
+                  r * (-1.38888892e-03f + r * (1.98412701e-04f + r * -2.48015876e-05f))))));
+  const float m = r * M;
+  const float s = __builtin_ldexpf(1.0f, -(int)n);
+  E = s * (1.0f - m);
+  u0 = __builtin_fmaf(s, m, 1.0f - s);
+}
+
 // Weights of the linear step over optical depth t (the quantities of
 // linear_weights, SRC/rt_solvers_mod.f90:785-797): E = exp(-t), Psi = 1 - u0/t,
 // W = u0 - Psi, Esc = 1 - Psi with u0 = 1 - E.
 //
-// FP32 form for gfx950: E from the hardware exponential, E = 2^(-t*log2 e) with t
-// clamped at 87 as the reference clamps it.  For |t| <= 1, where 1 - E and 1 - u0/t
-// cancel, Psi = t*P(t) with the reference's degree-7 minimax P (lw_series,
-// SRC/rt_solvers_mod.f90:803-809) and u0 = t*(1 - Psi), an identity of the exact
-// weights that involves no cancellation; above, u0 = 1 - E and u0/t by the
-// hardware reciprocal.  Both forms are evaluated and one is selected, so the
-// wavefront never diverges.
+// FP32 form for gfx950: E and u0 from exp_neg above.  For |t| <= 1, where 1 - u0/t
+// cancels, Psi = t*P(t) with the reference's degree-7 minimax P (lw_series,
+// SRC/rt_solvers_mod.f90:803-809) and W = u0 - Psi; above, u0/t by the hardware
+// reciprocal, Psi = 1 - u0/t, W = u0/t - E.  Both forms are evaluated and one is
+// selected, so the wavefront never diverges.  Every output is within the worst
+// error of the reference's own FP32 arithmetic over -2 <= t <= 90 (E to 87), plus
+// at most one ulp.
 __device__ __forceinline__ void lin_weights(float t, float &E, float &W, float &Psi, float &Esc) {
   const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
                    t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
                    t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
-  const float tc = fminf(t, 87.0f);
-  E = __builtin_amdgcn_exp2f(tc * -1.44269504088896341f);
-  const float esc_thin = 1.0f - ps;
-  const float u0_thin = t * esc_thin;
-  const float u0 = 1.0f - E;
+  float u0;
+  exp_neg(t, E, u0);
   const float ut = u0 * rcp(t);
   const bool thin = fabsf(t) <= 1.0f;
   Psi = thin ? ps : 1.0f - ut;
-  Esc = thin ? esc_thin : ut;
-  W = thin ? u0_thin - ps : ut - E;
+  Esc = thin ? 1.0f - ps : ut;
+  W = thin ? u0 - ps : ut - E;
 }
 
-// FP64 form: exact weights, the plain FP64 reference of the sweep.
 // parab_moments (SRC/rt_solvers_mod.f90:862-877): E = e^-t, U0 = 1 - E, U1 = t - U0
 // (t * Psi(t) below |t| = 1, Psi the degree-7 series of linear_weights), q = U2/t
-// (degree-8 minimax below |t| = 2).  E from the hardware exponential, as the linear
-// weights.
+// (degree-8 minimax below |t| = 2).  E and U0 from exp_neg, as the linear weights.
 __device__ __forceinline__ void parab_moments(float t, float &E, float &U0, float &U1, float &q) {
 #pragma clang fp contract(off)
   const float ps = t * (5.000000000e-01f + t * (-1.666666716e-01f + t * (4.166666791e-02f +
                    t * (-8.333330974e-03f + t * (1.388740959e-03f + t * (-1.983609400e-04f +
                    t * (2.522676368e-05f + t * (-2.834696488e-06f))))))));
-  E = __builtin_amdgcn_exp2f(fminf(t, 87.0f) * -1.44269504088896341f);
+  exp_neg(t, E, U0);
   const bool thin = fabsf(t) <= 1.0f;
-  U0 = thin ? t * (1.0f - ps) : 1.0f - E;
   U1 = thin ? t * ps : t - U0;
   const float qs = t * t * (3.333333433e-01f + t * (-8.333329856e-02f + t * (1.666668802e-02f +
                    t * (-2.777901944e-03f + t * (3.968053206e-04f + t * (-4.948913192e-05f +
@@ -131,6 +156,7 @@ __device__ __forceinline__ void parab_moments(float t, float &E, float &U0, floa
   q = fabsf(t) <= 2.0f ? qs : t - 2.0f * (U1 / t);
 }
 
+// FP64 form: exact weights, the plain FP64 reference of the sweep.
 __device__ __forceinline__ void lin_weights(double t, double &E, double &W, double &Psi, double &Esc) {
   E = exp(-t);
   const double u0 = -expm1(-t);

@@ -64,11 +64,11 @@ __device__ __forceinline__ float b3_p3(float t) {
 // bezier3_weights (SRC/rt_solvers_mod.f90:1301-1323): w_k = t int_0^1 b_k(u) e^-tu du
 // for the Bernstein basis u^3, 3u^2(1-u), 3u(1-u)^2, (1-u)^3 (S_up, the two control
 // points, S_target).  Below t = 3 the series (w0, w1 by the reflection u -> 1-u);
-// above, the moments M_n = n M_(n-1) - t^n E from M0 = 1 - E.  E from the hardware
-// exponential.
+// above, the moments M_n = n M_(n-1) - t^n E from M0 = 1 - E.  E and M0 from
+// dev::exp_neg.
 __device__ __forceinline__ void bezier3(float t, float &E, float &w0, float &w1, float &w2, float &w3) {
-  E = __builtin_amdgcn_exp2f(fminf(t, 87.0f) * -1.44269504088896341f);
-  const float M0 = 1.0f - E;
+  float M0;
+  dev::exp_neg(t, E, M0);
   const float tb = fmaxf(t, 3.0f), tb2 = tb * tb, tb3 = tb2 * tb, r3 = 1.0f / tb3;
   const float M1 = M0 - tb * E;
   const float M2 = 2.0f * M1 - tb2 * E;

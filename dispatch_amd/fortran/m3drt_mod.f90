@@ -32,6 +32,8 @@ module m3drt
   integer(c_int), parameter, public :: M3DRT_BG_SCAT_ALI = 1, M3DRT_BG_J_TO_B = 2
   integer(c_int), parameter, public :: M3DRT_ETA_GHOST = 1
   integer(c_int), parameter, public :: M3DRT_LCH_TOP = 0, M3DRT_LCH_BOTTOM = 1, M3DRT_LCH_UPSTREAM = 2
+  integer(c_int), parameter, public :: M3DRT_GHOST_TOP = 0, M3DRT_GHOST_LTE = 1, M3DRT_GHOST_PCHIP = 2, &
+                                       M3DRT_GHOST_UPSTREAM = 3
 
   type, bind(C), public :: m3drt_sweep_desc
     integer(c_int32_t) :: nf, nx, ny, nz
@@ -414,6 +416,121 @@ module m3drt
     integer(c_int32_t) :: threads
   end type m3drt_b1d_tile
 
+  type, bind(C), public :: m3drt_background_lte_args
+    integer(c_int32_t) :: nf
+    integer(c_int64_t) :: ncell
+    type(c_ptr) :: nu = c_null_ptr
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: Bnu = c_null_ptr
+    integer(c_int64_t) :: sf = 0
+    integer(c_int64_t) :: sc = 0
+  end type m3drt_background_lte_args
+
+  type, bind(C), public :: m3drt_boundary_args
+    integer(c_int32_t) :: kind
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: nx
+    integer(c_int32_t) :: ny
+    integer(c_int64_t) :: ps_f = 0
+    integer(c_int64_t) :: ps_x = 0
+    integer(c_int64_t) :: ps_y = 0
+    type(c_ptr) :: chi1 = c_null_ptr
+    type(c_ptr) :: S1 = c_null_ptr
+    type(c_ptr) :: chi2 = c_null_ptr
+    type(c_ptr) :: S2 = c_null_ptr
+    type(c_ptr) :: g500 = c_null_ptr
+    type(c_ptr) :: chi500_1 = c_null_ptr
+    type(c_ptr) :: T_ghost = c_null_ptr
+    type(c_ptr) :: nu = c_null_ptr
+    real(c_double) :: ds3 = 0
+    type(c_ptr) :: exit_I = c_null_ptr
+    type(c_ptr) :: mem_chi = c_null_ptr
+    type(c_ptr) :: mem_S = c_null_ptr
+    type(c_ptr) :: mem_I = c_null_ptr
+    type(c_ptr) :: ghost_chi = c_null_ptr
+    type(c_ptr) :: ghost_S = c_null_ptr
+    type(c_ptr) :: ghost_I = c_null_ptr
+  end type m3drt_boundary_args
+
+  type, bind(C), public :: m3drt_quadrature_args
+    type(c_ptr) :: scheme = c_null_ptr
+    integer(c_int32_t) :: ntheta
+    integer(c_int32_t) :: nphi
+    integer(c_int32_t) :: jnu_weights
+    integer(c_int32_t) :: one_dimensional
+    type(c_ptr) :: mu = c_null_ptr
+    real(c_double) :: ds(3)
+  end type m3drt_quadrature_args
+
+  type, bind(C), public :: m3drt_optab_table
+    integer(c_int32_t) :: nnu
+    integer(c_int32_t) :: ntemp
+    integer(c_int32_t) :: nne
+    real(c_double) :: mu(2)
+    real(c_double) :: rot(4)
+    real(c_double) :: c0(2)
+    real(c_double) :: dc(2)
+    type(c_ptr) :: lam = c_null_ptr
+    type(c_ptr) :: val = c_null_ptr
+    type(c_ptr) :: scat = c_null_ptr
+  end type m3drt_optab_table
+
+  type, bind(C), public :: m3drt_optab_query
+    integer(c_int64_t) :: ncell
+    integer(c_int32_t) :: nf
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: ne = c_null_ptr
+    type(c_ptr) :: chi500 = c_null_ptr
+    type(c_ptr) :: lam_q = c_null_ptr
+    type(c_ptr) :: back = c_null_ptr
+    type(c_ptr) :: scat = c_null_ptr
+    integer(c_int64_t) :: sf = 0
+    integer(c_int64_t) :: sc = 0
+  end type m3drt_optab_query
+
+  type, bind(C), public :: m3drt_isotope
+    type(m3drt_linelist) :: lines
+    real(c_double) :: awgt
+    real(c_double) :: frac
+  end type m3drt_isotope
+
+  type, bind(C), public :: m3drt_species
+    integer(c_int32_t) :: niso
+    type(c_ptr) :: iso = c_null_ptr
+    type(c_ptr) :: n = c_null_ptr
+    real(c_double) :: scale = 1
+    type(c_ptr) :: part = c_null_ptr
+    type(c_ptr) :: dep = c_null_ptr
+    integer(c_int32_t) :: nlev = 0
+    integer(c_int32_t) :: scat_group = 0
+  end type m3drt_species
+
+  type, bind(C), public :: m3drt_linelist_opacity_args
+    integer(c_int32_t) :: nf
+    integer(c_int32_t) :: nsp
+    integer(c_int32_t) :: equidistant
+    integer(c_int64_t) :: ncell
+    type(c_ptr) :: llam = c_null_ptr
+    type(c_ptr) :: sp = c_null_ptr
+    type(c_ptr) :: temp = c_null_ptr
+    type(c_ptr) :: vmic = c_null_ptr
+    type(c_ptr) :: vproj = c_null_ptr
+    type(c_ptr) :: e_kT = c_null_ptr
+    type(c_ptr) :: ne = c_null_ptr
+    type(c_ptr) :: tothi = c_null_ptr
+    type(c_ptr) :: stark_pe = c_null_ptr
+    type(c_ptr) :: all_chi = c_null_ptr
+    type(c_ptr) :: all_eta = c_null_ptr
+    type(c_ptr) :: one_m_eps = c_null_ptr
+    type(c_ptr) :: chi_scat = c_null_ptr
+    type(c_ptr) :: beta = c_null_ptr
+    type(c_ptr) :: chi_lines = c_null_ptr
+    type(c_ptr) :: eta_lines = c_null_ptr
+    real(c_float) :: tolerance = 1e-4
+    real(c_float) :: line_reject_frac = 0
+  end type m3drt_linelist_opacity_args
+
   public :: m3drt_create, m3drt_destroy, m3drt_synchronize, m3drt_last_error
   public :: m3drt_device_alloc, m3drt_device_free, m3drt_copy_to_device, m3drt_copy_to_host
   public :: m3drt_sc_sweep, m3drt_reference_strides, m3drt_native_strides
@@ -425,6 +542,10 @@ module m3drt
   public :: m3drt_normalize_rates, m3drt_rate_solve_f64
   public :: m3drt_b1d_synth_plain, m3drt_b1d_plan_create, m3drt_b1d_plan_run, m3drt_b1d_plan_info
   public :: m3drt_b1d_plan_destroy
+  public :: m3drt_lte_background, m3drt_set_upstream_boundary, m3drt_set_downstream_boundary
+  public :: m3drt_quadrature, m3drt_linear_weights, m3drt_parab_moments
+  public :: m3drt_optab_plan_create, m3drt_optab_slopes, m3drt_optab_interpolate, m3drt_optab_plan_destroy
+  public :: m3drt_add_linelist_opacities
   public :: short_char2_gpu, m3drt_error_message
 
   interface
@@ -572,6 +693,66 @@ module m3drt
       type(m3drt_rate_solve_args), intent(in) :: args
       type(m3drt_rate_solve_result), intent(inout) :: res
     end function m3drt_rate_solve_f64
+    integer(c_int) function m3drt_lte_background(ctx, args) bind(C, name='m3drt_lte_background')
+      import :: c_int, c_ptr, m3drt_background_lte_args
+      type(c_ptr), value :: ctx
+      type(m3drt_background_lte_args), intent(in) :: args
+    end function m3drt_lte_background
+    integer(c_int) function m3drt_set_upstream_boundary(ctx, args) bind(C, name='m3drt_set_upstream_boundary')
+      import :: c_int, c_ptr, m3drt_boundary_args
+      type(c_ptr), value :: ctx
+      type(m3drt_boundary_args), intent(in) :: args
+    end function m3drt_set_upstream_boundary
+    integer(c_int) function m3drt_set_downstream_boundary(ctx, args) bind(C, name='m3drt_set_downstream_boundary')
+      import :: c_int, c_ptr, m3drt_boundary_args
+      type(c_ptr), value :: ctx
+      type(m3drt_boundary_args), intent(in) :: args
+    end function m3drt_set_downstream_boundary
+    ! vec(3, cap) and wts(cap) are host arrays; scheme a NUL-terminated string
+    integer(c_int) function m3drt_quadrature(q, cap, vec, wts, nray) bind(C, name='m3drt_quadrature')
+      import :: c_int, c_int32_t, c_double, m3drt_quadrature_args
+      type(m3drt_quadrature_args), intent(in) :: q
+      integer(c_int32_t), value :: cap
+      real(c_double) :: vec(*), wts(*)
+      integer(c_int32_t), intent(out) :: nray
+    end function m3drt_quadrature
+    integer(c_int) function m3drt_linear_weights(ctx, n, t, E, W, Psi, Esc) bind(C, name='m3drt_linear_weights')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: t, E, W, Psi, Esc
+    end function m3drt_linear_weights
+    integer(c_int) function m3drt_parab_moments(ctx, n, t, E, U0, U1, q) bind(C, name='m3drt_parab_moments')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: t, E, U0, U1, q
+    end function m3drt_parab_moments
+    integer(c_int) function m3drt_optab_plan_create(ctx, table, plan) bind(C, name='m3drt_optab_plan_create')
+      import :: c_int, c_ptr, m3drt_optab_table
+      type(c_ptr), value :: ctx
+      type(m3drt_optab_table), intent(in) :: table
+      type(c_ptr), intent(out) :: plan
+    end function m3drt_optab_plan_create
+    integer(c_int) function m3drt_optab_slopes(plan, dl, dt, de, sl, st, se) bind(C, name='m3drt_optab_slopes')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, dl, dt, de, sl, st, se
+    end function m3drt_optab_slopes
+    integer(c_int) function m3drt_optab_interpolate(plan, query) bind(C, name='m3drt_optab_interpolate')
+      import :: c_int, c_ptr, m3drt_optab_query
+      type(c_ptr), value :: plan
+      type(m3drt_optab_query), intent(in) :: query
+    end function m3drt_optab_interpolate
+    integer(c_int) function m3drt_optab_plan_destroy(plan) bind(C, name='m3drt_optab_plan_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan
+    end function m3drt_optab_plan_destroy
+    ! args%sp points to a host array of m3drt_species, each %iso to a host array of m3drt_isotope
+    integer(c_int) function m3drt_add_linelist_opacities(ctx, args) bind(C, name='m3drt_add_linelist_opacities')
+      import :: c_int, c_ptr, m3drt_linelist_opacity_args
+      type(c_ptr), value :: ctx
+      type(m3drt_linelist_opacity_args), intent(in) :: args
+    end function m3drt_add_linelist_opacities
     integer(c_int) function m3drt_rotate_array(ctx, args) bind(C, name='m3drt_rotate_array')
       import :: c_int, c_ptr, m3drt_rotate_args
       type(c_ptr), value :: ctx

@@ -990,3 +990,278 @@ def _rotate_array(self, x, out, vec, ds, bot2pivot, ray_offset=(0.0, 0.0), layou
 
 Context.rotate_array = _rotate_array
 
+
+
+def _weights(self, t, which):
+    L = lib()
+    if not getattr(L, "_w_ok", False):
+        for n in ("m3drt_linear_weights", "m3drt_parab_moments"):
+            getattr(L, n).argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 5
+            getattr(L, n).restype = C.c_int
+        L._w_ok = True
+    if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+        raise M3drtError("weights: t is a contiguous float32 device tensor")
+    out = [torch.empty_like(t) for _ in range(4)]
+    self._bind_stream()
+    name = "m3drt_linear_weights" if which == "linear" else "m3drt_parab_moments"
+    _check(getattr(L, name)(self._h, t.numel(), _p(t), *[_p(o) for o in out]), name)
+    return tuple(out)
+
+
+def _linear_weights(self, t):
+    """linear_weights of the kernels elementwise: (E, W, Psi, Esc)."""
+    return _weights(self, t, "linear")
+
+
+def _parab_moments(self, t):
+    """parab_moments of the kernels elementwise: (E, U0, U1, q)."""
+    return _weights(self, t, "parab")
+
+
+Context.linear_weights = _linear_weights
+Context.parab_moments = _parab_moments
+
+
+class QuadratureArgs(C.Structure):
+    _fields_ = [("scheme", C.c_char_p), ("ntheta", C.c_int32), ("nphi", C.c_int32), ("jnu_weights", C.c_int32),
+                ("one_dimensional", C.c_int32), ("mu", C.c_void_p), ("ds", C.c_double * 3)]
+
+
+def quadrature(scheme, ntheta=0, nphi=1, jnu_weights=True, one_dimensional=False, mu=None, ds=(1.0, 1.0, 1.0)):
+    """setup_quadrature's ray set: (vec (n, 3), wts (n)) as float64 numpy arrays."""
+    L = lib()
+    if not getattr(L, "_quad_ok", False):
+        L.m3drt_quadrature.argtypes = [C.POINTER(QuadratureArgs), C.c_int32, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_int32)]
+        L.m3drt_quadrature.restype = C.c_int
+        L._quad_ok = True
+    m = None if mu is None else np.ascontiguousarray(mu, np.float64)
+    if m is not None:
+        ntheta = m.size
+    q = QuadratureArgs(scheme.encode(), int(ntheta), int(nphi), int(bool(jnu_weights)), int(bool(one_dimensional)),
+                       None if m is None else m.ctypes.data_as(C.c_void_p), (C.c_double * 3)(*ds))
+    n = C.c_int32(0)
+    st = L.m3drt_quadrature(C.byref(q), 0, None, None, C.byref(n))
+    if st not in (M3DRT_OK, 4):
+        _check(st, "m3drt_quadrature")
+    vec = np.zeros((n.value, 3))
+    w = np.zeros(n.value)
+    _check(L.m3drt_quadrature(C.byref(q), n.value, vec.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                              C.byref(n)), "m3drt_quadrature")
+    return vec, w
+
+
+class BackgroundLteArgs(C.Structure):
+    _fields_ = [("nf", C.c_int32), ("ncell", C.c_int64), ("nu", C.c_void_p), ("temp", C.c_void_p),
+                ("beta", C.c_void_p), ("Bnu", C.c_void_p), ("sf", C.c_int64), ("sc", C.c_int64)]
+
+
+class BoundaryArgs(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("kind", "nf", "nx", "ny")]
+                + [(n, C.c_int64) for n in ("ps_f", "ps_x", "ps_y")]
+                + [(n, C.c_void_p) for n in ("chi1", "S1", "chi2", "S2", "g500", "chi500_1", "T_ghost", "nu")]
+                + [("ds3", C.c_double)]
+                + [(n, C.c_void_p) for n in ("exit_I", "mem_chi", "mem_S", "mem_I", "ghost_chi", "ghost_S",
+                                              "ghost_I")])
+
+
+M3DRT_GHOST = {"top": 0, "lte": 1, "pchip": 2, "upstream": 3}
+
+
+def _bnd_lib():
+    L = lib()
+    if not getattr(L, "_bnd_ok", False):
+        L.m3drt_lte_background.argtypes = [C.c_void_p, C.POINTER(BackgroundLteArgs)]
+        L.m3drt_lte_background.restype = C.c_int
+        for n in ("m3drt_set_upstream_boundary", "m3drt_set_downstream_boundary"):
+            getattr(L, n).argtypes = [C.c_void_p, C.POINTER(BoundaryArgs)]
+            getattr(L, n).restype = C.c_int
+        L._bnd_ok = True
+    return L
+
+
+def _lte_background(self, nu, temp, beta, Bnu):
+    """beta and Bnu (nf, ncell) float32 device tensors from nu (nf) and temp (ncell)
+    float64 device tensors (lte_background_opacities' factors)."""
+    L = _bnd_lib()
+    for t, dt in ((nu, torch.float64), (temp, torch.float64), (beta, torch.float32), (Bnu, torch.float32)):
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+            raise M3drtError("lte_background: nu, temp float64, beta, Bnu float32 contiguous device tensors")
+    nf, nc = nu.numel(), temp.numel()
+    if beta.numel() != nf * nc or Bnu.numel() != nf * nc:
+        raise M3drtError("lte_background: beta and Bnu hold nf * ncell values")
+    a = BackgroundLteArgs(nf, nc, _p(nu), _p(temp), _p(beta), _p(Bnu), nc, 1)
+    self._bind_stream()
+    _check(L.m3drt_lte_background(self._h, C.byref(a)), "m3drt_lte_background")
+
+
+def _boundary_args(kind, ghost, nf, ny, nx, **kw):
+    a = BoundaryArgs()
+    a.kind = M3DRT_GHOST[kind] if isinstance(kind, str) else int(kind)
+    a.nf, a.nx, a.ny = nf, nx, ny
+    a.ps_f, a.ps_x, a.ps_y = nx * ny, 1, nx
+    a.ds3 = float(kw.pop("ds3", 0.0))
+    for k, t in kw.items():
+        if t is not None:
+            if not t.is_cuda or not t.is_contiguous():
+                raise M3drtError(f"boundary: {k} is a contiguous device tensor")
+            setattr(a, k, t.data_ptr())
+    if ghost is not None:
+        a.ghost_chi, a.ghost_S, a.ghost_I = [t.data_ptr() for t in ghost]
+    return a
+
+
+def _set_upstream_boundary(self, kind, ghost, *, chi1=None, S1=None, chi2=None, S2=None, g500=None, chi500_1=None,
+                           T_ghost=None, nu=None, ds3=0.0, mem=None):
+    """set_upstream_boundary into ghost = (chi, S, I), native (nf, ny, nx) float32
+    planes: kind "top", "lte", "pchip" or "upstream" (mem = (chi, S, I))."""
+    L = _bnd_lib()
+    nf, ny, nx = ghost[0].shape
+    mc, ms, mi = mem if mem is not None else (None, None, None)
+    a = _boundary_args(kind, ghost, nf, ny, nx, chi1=chi1, S1=S1, chi2=chi2, S2=S2, g500=g500, chi500_1=chi500_1,
+                       T_ghost=T_ghost, nu=nu, ds3=ds3, mem_chi=mc, mem_S=ms, mem_I=mi)
+    self._bind_stream()
+    _check(L.m3drt_set_upstream_boundary(self._h, C.byref(a)), "m3drt_set_upstream_boundary")
+
+
+def _set_downstream_boundary(self, chi_exit, S_exit, I_exit, mem):
+    """set_downstream_boundary: the exit plane (chi, S, I) into mem = (chi, S, I)."""
+    L = _bnd_lib()
+    nf, ny, nx = chi_exit.shape
+    a = _boundary_args("top", None, nf, ny, nx, chi1=chi_exit, S1=S_exit, exit_I=I_exit, mem_chi=mem[0],
+                       mem_S=mem[1], mem_I=mem[2])
+    self._bind_stream()
+    _check(L.m3drt_set_downstream_boundary(self._h, C.byref(a)), "m3drt_set_downstream_boundary")
+
+
+Context.lte_background = _lte_background
+Context.set_upstream_boundary = _set_upstream_boundary
+Context.set_downstream_boundary = _set_downstream_boundary
+
+
+class OptabTable(C.Structure):
+    _fields_ = [("nnu", C.c_int32), ("ntemp", C.c_int32), ("nne", C.c_int32), ("mu", C.c_double * 2),
+                ("rot", C.c_double * 4), ("c0", C.c_double * 2), ("dc", C.c_double * 2), ("lam", C.c_void_p),
+                ("val", C.c_void_p), ("scat", C.c_void_p)]
+
+
+class OptabQuery(C.Structure):
+    _fields_ = [("ncell", C.c_int64), ("nf", C.c_int32), ("temp", C.c_void_p), ("ne", C.c_void_p),
+                ("chi500", C.c_void_p), ("lam_q", C.c_void_p), ("back", C.c_void_p), ("scat", C.c_void_p),
+                ("sf", C.c_int64), ("sc", C.c_int64)]
+
+
+class OptabPlan:
+    """The sampled background table on the device (m3drt_optab_plan_*).  lam (nnu)
+    float64, val and scat (nne, ntemp, nnu) float32 device tensors -- the
+    reference's (nnu, ntemp, nne) with nnu fastest; mu, rot (2x2, PC directions as
+    columns), c0, dc of the PC axes."""
+
+    def __init__(self, ctx, lam, val, scat, mu, rot, c0, dc):
+        L = lib()
+        if not getattr(L, "_optab_ok", False):
+            L.m3drt_optab_plan_create.argtypes = [C.c_void_p, C.POINTER(OptabTable), C.POINTER(C.c_void_p)]
+            L.m3drt_optab_slopes.argtypes = [C.c_void_p] * 7
+            L.m3drt_optab_interpolate.argtypes = [C.c_void_p, C.POINTER(OptabQuery)]
+            L.m3drt_optab_plan_destroy.argtypes = [C.c_void_p]
+            for n in ("m3drt_optab_plan_create", "m3drt_optab_slopes", "m3drt_optab_interpolate",
+                      "m3drt_optab_plan_destroy"):
+                getattr(L, n).restype = C.c_int
+            L._optab_ok = True
+        for t, dt in ((lam, torch.float64), (val, torch.float32), (scat, torch.float32)):
+            if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+                raise M3drtError("optab: lam float64, val and scat float32 contiguous device tensors")
+        ne_, nt, nnu = val.shape
+        rot = np.asarray(rot, np.float64)
+        t = OptabTable(nnu, nt, ne_, (C.c_double * 2)(*mu), (C.c_double * 4)(*rot.ravel(order="F")),
+                       (C.c_double * 2)(*c0), (C.c_double * 2)(*dc), lam.data_ptr(), val.data_ptr(), scat.data_ptr())
+        self.ctx, self.shape = ctx, val.shape
+        self._h = C.c_void_p()
+        ctx._bind_stream()
+        _check(L.m3drt_optab_plan_create(ctx._h, C.byref(t), C.byref(self._h)), "m3drt_optab_plan_create")
+
+    def slopes(self):
+        out = [torch.empty(self.shape, dtype=torch.float32, device="cuda") for _ in range(6)]
+        self.ctx._bind_stream()
+        _check(lib().m3drt_optab_slopes(self._h, *[_p(o) for o in out]), "m3drt_optab_slopes")
+        return out
+
+    def interpolate(self, temp, ne, chi500, lam_q, back, scat):
+        """back, scat (nf, ncell) float32 device tensors."""
+        nc, nf = temp.numel(), lam_q.numel()
+        if back.numel() != nc * nf or scat.numel() != nc * nf:
+            raise M3drtError("optab: back and scat hold nf * ncell values")
+        q = OptabQuery(nc, nf, _p(temp), _p(ne), _p(chi500), _p(lam_q), _p(back), _p(scat), nc, 1)
+        self.ctx._bind_stream()
+        _check(lib().m3drt_optab_interpolate(self._h, C.byref(q)), "m3drt_optab_interpolate")
+
+    def close(self):
+        if self._h:
+            lib().m3drt_optab_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Isotope(C.Structure):
+    _fields_ = [("lines", Linelist), ("awgt", C.c_double), ("frac", C.c_double)]
+
+
+class Species(C.Structure):
+    _fields_ = [("niso", C.c_int32), ("iso", C.c_void_p), ("n", C.c_void_p), ("scale", C.c_double),
+                ("part", C.c_void_p), ("dep", C.c_void_p), ("nlev", C.c_int32), ("scat_group", C.c_int32)]
+
+
+class LinelistOpacityArgs(C.Structure):
+    _fields_ = ([("nf", C.c_int32), ("nsp", C.c_int32), ("equidistant", C.c_int32), ("ncell", C.c_int64),
+                 ("llam", C.c_void_p), ("sp", C.c_void_p)]
+                + [(n, C.c_void_p) for n in ("temp", "vmic", "vproj", "e_kT", "ne", "tothi", "stark_pe", "all_chi",
+                                              "all_eta", "one_m_eps", "chi_scat", "beta", "chi_lines", "eta_lines")]
+                + [("tolerance", C.c_float), ("line_reject_frac", C.c_float)])
+
+
+def _add_linelist_opacities(self, species, llam, all_chi, all_eta, cells, equidistant=True, one_m_eps=None,
+                            chi_scat=None, beta=None, chi_lines=None, eta_lines=None, tolerance=1e-4,
+                            line_reject_frac=0.0):
+    """add_linelist_opacities for metals and molecules on the device.  species: list
+    of dicts -- isotopes: list of (lines dict of device tensors as _line_opacity_equi
+    takes, awgt, frac); n, part (ncell) float64 device tensors; scale; dep (nlev,
+    ncell) float32 or None; scat_group.  Spectra (all_chi, all_eta, chi_scat, beta,
+    chi_lines, eta_lines) are (nf, ncell) float32, one_m_eps (2, nf, ncell); cells:
+    temp, vmic, vproj, ekt, ne, tothi, spe (ncell) float32."""
+    L = lib()
+    if not getattr(L, "_llo_ok", False):
+        L.m3drt_add_linelist_opacities.argtypes = [C.c_void_p, C.POINTER(LinelistOpacityArgs)]
+        L.m3drt_add_linelist_opacities.restype = C.c_int
+        L._llo_ok = True
+    nf, ncell = all_chi.shape
+    for t in (all_chi, all_eta, chi_scat, beta, chi_lines, eta_lines):
+        if t is not None and (t.dtype != torch.float32 or tuple(t.shape) != (nf, ncell) or not t.is_contiguous()):
+            raise M3drtError("spectra are contiguous (nf, ncell) float32 device tensors")
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    keep = []
+    sps = (Species * max(1, len(species)))()
+    for s, sp in enumerate(species):
+        isos = (Isotope * max(1, len(sp["isotopes"])))()
+        for k, (lines, awgt, frac) in enumerate(sp["isotopes"]):
+            nl = int(lines["llam0"].numel())
+            isos[k] = Isotope(Linelist(nl, *[p(lines.get(n)) for n in (
+                "llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk", "unsold", "lev_i", "lev_j", "ipart")],
+                None, None, None), float(awgt), float(frac))
+        keep.append(isos)
+        dep = sp.get("dep")
+        sps[s] = Species(len(sp["isotopes"]), C.cast(isos, C.c_void_p), p(sp["n"]), float(sp.get("scale", 1.0)),
+                         p(sp["part"]), p(dep), 0 if dep is None else dep.shape[0], int(sp.get("scat_group", 0)))
+    a = LinelistOpacityArgs(nf, len(species), int(bool(equidistant)), ncell, p(llam), C.cast(sps, C.c_void_p),
+                            *[p(cells[k]) for k in ("temp", "vmic", "vproj", "ekt", "ne", "tothi", "spe")],
+                            p(all_chi), p(all_eta), p(one_m_eps), p(chi_scat), p(beta), p(chi_lines), p(eta_lines),
+                            tolerance, line_reject_frac)
+    self._bind_stream()
+    _check(L.m3drt_add_linelist_opacities(self._h, C.byref(a)), "m3drt_add_linelist_opacities")
+
+
+Context.add_linelist_opacities = _add_linelist_opacities

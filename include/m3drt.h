@@ -590,6 +590,176 @@ int m3drt_native_strides(m3drt_sweep_desc *desc);
 /* Strides of the reference layout work(f,x,y,z), f fastest. */
 int m3drt_reference_strides(m3drt_sweep_desc *desc);
 
+/* The line-list opacity of a composition: replaces add_linelist_opacities
+ * (HOP/composition_mod.f90:1663-1954) for its metals and molecules (the hydrogen
+ * Stark lines are a separate profile) together with add_particle_opacity
+ * (HOP/linelist_mod.f90:970-1070) for each of their isotopes.
+ *   logref = ln(tolerance/voigt_peak) + ln all_chi (-huge where all_chi <= 0), once,
+ *     from the working opacity before any line (:1766-1772);
+ *   per species and isotope: inv_df = c/sqrt(2kT/(m_u awgt) + vmic^2),
+ *     vbar2 = 8kT/(pi m_u) (m_u/m_H + 1/awgt) (HOP/ion_mod.f90:198-221),
+ *     lnbase = ln max(n, tiny) [+ ln scale] - part + ln inv_df + ln frac, and the
+ *     Voigt summation of m3drt_line_opacity_equi (equidistant) or m3drt_line_opacity
+ *     into the increments chi_l, eta_l, with all_chi as the tolerance reference;
+ *     species with departures (dep) in NLTE, the others in LTE with the line
+ *     scattering fraction of their stage group (one_m_eps group 1 for neutrals,
+ *     2 for ions, :1928-1934) when chi_scat is given;
+ *   all_chi += chi_l, all_eta += eta_l (and chi_lines, eta_lines = the increments);
+ *   chi_scat *= 1 - beta (working to physical units, :1879-1890) when beta is given.
+ * All (f, cell) arrays have cells fastest (f*ncell + cell): all_chi, all_eta,
+ * chi_scat, beta, chi_lines, eta_lines, and one_m_eps (f, cell, 2).  Species and
+ * isotope descriptors are HOST arrays, their members DEVICE arrays. */
+typedef struct {
+  m3drt_linelist lines;       /* the isotope's line list (nline 0: skipped) */
+  double awgt;                /* atomic weight [u] */
+  double frac;                /* isotope fraction (1 for the main list) */
+} m3drt_isotope;
+typedef struct {
+  int32_t niso;
+  const m3drt_isotope *iso;   /* host array of niso */
+  const double *n;            /* (ncell) number density [cm^-3] */
+  double scale;               /* population_scale (1: none) */
+  const double *part;         /* (ncell) ln of the partition function */
+  const float *dep;           /* (ncell, nlev) departure coefficients, NULL: LTE */
+  int32_t nlev;
+  int32_t scat_group;         /* one_m_eps group of an LTE species: 1 neutral, 2 ion, 0 none */
+} m3drt_species;
+typedef struct {
+  int32_t nf, nsp, equidistant;
+  int64_t ncell;
+  const double *llam;                              /* (nf) ln(lambda) */
+  const m3drt_species *sp;                         /* host array of nsp */
+  const float *temp, *vmic, *vproj, *e_kT, *ne, *tothi, *stark_pe;  /* (ncell); vmic [km/s] */
+  float *all_chi, *all_eta;                        /* working xhat, ehat: added to */
+  const float *one_m_eps;                          /* (f, cell, 2) or NULL */
+  float *chi_scat;                                 /* or NULL (with one_m_eps) */
+  const float *beta;                               /* or NULL */
+  float *chi_lines, *eta_lines;                    /* or NULL: the increments */
+  float tolerance, line_reject_frac;               /* 1e-4 and 0 in the reference */
+} m3drt_linelist_opacity_args;
+int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist_opacity_args *args);
+
+/* The LTE background factors per (frequency, cell): replaces the beta/Bnu loop of
+ * lte_background_opacities (SRC/rt_patch_mod.f90:2004-2023): h_kT = h/k / T and
+ * hv_kT = nu h_kT in FP32, beta = max(exp(-hv_kT), the smallest normal FP32 number),
+ * Bnu = beta/(1 - beta) (occupation units), with the reference's h and k
+ * (HOP/const_mod.f90:39-43).  nu (nf) and temp (ncell) FP64; beta and Bnu FP32 at
+ * f*sf + cell*sc (sf = sc = 0: sf = ncell, sc = 1). */
+typedef struct {
+  int32_t nf;
+  int64_t ncell;
+  const double *nu, *temp;
+  float *beta, *Bnu;
+  int64_t sf, sc;
+} m3drt_background_lte_args;
+int m3drt_lte_background(m3drt_ctx *ctx, const m3drt_background_lte_args *args);
+
+/* The sampled background-opacity table of HOP (HOP/optab_mod.f90): the LTE
+ * continuum extinction and its scattering part per chi500, sampled on (wavelength,
+ * PC axis 1, PC axis 2) nodes, the PC axes a rotation of (ln T, ln ne).
+ * m3drt_optab_plan_create replaces sample_build_derivatives (:127-166): it copies
+ * the table to the device with its monotone PCHIP node slopes along every axis.
+ * m3drt_optab_interpolate replaces sample_interpolate (:181-287), the separable
+ * tensor-product cubic Hermite of lte_background_opacities' sampled path
+ * (SRC/rt_patch_mod.f90:2115-2117): back = (kappa + kappa_scat) chi500 and
+ * scat = kappa_scat chi500 per (frequency, cell), at f*sf + cell*sc (zeros: sf =
+ * ncell, sc = 1).  m3drt_optab_slopes copies out the node slopes (dl, dt, de of
+ * val, sl, st, se of scat), each shaped like val.  Table and query arrays are
+ * DEVICE arrays; the plan keeps its own copies. */
+typedef struct {
+  int32_t nnu, ntemp, nne;
+  double mu[2];              /* mean of (ln T, ln ne) */
+  double rot[4];             /* PC directions as columns, column-major: rot(1,1), rot(2,1), rot(1,2), rot(2,2) */
+  double c0[2], dc[2];       /* first node and spacing of each PC axis */
+  const double *lam;         /* (nnu) wavelength nodes, ascending */
+  const float *val, *scat;   /* (nnu, ntemp, nne), nnu fastest */
+} m3drt_optab_table;
+typedef struct m3drt_optab_plan m3drt_optab_plan;
+typedef struct {
+  int64_t ncell;
+  int32_t nf;
+  const double *temp, *ne;   /* (ncell) [K], [cm^-3] */
+  const float *chi500;       /* (ncell) */
+  const double *lam_q;       /* (nf) wavelengths of the frequencies, the table's unit */
+  float *back, *scat;
+  int64_t sf, sc;
+} m3drt_optab_query;
+int m3drt_optab_plan_create(m3drt_ctx *ctx, const m3drt_optab_table *table, m3drt_optab_plan **plan);
+int m3drt_optab_slopes(m3drt_optab_plan *plan, float *dl, float *dt, float *de, float *sl, float *st, float *se);
+int m3drt_optab_interpolate(m3drt_optab_plan *plan, const m3drt_optab_query *query);
+int m3drt_optab_plan_destroy(m3drt_optab_plan *plan);
+
+/* Boundary planes of a directional sweep.  m3drt_set_upstream_boundary replaces
+ * set_upstream_boundary (SRC/rt_patch_mod.f90:3270-3374): it fills the ghost plane
+ * (chi, S, I) the sweep starts from, of one of four kinds:
+ *   M3DRT_GHOST_TOP       zeros (the top of the domain, downward rays);
+ *   M3DRT_GHOST_LTE       lte_ghost (:3352-3373): chi = chi1 * g500/chi500_1,
+ *                         S = I = Bnu(T_ghost) as m3drt_lte_background forms it;
+ *   M3DRT_GHOST_PCHIP     the other solvers' bottom ghost (:3287-3300): chi and S
+ *                         extrapolated by pchip_mono (HOP/interpol_mod.f90:289-365)
+ *                         from the first three planes, i*ds3 apart, to one plane below,
+ *                         clamped at 0; I = S (only the first two planes enter);
+ *   M3DRT_GHOST_UPSTREAM  the upstream slab's exit plane mem (chi, S, I), copied.
+ * m3drt_set_downstream_boundary replaces set_downstream_boundary (:3471-3511): it
+ * copies the exit plane (chi1, S1: the slab's last plane in the sweep direction;
+ * exit_I: its intensity) into mem.  Planes are (nf, ny, nx) addressed
+ * f*ps_f + x*ps_x + y*ps_y (zeros: native strides nx*ny, 1, nx); every plane of one
+ * call shares the strides. */
+#define M3DRT_GHOST_TOP 0
+#define M3DRT_GHOST_LTE 1
+#define M3DRT_GHOST_PCHIP 2
+#define M3DRT_GHOST_UPSTREAM 3
+typedef struct {
+  int32_t kind, nf, nx, ny;
+  int64_t ps_f, ps_x, ps_y;
+  const float *chi1, *S1, *chi2, *S2;   /* the first (PCHIP: and second) plane of the slab */
+  const float *g500, *chi500_1, *T_ghost; /* LTE: per cell (nx*ny, x fastest) */
+  const double *nu;                     /* LTE: frequencies [Hz] (nf) */
+  double ds3;                           /* PCHIP: cell size in z */
+  const float *exit_I;                  /* downstream: the exit plane's intensity */
+  float *mem_chi, *mem_S, *mem_I;       /* upstream: read; downstream: written */
+  float *ghost_chi, *ghost_S, *ghost_I; /* upstream: written */
+} m3drt_boundary_args;
+int m3drt_set_upstream_boundary(m3drt_ctx *ctx, const m3drt_boundary_args *args);
+int m3drt_set_downstream_boundary(m3drt_ctx *ctx, const m3drt_boundary_args *args);
+
+/* Angular quadrature of a transfer pass: replaces setup_quadrature
+ * (SRC/rt_params_mod.f90:170-220) and the sets it calls, set_radau_quadratures,
+ * set_lobatto_quadratures, set_gauss_quadratures, set_carlson_quadratures,
+ * set_custom_quadratures, set_grid_quadratures and nphi_copies
+ * (SRC/ray_mod.f90:36-467; Gauss-Radau nodes of solvers/rt/radau_mod.f90:15-115).
+ * The defaults of the reference are "radau", ntheta 2, nphi 4 for the iteration
+ * pass (jnu_weights 1) and "lobatto", ntheta 4, nphi 4 for the output pass
+ * (jnu_weights 0; SRC/rt_params_mod.f90:122-130).  HOST arrays: vec (cap, 3) holds
+ * the directions as the reference's vec(3, nray), wts (cap) the weights; *nray is
+ * set to the number of rays also when cap is too small (M3DRT_ELIMIT).  Ranges of
+ * the reference that stop its run (ntheta beyond a table) are M3DRT_EINVAL. */
+typedef struct {
+  const char *scheme;       /* "radau", "lobatto", "gauss", "custom", "grid+", "gridx", "grid*",
+                               "disk_center", "set_a2" ... "set_a8", "set_b4" ... "set_b8", "none" */
+  int32_t ntheta, nphi;     /* inclinations (custom: the number of mu values), azimuths */
+  int32_t jnu_weights;      /* 1: the set mirrored downward, weights summing to 1 (mean intensity);
+                               0: the upward rays with flux weights (w * mu) */
+  int32_t one_dimensional;  /* plane-parallel atmosphere: one azimuth per inclination */
+  const double *mu;         /* custom: ntheta inclinations (host) */
+  double ds[3];             /* cell size, for the grid schemes */
+} m3drt_quadrature_args;
+int m3drt_quadrature(const m3drt_quadrature_args *q, int32_t cap, double *vec, double *wts, int32_t *nray);
+
+/* The FP32 step weights the sweep kernels use, elementwise over n optical depths t
+ * (device arrays of n floats each).
+ * m3drt_linear_weights replaces linear_weights / linear_weights_array
+ * (SRC/rt_solvers_mod.f90:785-822): E = exp(-t), W = u0 - Psi, Psi = 1 - u0/t,
+ * Esc = 1 - Psi with u0 = 1 - E.
+ * m3drt_parab_moments replaces parab_moments (SRC/rt_solvers_mod.f90:862-877, the
+ * pure procedure of rt_patch_mod's interface): E = exp(-t), U0 = 1 - E,
+ * U1 = t - U0, q = (t^2 - 2 U1)/t.
+ * Both take E and 1 - E from exp_neg's reduction (HOP/exp_neg.inc:20-37), t
+ * clamped at 87 for the exponential. */
+int m3drt_linear_weights(m3drt_ctx *ctx, int64_t n, const float *t, float *E, float *W, float *Psi,
+                         float *Esc);
+int m3drt_parab_moments(m3drt_ctx *ctx, int64_t n, const float *t, float *E, float *U0, float *U1, float *q);
+
 #ifdef __cplusplus
 }
 #endif

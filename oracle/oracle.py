@@ -955,3 +955,118 @@ def pchip_ghost(y1, y2, y3, ds3):
     out = np.empty_like(ys[0])
     L.orc_pchip_ghost(out.size, float(ds3), *[_p(y) for y in ys], _p(out))
     return out
+
+
+class OrcOptab(C.Structure):
+    _fields_ = [("nnu", C.c_int), ("ntemp", C.c_int), ("nne", C.c_int), ("mu", C.c_double * 2),
+                ("rot", C.c_double * 4), ("c0", C.c_double * 2), ("dc", C.c_double * 2), ("lam", C.c_void_p),
+                ("val", C.c_void_p), ("scat", C.c_void_p)]
+
+
+def _optab(lam, val, scat, mu, rot, c0, dc, keep):
+    nnu, nt, ne = val.shape
+    F = lambda a: keep.p(np.asfortranarray(a, np.float32).ravel(order="F"), np.float32)
+    return OrcOptab(nnu, nt, ne, (C.c_double * 2)(*mu), (C.c_double * 4)(*np.asarray(rot, np.float64).ravel(order="F")),
+                    (C.c_double * 2)(*c0), (C.c_double * 2)(*dc), keep.p(lam, np.float64), F(val), F(scat))
+
+
+def optab_slopes(lam, f, mu, rot, c0, dc):
+    """sample_build_derivatives' node slopes of f (nnu, ntemp, nne): (dl, dt, de)."""
+    L = lib()
+    if not getattr(L, "_opt_ok", False):
+        L.orc_optab_slopes.argtypes = [C.POINTER(OrcOptab)] + [C.c_void_p] * 4
+        L.orc_optab_slopes.restype = None
+        L.orc_optab_interpolate.argtypes = [C.POINTER(OrcOptab)] + [C.c_void_p] * 6 + [C.c_int] + [C.c_void_p] * 3 + \
+                                           [C.c_int] + [C.c_void_p] * 3
+        L.orc_optab_interpolate.restype = None
+        L._opt_ok = True
+    keep = _Keep()
+    t = _optab(lam, f, f, mu, rot, c0, dc, keep)
+    fF = np.asfortranarray(f, np.float32).ravel(order="F")
+    out = [np.zeros_like(fF) for _ in range(3)]
+    L.orc_optab_slopes(C.byref(t), _p(fF), *[_p(o) for o in out])
+    return tuple(o.reshape(f.shape, order="F") for o in out)
+
+
+def optab_interpolate(lam, val, scat, mu, rot, c0, dc, temp, ne, chi500, lam_q):
+    """sample_interpolate over cells: (back, scat), each (ncell, nf) float32, and the
+    slopes it used (dl, dt, de, sl, st, se)."""
+    sv = optab_slopes(lam, val, mu, rot, c0, dc)
+    ss = optab_slopes(lam, scat, mu, rot, c0, dc)
+    L = lib()
+    keep = _Keep()
+    t = _optab(lam, val, scat, mu, rot, c0, dc, keep)
+    sl = [keep.p(np.asfortranarray(a).ravel(order="F"), np.float32) for a in sv + ss]
+    temp = np.ascontiguousarray(temp, np.float64)
+    lam_q = np.ascontiguousarray(lam_q, np.float64)
+    back = np.zeros((temp.size, lam_q.size), np.float32)
+    sc = np.zeros_like(back)
+    L.orc_optab_interpolate(C.byref(t), *sl, temp.size, _p(temp), keep.p(ne, np.float64), keep.p(chi500, np.float32),
+                            lam_q.size, _p(lam_q), _p(back), _p(sc))
+    return back, sc, sv + ss
+
+
+# ------------------------------------------------------------ line-list composition
+
+def particle_base(temp, vmic, n, part, awgt, frac=1.0, scale=1.0):
+    """add_particle_opacity's per-isotope (inv_df, vbar2, lnbase), each (ncell) float32."""
+    L = lib()
+    if not getattr(L, "_pb_ok", False):
+        L.orc_particle_base.argtypes = [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                        C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_particle_base.restype = None
+        L.orc_logref.argtypes = [C.c_long, C.c_void_p, C.c_float, C.c_void_p]
+        L.orc_logref.restype = None
+        L._pb_ok = True
+    keep = _Keep()
+    nc = np.asarray(temp).size
+    out = [np.empty(nc, np.float32) for _ in range(3)]
+    L.orc_particle_base(nc, keep.p(temp, np.float32), keep.p(vmic, np.float32), keep.p(n, np.float64), float(scale),
+                        keep.p(part, np.float64), float(awgt), float(frac), *[_p(o) for o in out])
+    return tuple(out)
+
+
+def logref_of(chi, tolerance=1e-4):
+    lib()
+    particle_base(np.ones(1), np.zeros(1), np.ones(1), np.zeros(1), 1.0)   # declares orc_logref
+    chi = np.ascontiguousarray(chi, np.float32)
+    out = np.empty_like(chi)
+    lib().orc_logref(chi.size, _p(chi), np.float32(tolerance), _p(out))
+    return out
+
+
+def add_linelist_opacities(species, llam, all_chi, all_eta, cells, equidistant=True, one_m_eps=None,
+                           chi_scat=None, beta=None, tolerance=1e-4):
+    """add_linelist_opacities for metals and molecules (HOP/composition_mod.f90:1663-1954):
+    species = list of dicts (isotopes: list of (lines dict, awgt, frac); n, part (ncell)
+    float64; scale; dep (nlev, ncell) or None; scat_group); all_chi, all_eta, chi_scat,
+    beta (ncell, nf); one_m_eps (2, ncell, nf); cells: temp, vmic, vproj, ekt, ne,
+    tothi, spe (ncell).  Returns (all_chi, all_eta, chi_scat, chi_l, eta_l)."""
+    all_chi = np.array(all_chi, np.float32, copy=True)
+    all_eta = np.array(all_eta, np.float32, copy=True)
+    logref = logref_of(all_chi, tolerance)
+    chi_l = np.zeros_like(all_chi)
+    eta_l = np.zeros_like(all_chi)
+    csc = None if chi_scat is None else np.array(chi_scat, np.float32, copy=True)
+    for sp in species:
+        for lines, awgt, frac in sp["isotopes"]:
+            if len(lines["llam0"]) == 0:
+                continue
+            idf, vb2, lnb = particle_base(cells["temp"], cells["vmic"], sp["n"], sp["part"], awgt, frac,
+                                          sp.get("scale", 1.0))
+            cl = dict(vproj=cells["vproj"], idf=idf, vb2=vb2, ekt=cells["ekt"], ne=cells["ne"],
+                      tothi=cells["tothi"], spe=cells["spe"])
+            dep = sp.get("dep")
+            g = sp.get("scat_group", 0)
+            scat = dep is None and g and csc is not None
+            r = add_opacity_equi(lines, llam, chi_l, eta_l, all_chi, logref, cl, lnb[None, :], dep=dep,
+                                 do_nlte=dep is not None, one_m_eps=one_m_eps[g - 1] if scat else None,
+                                 chi_scat=csc if scat else None, tol=tolerance, _general=not equidistant)
+            chi_l, eta_l = r["chi"], r["eta"]
+            if scat:
+                csc = r["chi_scat"]
+    all_eta = (all_eta + eta_l).astype(np.float32)
+    all_chi = (all_chi + chi_l).astype(np.float32)
+    if csc is not None and beta is not None:
+        csc = (csc * (np.float32(1) - np.asarray(beta, np.float32))).astype(np.float32)
+    return all_chi, all_eta, csc, chi_l, eta_l

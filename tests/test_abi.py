@@ -48,6 +48,9 @@ MIRRORS = {  # ctypes mirror in dispatch_amd/m3drt.py -> header typedef
     "HermiteFields": "m3drt_hermite_fields", "AdjointBufs": "m3drt_adjoint_bufs",
     "B1dBatch": "m3drt_b1d_batch", "B1dTile": "m3drt_b1d_tile", "WindowArgs": "m3drt_window_args",
     "RotateArgs": "m3drt_rotate_args", "LcGhost": "m3drt_lc_ghost",
+    "QuadratureArgs": "m3drt_quadrature_args", "BackgroundLteArgs": "m3drt_background_lte_args",
+    "BoundaryArgs": "m3drt_boundary_args", "OptabTable": "m3drt_optab_table", "OptabQuery": "m3drt_optab_query",
+    "Isotope": "m3drt_isotope", "Species": "m3drt_species", "LinelistOpacityArgs": "m3drt_linelist_opacity_args",
 }
 
 
@@ -111,13 +114,14 @@ def _header_structs():
     import re
     src = open(os.path.join(ROOT, "include", "m3drt.h")).read()
     out = []
-    for body, name in re.findall(r"typedef struct \{(.*?)\} (m3drt_[a-z_]+);", src, re.S):
+    for body, name in re.findall(r"typedef struct \{(.*?)\} (m3drt_[a-z0-9_]+);", src, re.S):
         fields = []
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         for line in body.split("\n"):
-            line = re.sub(r"/\*.*?\*/", "", line).strip().rstrip(";").replace("const ", "")
+            line = line.strip().rstrip(";").replace("const ", "")
             if not line:
                 continue
-            rest = re.match(r"(?:int32_t|int64_t|float|double|void|uint8_t)\s+(.*)", line).group(1)
+            rest = re.match(r"(?:int32_t|int64_t|float|double|void|uint8_t|char|m3drt_[a-z0-9_]+)\s+(.*)", line).group(1)
             fields += [re.sub(r"\[\d+\]", "", v.strip().lstrip("*").strip()) for v in rest.split(",")]
         out.append((name, fields))
     return out
