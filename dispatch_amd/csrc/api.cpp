@@ -26,9 +26,12 @@ int ensure_scratch(m3drt_ctx *ctx, size_t bytes) {
   M3DRT_HIP(hipStreamIsCapturing(ctx->stream, &cs));
   const bool capturing = cs != hipStreamCaptureStatusNone;
   if (bytes <= ctx->scratch_bytes) {
-    if (capturing) ctx->scratch_captured = true;
+    if (capturing && !ctx->scratch_fixed) ctx->scratch_captured = true;
     return M3DRT_OK;
   }
+  if (ctx->scratch_fixed)
+    return set_error(M3DRT_ELIMIT, "this call needs " + std::to_string(bytes) + " bytes of scratch, beyond the fixed " +
+                                       std::to_string(ctx->scratch_bytes) + " bytes of m3drt_sc_sweep_noalloc");
   if (capturing)
     return set_error(M3DRT_ELIMIT, "the scratch buffer would grow inside a stream capture: run the call once "
                                    "before capturing it");
@@ -72,6 +75,10 @@ int m3drt_create(m3drt_ctx **ctx, int device, int flags) {
   M3DRT_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_error(M3DRT_EINVAL, "m3drt_create: no such device");
   M3DRT_HIP(hipSetDevice(device));
+  {
+    const int rc = m3drt::noalloc_setup(device);
+    if (rc) return rc;
+  }
   hipDeviceProp_t prop;
   M3DRT_HIP(hipGetDeviceProperties(&prop, device));
   m3drt_ctx *c = new m3drt_ctx();

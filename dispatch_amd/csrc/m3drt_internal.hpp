@@ -13,6 +13,7 @@ struct m3drt_ctx {
   void *scratch = nullptr;   // per-call scratch (dt500, rc per cell, ...); grown on demand
   size_t scratch_bytes = 0;
   bool scratch_captured = false;  // a stream capture recorded launches that use `scratch`
+  bool scratch_fixed = false;     // `scratch` is a caller-fixed buffer for this call: never grown
   std::vector<void *> retired;    // outgrown scratch buffers captured graphs may still use
   int lds_max = 0;           // dynamic LDS bytes available per workgroup
   float *qlg_tab = nullptr;  // QLG-bot's two tables on the device (2 x 9020 floats)
@@ -26,10 +27,15 @@ int hip_check(hipError_t e, const char *what);
 // Makes ctx->scratch hold at least `bytes`.  Launches recorded by a stream capture
 // keep the scratch address, so a buffer that has been captured is never freed when
 // it is outgrown (it is retired and freed by m3drt_destroy), and growth inside a
-// capture is refused.
+// capture is refused.  With a fixed buffer (m3drt_sc_sweep_noalloc) a larger need
+// is refused with M3DRT_ELIMIT.
 int ensure_scratch(m3drt_ctx *ctx, size_t bytes);
-// the sweep entry point's checks and dispatch
+// the sweep entry point's checks and dispatch (m3drt_sc_sweep, m3drt_sc_sweep_noalloc)
 int sc_sweep_checked(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b, const char *who);
+// the fixed device scratch of m3drt_sc_sweep_noalloc: its address on `device`
+// (resolved once per device by noalloc_setup, which m3drt_create calls)
+int noalloc_setup(int device);
+void *noalloc_scratch(int device, size_t *bytes);
 
 #define M3DRT_HIP(call)                                         \
   do {                                                          \

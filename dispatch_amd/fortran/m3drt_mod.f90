@@ -545,7 +545,7 @@ module m3drt
   public :: m3drt_lte_background, m3drt_set_upstream_boundary, m3drt_set_downstream_boundary
   public :: m3drt_quadrature, m3drt_linear_weights, m3drt_parab_moments
   public :: m3drt_optab_plan_create, m3drt_optab_slopes, m3drt_optab_interpolate, m3drt_optab_plan_destroy
-  public :: m3drt_add_linelist_opacities
+  public :: m3drt_add_linelist_opacities, m3drt_sc_sweep_noalloc
   public :: short_char2_gpu, m3drt_error_message
 
   interface
@@ -753,6 +753,12 @@ module m3drt
       type(c_ptr), value :: ctx
       type(m3drt_linelist_opacity_args), intent(in) :: args
     end function m3drt_add_linelist_opacities
+    integer(c_int) function m3drt_sc_sweep_noalloc(ctx, desc, bufs) bind(C, name='m3drt_sc_sweep_noalloc')
+      import :: c_int, c_ptr, m3drt_sweep_desc, m3drt_sweep_bufs
+      type(c_ptr), value :: ctx
+      type(m3drt_sweep_desc), intent(in) :: desc
+      type(m3drt_sweep_bufs), intent(in) :: bufs
+    end function m3drt_sc_sweep_noalloc
     integer(c_int) function m3drt_rotate_array(ctx, args) bind(C, name='m3drt_rotate_array')
       import :: c_int, c_ptr, m3drt_rotate_args
       type(c_ptr), value :: ctx

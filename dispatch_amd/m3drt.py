@@ -187,13 +187,15 @@ class Context:
     def sc_sweep(self, chi, S, ghost_chi, ghost_S, ghost_I, chi500, chi500_up, g500, *, dir, ds, vec,
                  layout="native", wj=0.0, J=None, PsiJ=None, I_out=None, Psi_out=None, Irem_out=None,
                  escape_out=None, exit_I=None, eta_row=None, eta_mask=None, split_rates=False,
-                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2", hermite=None):
+                 get_rates=None, maser_dtau_max=2.0, maser_Inu_max=1e30, solver="linear2", hermite=None,
+                 noalloc=False):
         """One directional formal solution (short_char2 in 3D, long_char2 when
         nx = ny = 1; solver="quadratic": long_char_quadratic along every column;
         solver="linear": short_char, the Multi3D-comparison preset; solver="long_linear":
         long_char_linear along every column; hermite=dict of
         the composition fields of m3drt_hermite_fields: short_char_hermite)
-        with J += wj*I and PsiJ += wj*Psi fused."""
+        with J += wj*I and PsiJ += wj*Psi fused.  noalloc: through
+        m3drt_sc_sweep_noalloc (the library's fixed device scratch, no allocation)."""
         if solver not in SOLVERS:
             raise M3drtError(f"unknown solver {solver!r}")
         if get_rates is None:
@@ -229,6 +231,14 @@ class Context:
                 raise M3drtError("hermite beta is a cube shaped like chi")
             h = HermiteFields(*[p(hermite[k]) for k, _ in HermiteFields._fields_])
             _check(L.m3drt_sc_sweep_hermite(self._h, C.byref(d), C.byref(b), C.byref(h)), "m3drt_sc_sweep_hermite")
+            return
+        if noalloc:
+            L = lib()
+            if not getattr(L, "_noalloc_ok", False):
+                L.m3drt_sc_sweep_noalloc.argtypes = [C.c_void_p, C.POINTER(SweepDesc), C.POINTER(SweepBufs)]
+                L.m3drt_sc_sweep_noalloc.restype = C.c_int
+                L._noalloc_ok = True
+            _check(L.m3drt_sc_sweep_noalloc(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep_noalloc")
             return
         _check(lib().m3drt_sc_sweep(self._h, C.byref(d), C.byref(b)), "m3drt_sc_sweep")
 

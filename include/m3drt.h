@@ -120,6 +120,17 @@ typedef struct {
  */
 int m3drt_sc_sweep(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs);
 
+/* m3drt_sc_sweep without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same checks and kernels as m3drt_sc_sweep,
+ * with the per-cell scratch of the sweep (8 bytes per cell and plane of the
+ * straight-line kernel, 2 x 4 or 2 x 8 of the general one) in a fixed 32 MiB device
+ * array the library holds on every device it runs on until the process ends; a
+ * call that needs more is refused with M3DRT_ELIMIT before any launch (the
+ * 120 x 120 x 230 FP32 box takes 26.6 MB).  Every call on one device shares the
+ * array: calls that overlap on one device (two streams or threads) race. */
+int m3drt_sc_sweep_noalloc(m3drt_ctx *ctx, const m3drt_sweep_desc *desc, const m3drt_sweep_bufs *bufs);
+
 /* ------------------------------------------------------------------ adjoint sweep */
 
 /* Buffers of one direction's adjoint (reverse) sweep.  Layouts and strides as the
