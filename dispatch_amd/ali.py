@@ -41,9 +41,7 @@ import numpy as np
 import torch
 
 from . import m3drt, synth
-
-H_KT = synth.H_PLANCK / synth.K_BOLTZ
-TWO_H_C2 = 2.0 * synth.H_PLANCK / synth.C_LIGHT ** 2
+from .constants import BK, CC, E_K, H_4PI, TWOH_C2, UU
 PI_E2_MC = 0.0265400   # pi e^2 / (m_e c) [cm^2 Hz]
 
 
@@ -75,7 +73,15 @@ class AliConfig:
     conv: float = 1e-4
     ng: bool = True           # Ng acceleration (the reference's automatic trigger)
     ng_step: int = 0          # > 0: the reference's fixed schedule instead (&nlte_params ng_step,
-    ng_start: int = 8         # ng_start; SRC/atom_mod.f90:278-287, 2500-2502)
+    ng_start: int = 8         # ng_start; SRC/atom_mod.f90:278-287, 2500-2502); 1 or 2 is raised
+                              # to 3 (SRC/m3d_patch_mod.f90:472-476)
+
+    def __post_init__(self):
+        # Ng needs three iterates after an extrapolation before the next one
+        if 0 < self.ng_step < 3:
+            import warnings
+            warnings.warn(f"NG acceleration needs 3 initial iterations; ng_step {self.ng_step} set to 3")
+            self.ng_step = 3
     precond: bool = True      # preconditioned (ALI) rates; False: plain Lambda iteration
     fused: bool = True        # one kernel per (window, direction) for continuum, lines and units
     graph: bool = False       # replay the rate pass of an iteration as one captured HIP graph
@@ -213,16 +219,16 @@ class NlteProblem:
         self.nstar = dev(nstar, torch.float64)
         self.totn = dev(totn, torch.float64)
         self.b = torch.ones_like(self.nstar)
-        self.e_kT64 = dev(synth.EV_K / T, torch.float64)
+        self.e_kT64 = dev(E_K / T, torch.float64)
         self.temp = dev(T, torch.float64)
         # Seaton's ionisation with gbar = 0.1 and the edge cross section
         self.cup = dev(synth.van_regemorter(atom, T, ne, nstar, ion_scale=0.1 * float(atom.sigma0[0])),
                        torch.float64)
         self.atom_dev = m3drt.AtomOnDevice(atom) if device == "cuda" else atom
         # per cell: Doppler width (mass 56 plus 1 km/s), damping, e/kT
-        vth = np.sqrt(2 * synth.K_BOLTZ * T / (56.0 * synth.AMU) + 1e5 ** 2) / synth.C_LIGHT
+        vth = np.sqrt(2 * BK * T / (56.0 * UU) + 1e5 ** 2) / CC
         self.inv_df = dev(1.0 / vth, torch.float32)
-        self.e_kT32 = dev(synth.EV_K / T, torch.float32)
+        self.e_kT32 = dev(E_K / T, torch.float32)
         self.adamp = dev(np.tile((2e-3 + 1e-2 * (ne / 1e14) ** 0.3)[None, :], (atom.nline, 1)), torch.float32)
         self.pop_ratio = torch.ones(ncell, dtype=torch.float64, device=self.device)
         self.vz = box.vz.reshape(-1)
@@ -244,15 +250,18 @@ class NlteProblem:
     # ---------------------------------------------------------------- spectra
     def _background(self, nu):
         """LTE background continuum on frequencies nu: chi_c, beta = e^(-h nu/kT) and
-        B = beta / (1 - beta), FP32 (nf, ncell); and B of the ghost plane below."""
-        T = self.temp[None, :]
+        B = beta / (1 - beta), FP32 (nf, ncell), from m3drt_lte_background
+        (lte_background_opacities' factors); and B of the ghost plane below."""
         c5 = self.box.chi500.double().reshape(1, -1)
-        nu = torch.as_tensor(nu, dtype=torch.float64, device=self.device)[:, None]
-        chi_c = c5 * (0.7 + 0.3 * (6e14 / nu) ** 3)
-        beta = torch.exp(-H_KT * nu / T).clamp(min=np.finfo(np.float32).tiny)
-        B = beta / (1.0 - beta)
-        bg = torch.exp(-H_KT * nu / self.box.T_ghost_bot.reshape(1, -1)).clamp(min=np.finfo(np.float32).tiny)
-        return chi_c.float(), beta.float(), B.float(), (bg / (1.0 - bg)).float().contiguous()
+        nu = torch.as_tensor(nu, dtype=torch.float64, device=self.device)
+        chi_c = c5 * (0.7 + 0.3 * (6e14 / nu[:, None]) ** 3)
+        planes = []
+        for T in (self.temp, self.box.T_ghost_bot.reshape(-1).double().contiguous()):
+            beta = torch.empty((nu.numel(), T.numel()), dtype=torch.float32, device=self.device)
+            B = torch.empty_like(beta)
+            self.ctx.lte_background(nu, T.contiguous(), beta, B)
+            planes.append((beta, B))
+        return chi_c.float(), planes[0][0], planes[0][1], planes[1][1]
 
     def _window(self, lines):
         """A window of lines: each line's nq frequencies with the reference's abscissa
@@ -327,7 +336,7 @@ class NlteProblem:
             fs = j * cfg.ncont_f
             al[j, fs:fs + cfg.ncont_f] = atom.sigma0[j] * (atom.nu0[nl + j] / nus[j]) ** 3
             # 4 pi / (h nu) dnu (SRC/atom_mod.f90:2053-2055)
-            wnu[j, fs:fs + cfg.ncont_f] = _trapezoid(nus[j]) / (synth.H_PLANCK / (4 * math.pi) * nus[j])
+            wnu[j, fs:fs + cfg.ncont_f] = _trapezoid(nus[j]) / (H_4PI * nus[j])
         chi_c, beta, B, gB = self._background(nu)
         self.cont = dict(chi_c=chi_c, B=B, ghost_B=gB)
         dt = self.sweep_dtype
@@ -340,7 +349,7 @@ class NlteProblem:
         self.bf = dict(iblue=d(np.arange(nc) * cfg.ncont_f + 1, "iblue"), ired=d((np.arange(nc) + 1) * cfg.ncont_f, "ired"),
                        lev_i=d(atom.ti[nl:], "lev_i"), lev_j=d(atom.tj[nl:], "lev_j"),
                        active=d(np.ones(nc), "active"), ev=d(atom.ev, "ev"), wnu=d(wnu, "wnu"), alpha=d(al, "alpha"),
-                       twohnu3_c2=d(TWO_H_C2 * nu ** 3, "twohnu3_c2"), e_kT=self.e_kT64, nstar=self.nstar,
+                       twohnu3_c2=d(TWOH_C2 * nu ** 3, "twohnu3_c2"), e_kT=self.e_kT64, nstar=self.nstar,
                        Jnu=J.float().contiguous(), beta=beta, PsiJ=PsiJ.float().contiguous())
 
     # ---------------------------------------------------------------- iteration
@@ -358,7 +367,7 @@ class NlteProblem:
         cells = dict(b=self.b, nstar=self.nstar, pop_ratio=self.pop_ratio, inv_df=self.inv_df, e_kT=self.e_kT32,
                      adamp=self.adamp[l0:l1])
         for r, v in enumerate(self.vec):
-            vproj = (self.vz * (v[2] / synth.C_LIGHT)).float().contiguous()
+            vproj = (self.vz * (v[2] / CC)).float().contiguous()
             d = dict(w.const, vproj=vproj, **cells)
             if self.cfg.fused:
                 # the same steps as below in one kernel (m3drt_window_opacity)

@@ -3,8 +3,9 @@
 
 For each upward output ray the work arrays are rotated so that the rays become
 columns (rotate_array, SRC/rt_patch_mod.f90:3809-3863, pivoting about the top
-plane), a column solver runs from the LTE ghost plane below the bottom
-(long_char_linear, long_char_quadratic, long_char_hermite or
+plane), a column solver runs from the ghost plane below the bottom, built from
+the unrotated model (the LTE ghost for the hermite solvers, the PCHIP one for the
+others) (long_char_linear, long_char_quadratic, long_char_hermite or
 long_char_hermite2, SRC/rt_solvers_mod.f90:666-757, 891-1020, 1052-1504,
 1536-1895), and the exit planes give
 save_intensity's emergent products ie and flux (SRC/rt_patch_mod.f90:4151-4195,
@@ -29,6 +30,7 @@ import numpy as np
 import torch
 
 from . import m3drt, synth
+from .constants import CC, H_K
 from .save import emergent_products
 
 SOLVERS = ("long_linear", "quadratic", "hermite", "hermite2")
@@ -49,7 +51,37 @@ def dz_gradient(a, dz):
     return g
 
 
-def ray_state(ctx, box, chi, S, pe, vec):
+def ghost_planes(ctx, box, chi, S, pe):
+    """The bottom ghost planes of the output pass, built before the cubes are rotated
+    (set_upstream_boundary precedes radiative_transfer's rotate_array,
+    SRC/rt_patch_mod.f90:1034-1036), from the unrotated model: the LTE ghost for the
+    hermite solvers (lte_ghost_solver, SRC/rt_params_mod.f90:78-85; chi(first plane)
+    g500/chi500(first plane), S = I = B_nu(T below the bottom)), the PCHIP ghost for the
+    others (chi and S extrapolated from the first planes, clamped at 0, I = S), and the
+    model below the bottom the hermite solvers take (bottom_ghost_t: T, Pe, dPe/dz,
+    dlnT/dz)."""
+    nf, ny, nx = chi.shape[1:]
+    mk = lambda: [torch.empty((nf, ny, nx), dtype=torch.float32, device=chi.device) for _ in range(3)]
+    nu = torch.as_tensor(CC / box.lam, dtype=torch.float64, device=chi.device)
+    lte, pchip = mk(), mk()
+    ctx.set_upstream_boundary("lte", lte, chi1=chi[0].contiguous(), g500=box.g500_bot, chi500_1=box.chi500[0],
+                              T_ghost=box.T_ghost_bot.float().contiguous(), nu=nu)
+    ctx.set_upstream_boundary("pchip", pchip, chi1=chi[0].contiguous(), S1=S[0].contiguous(),
+                              chi2=chi[1].contiguous(), S2=S[1].contiguous(), ds3=box.ds[2])
+    dz = box.ds[2]
+    dpe0 = (pe[1] - pe[0]) / dz
+    Tb = box.T.float()
+    dlnt0 = (torch.log(Tb[1]) - torch.log(Tb[0])) / dz
+    model = dict(pe=torch.clamp(pe[0] - dz * dpe0, min=1e-3).contiguous(), dpe_dz=dpe0.contiguous(),
+                 temp=box.T_ghost_bot.float().contiguous(), dlnt_dz=dlnt0.contiguous())
+    return dict(lte=lte, pchip=pchip, model=model)
+
+
+def ghost_of(solver, ghosts):
+    return ghosts["lte"] if solver in ("hermite", "hermite2") else ghosts["pchip"]
+
+
+def ray_state(ctx, box, chi, S, pe, vec, ghosts):
     """The rotated cubes and fields of one ray; native layout (nz, nf, ny, nx)."""
     nz = box.nz
     dz = box.ds[2]
@@ -59,27 +91,28 @@ def ray_state(ctx, box, chi, S, pe, vec):
     cell = torch.stack([box.T.float(), pe])[:, :, None]                  # (2, nz, 1, ny, nx)
     cell = rotate(ctx, cell.contiguous(), vec, box.ds, top)[:, :, 0]
     T, pe_r = cell[0].contiguous(), cell[1].contiguous()
-    lam = torch.as_tensor(box.lam, dtype=torch.float64, device=chi.device)
-    hk_lam = (synth.H_PLANCK * synth.C_LIGHT / (lam * synth.K_BOLTZ)).float()   # h nu / k per frequency
-    # beta = max(exp(-h nu / k T), tiny) in FP32, as lte_background_opacities keeps it
-    beta = torch.clamp(torch.exp(-hk_lam[None, :, None, None] / T[:, None]), min=np.finfo(np.float32).tiny)
+    nu = torch.as_tensor(CC / box.lam, dtype=torch.float64, device=chi.device)
+    hknu = (H_K * nu).float()                       # real(h_k nu, 4) (SRC/rt_solvers_mod.f90:1096)
+    # beta = max(exp(-h nu / k T), tiny) of lte_background_opacities on the rotated
+    # composition, plane by plane into the native layout
+    nf = nu.numel()
+    beta = torch.empty_like(chi_r)
+    B = torch.empty((nf, box.ny * box.nx), dtype=torch.float32, device=chi.device)
+    T64 = T.double()
+    for z in range(nz):
+        ctx.lte_background(nu, T64[z].reshape(-1), beta[z], B)
     dpe = dz_gradient(pe_r, dz)
     dlnt = dz_gradient(torch.log(T), dz)
-    # the LTE ghost plane below the bottom (lte_ghost): chi scaled by the chi500 ratio,
-    # S = I = B_nu(T_ghost); its Pe and gradients extrapolated from the bottom plane
-    gh = synth.ghost_planes(box, chi_r, torch.float32)["up"]
-    Tg = box.T_ghost_bot.float().contiguous()
-    ghost = dict(pe=torch.clamp(pe_r[0] - dz * dpe[0], min=1e-3).contiguous(), dpe_dz=dpe[0].contiguous(),
-                 temp=Tg, dlnt_dz=dlnt[0].contiguous())
-    fields = dict(beta=beta.contiguous(), pe4=pe_r, temp=T, dpe_dz=dpe.contiguous(), dlnt_dz=dlnt.contiguous(),
-                  hknu=hk_lam.contiguous())
-    return chi_r, S_r, gh, fields, ghost
+    fields = dict(beta=beta, pe4=pe_r, temp=T, dpe_dz=dpe.contiguous(), dlnt_dz=dlnt.contiguous(),
+                  hknu=hknu.contiguous())
+    return chi_r, S_r, fields
 
 
-def solve(ctx, solver, chi, S, gh, fields, ghost, box, vec, exit_I):
+def solve(ctx, solver, chi, S, fields, ghosts, box, vec, exit_I):
     kw = dict(dir=1, ds=box.ds, vec=vec, layout="native", exit_I=exit_I)
+    gh = ghost_of(solver, ghosts)
     if solver in ("hermite", "hermite2"):
-        ctx.lc_sweep_hermite(chi, S, *gh, fields=fields, ghost="bottom", ghost_fields=ghost,
+        ctx.lc_sweep_hermite(chi, S, *gh, fields=fields, ghost="bottom", ghost_fields=ghosts["model"],
                              hybrid=solver == "hermite2", **kw)
     else:
         ctx.sc_sweep(chi, S, *gh, None, None, None, solver=solver, **kw)
@@ -108,6 +141,7 @@ def run(nf=256, windows=1, solvers=SOLVERS, rays=None, device=0, save_dir=None, 
         box = synth.make_box(nf=nf, seed=20261203, lam0_A=5000.0 + 4.0 * w, device=f"cuda:{device}")
         chi, S = synth.build_cubes(box)
         pe = synthetic_pe(box)
+        ghosts = ghost_planes(ctx, box, chi, S, pe)
         nbytes_cube = chi.numel() * 4
         exits = {s: torch.zeros((len(up), nf, box.ny, box.nx), dtype=torch.float32, device=chi.device)
                  for s in solvers}
@@ -115,23 +149,23 @@ def run(nf=256, windows=1, solvers=SOLVERS, rays=None, device=0, save_dir=None, 
             vec = [float(v) for v in vecs[i]]
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            state = ray_state(ctx, box, chi, S, pe, vec)
+            state = ray_state(ctx, box, chi, S, pe, vec, ghosts)
             e1.record()
             torch.cuda.synchronize()
             tim["rotate"].append(e0.elapsed_time(e1))
             for s in solvers:
                 for _ in range(reps):
                     e0.record()
-                    solve(ctx, s, *state, box, vec, exits[s][j])
+                    solve(ctx, s, *state, ghosts, box, vec, exits[s][j])
                     e1.record()
                     torch.cuda.synchronize()
                     tim[s].append(e0.elapsed_time(e1))
             del state
-        nu = synth.C_LIGHT / box.lam
+        nu = CC / box.lam
         for s in solvers:
             ie, flux, _ = emergent_products(exits[s], nu, wflux)
-            ie_all[s].append(ie.cpu().numpy())
-            flux_all[s].append(flux.cpu().numpy())
+            ie_all[s].append(ie)
+            flux_all[s].append(flux)
             if save_dir is not None:
                 from .save import save_intensity
                 save_intensity(f"{save_dir}/{s}", exits[s], nu, wflux, key=(1, w + 1))

@@ -15,8 +15,8 @@ line ``* filename ... bytes``.  rt_patch_t%save_intensity
 * ``flux`` (nf, 1, 1, 1, n_set): 2 pi sum_mu wmu ie, wmu the output rays' weights
   that already carry the cosine.
 
-Arrays are formed on the device in FP32, in the reference's order of operations, and
-copied to the host once.
+i3 is formed on the device; the surface and ray sums run on the host in the
+reference's order, so the files match the reference's bit for bit.
 """
 from __future__ import annotations
 
@@ -25,10 +25,10 @@ import os
 import numpy as np
 import torch
 
-H_PLANCK = 6.62607015e-27
-C_LIGHT = 2.99792458e10
-CM_TO_AA = 1e8
-PI = 3.14159265359          # the reference's pi (HOP/const_mod.f90)
+from .constants import CC, CM_TO_AA, HH, PI
+
+# 2 h / (c^3 1e8): occupation units to erg/s/cm^2/A/sr per nu^5 (rt_patch_mod.f90:4154)
+CONVERSION = 2 * HH / (CC ** 3 * CM_TO_AA)
 
 
 def file_name(label: str, key) -> str:
@@ -71,17 +71,29 @@ def emergent_products(exits, nu, wmu, resolved=False):
     exits: (nmu, nf, ny, nx) emergent intensities in occupation units (FP32, device),
     nu: (nf) frequencies [Hz], wmu: (nmu) the output rays' flux weights (cosine
     included).  Returns Fortran-shaped arrays: ie (nf, 1, 1, nmu, 1), flux
-    (nf, 1, 1, 1, 1), i3 (nf, nx, ny, nmu, 1) or None."""
+    (nf, 1, 1, 1, 1), i3 (nf, nx, ny, nmu, 1) or None.
+
+    The reference's arithmetic, step for step (SRC/rt_patch_mod.f90:4165-4192):
+    i3 = conversion nu^5 I formed in real(8) and rounded once to real(4) (on the
+    device); ie the real(4) sum over the surface cells, y outer and x inner, one
+    cell after another, then divided by real(nx ny); flux the real(4) sum over the
+    rays of real(2 pi wmu, 4) ie.  The two sums run on the host (they are a few
+    thousand vector adds of nmu x nf values, and their order is what fixes the
+    result's last bit)."""
     nmu, nf, ny, nx = exits.shape
-    conv = torch.as_tensor(2 * H_PLANCK / (C_LIGHT ** 3 * CM_TO_AA), dtype=torch.float64)
-    scale = (conv * torch.as_tensor(nu, dtype=torch.float64) ** 5).float().to(exits.device)
-    i3 = exits.float() * scale[None, :, None, None]                  # (mu, f, y, x) real(4)
-    ie = i3.sum(dim=(2, 3), dtype=torch.float32) / np.float32(nx * ny)   # (mu, f)
-    w = torch.as_tensor([np.float32(2 * PI * float(v)) for v in wmu], dtype=torch.float32, device=exits.device)
-    flux = torch.zeros(nf, dtype=torch.float32, device=exits.device)
+    scale64 = CONVERSION * np.asarray(nu, dtype=np.float64) ** 5
+    scale = torch.as_tensor(scale64, dtype=torch.float64, device=exits.device)
+    i3 = (scale[None, :, None, None] * exits.double()).float()            # (mu, f, y, x) real(4)
+    cells = i3.permute(2, 3, 0, 1).contiguous().cpu().numpy()             # (y, x, mu, f)
+    ie = np.zeros((nmu, nf), np.float32)
+    for y in range(ny):
+        for x in range(nx):
+            ie += cells[y, x]
+    ie /= np.float32(nx * ny)
+    flux = np.zeros(nf, np.float32)
     for m in range(nmu):
-        flux = flux + w[m] * ie[m]
-    ie_f = ie.t().reshape(nf, 1, 1, nmu, 1)
+        flux += np.float32(2 * PI * float(wmu[m])) * ie[m]
+    ie_f = np.ascontiguousarray(ie.T).reshape(nf, 1, 1, nmu, 1)
     i3_f = i3.permute(1, 3, 2, 0).reshape(nf, nx, ny, nmu, 1) if resolved else None
     return ie_f, flux.reshape(nf, 1, 1, 1, 1), i3_f
 

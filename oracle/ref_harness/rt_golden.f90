@@ -17,12 +17,16 @@
 !          pchip_mono at one plane below the first, before the clamp at 0
 !   optab  sample_interpolate of a sampled background table (HOP/optab_mod.f90:
 !          127-287) -> node slopes and (back, scat) at the query cells
+!   save   the statements of save_intensity (SRC/rt_patch_mod.f90:4151-4195) for
+!          one patch and one set, with the constants of HOP/const_mod.f90:
+!          nf, nx, ny, nmu, nu(nf), wmu(nmu), I(nf,nx,ny,nmu) -> i3, ie, flux
 ! All files are raw little-endian streams.
 !===============================================================================
 program rt_golden
   use ray_mod
   use interpol_mod, only: pchip_mono
   use optab_mod, only: sample_optab_type
+  use const_mod, only: HH, CC, CM_TO_AA, pi
   implicit none
   character(len=256) :: kind, fin, fout
   integer :: ui, uo
@@ -38,6 +42,8 @@ program rt_golden
     call do_pchip()
   case ('optab')
     call do_optab()
+  case ('save')
+    call do_save()
   case default
     error stop 'rt_golden: unknown kind'
   end select
@@ -115,5 +121,30 @@ contains
     call t%interpolate(temp, ne, chi500, back, scat, lam_q, 1, nf)
     write(uo) t%dl, t%dt, t%de, t%sl, t%st, t%se, back, scat
   end subroutine do_optab
+
+  subroutine do_save()
+    real(8), parameter :: conversion = 2 * HH / (CC**3 * CM_TO_AA)
+    integer :: nf, nx, ny, nmu, f, x, y, mu
+    real(8), allocatable :: nu(:), wmu(:)
+    real(4), allocatable :: mem(:,:,:,:), i3(:,:,:,:), ie(:,:), flux(:)
+    read(ui) nf, nx, ny, nmu
+    allocate(nu(nf), wmu(nmu), mem(nf, nx, ny, nmu), i3(nf, nx, ny, nmu), ie(nf, nmu), flux(nf))
+    read(ui) nu, wmu, mem
+    do f = 1, nf
+      i3(f,:,:,:) = conversion * nu(f)**5 * mem(f,:,:,:)
+    end do
+    ie = 0.
+    do y = 1, ny
+      do x = 1, nx
+        ie = ie + i3(:,x,y,:)
+      end do
+    end do
+    ie = ie / real(nx*ny)
+    flux = 0.
+    do mu = 1, nmu
+      flux = flux + real(2*pi*wmu(mu), 4) * ie(:,mu)
+    end do
+    write(uo) i3, ie, flux
+  end subroutine do_save
 
 end program rt_golden

@@ -46,6 +46,11 @@ class OracleCtx:
         if Irem_out is not None: Irem_out.copy_(back(irem))
         if escape_out is not None: escape_out.copy_(back(esc))
 
+    def lte_background(self, nu, temp, beta, Bnu):
+        b, B = O.lte_background(n(nu), n(temp))
+        beta.copy_(torch.from_numpy(b.reshape(beta.shape)))
+        Bnu.copy_(torch.from_numpy(B.reshape(Bnu.shape)))
+
     def convert_to_work_units(self, chi_c, xhat, ehat, beta):
         x, e = O.convert_to_work_units(n(chi_c), n(ehat), n(beta))
         xhat.copy_(torch.from_numpy(x)); ehat.copy_(torch.from_numpy(e))

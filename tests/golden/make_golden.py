@@ -322,6 +322,20 @@ def gen_optab(rng):
          sl=sl[3], st=sl[4], se=sl[5], back=back, scat=sc)
 
 
+def gen_save(rng):
+    nf, nx, ny, nmu = 37, 9, 7, 3
+    nu = np.sort(rng.uniform(3.0e14, 1.2e15, nf))
+    wmu = np.array([0.0833333333, 0.3101752434, 0.2731580899])
+    I = (10 ** rng.uniform(-12, -2, (nmu, ny, nx, nf))).astype(np.float32)   # C order = Fortran (nf,nx,ny,nmu)
+    out = np.frombuffer(run("save", i32(nf, nx, ny, nmu) + nu.tobytes() + wmu.tobytes() + I.tobytes(), EXE_RT),
+                        np.float32)
+    n3 = I.size
+    i3 = out[:n3].reshape(nmu, ny, nx, nf)
+    ie = out[n3:n3 + nf * nmu].reshape(nmu, nf)
+    flux = out[n3 + nf * nmu:]
+    save("rt_save_intensity.npz", rt=True, in_nu=nu, in_wmu=wmu, in_I=I, i3=i3, ie=ie, flux=flux)
+
+
 def main():
     """python make_golden.py [name ...]: every fixture, or only the named ones (the
     seeded draws are the same either way: each fixture has its own generator)."""
@@ -345,6 +359,7 @@ def main():
     gen_quad()
     gen_pchip(np.random.default_rng(20261220))
     gen_optab(np.random.default_rng(20261221))
+    gen_save(np.random.default_rng(20261222))
 
 
 if __name__ == "__main__":

@@ -47,3 +47,17 @@ def test_fixed_ng_schedule_follows_fortran_mod():
     assert [i for i in range(1, 30) if ng_fixed_fires(i, 4, 8)] == [9, 13, 17, 21, 25, 29]
     assert [i for i in range(1, 20) if ng_fixed_fires(i, 3, 6)] == [7, 10, 13, 16, 19]
     assert not any(ng_fixed_fires(i, 1, 8) for i in range(1, 40))   # mod(., 1) is never 1
+
+
+def test_ng_step_below_three_is_raised_to_three():
+    """as the reference's namelist check (SRC/m3d_patch_mod.f90:472-476): Ng needs 3
+    iterates after an extrapolation, so ng_step 1 or 2 becomes 3 (firing at 9, 12, ...)"""
+    import warnings
+    from dispatch_amd.ali import AliConfig, ng_fixed_fires
+    for step in (1, 2):
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            cfg = AliConfig(ng_step=step)
+        assert cfg.ng_step == 3 and any("set to 3" in str(x.message) for x in w)
+        assert [i for i in range(1, 20) if ng_fixed_fires(i, cfg.ng_step, cfg.ng_start)] == [9, 12, 15, 18]
+    assert AliConfig(ng_step=0).ng_step == 0 and AliConfig(ng_step=4).ng_step == 4

@@ -21,17 +21,28 @@ def test_output_pass_columns_match_oracle(ctx):
     vecs, _ = synth.gauss_rays(6)
     vec = [float(v) for v in vecs[1]]                 # mu = 0.789, azimuth 60 degrees
     print(f"output pass ray {vec}", flush=True)
-    c, s, gh, fields, ghost = OP.ray_state(ctx, box, chi, S, pe, vec)
+    ghosts = OP.ghost_planes(ctx, box, chi, S, pe)
+    c, s, fields = OP.ray_state(ctx, box, chi, S, pe, vec, ghosts)
+    ghost = ghosts["model"]
     h = lambda t: t.cpu().numpy()
     ref = lambda t: np.moveaxis(h(t), 1, 3)           # (nz, nf, ny, nx) -> (nz, ny, nx, nf)
     nz = box.nz
     work = lambda cube, plane: np.concatenate([np.moveaxis(h(plane), 0, 2)[None], ref(cube),
                                                np.zeros((1, box.ny, box.nx, nf), np.float32)], 0)
-    wchi, wS = work(c, gh[0]), work(s, gh[1])
-    wI = np.zeros_like(wchi)
-    wI[0] = np.moveaxis(h(gh[2]), 0, 2)
+    # the ghost planes against the oracle's boundary restatements on the unrotated model
+    lam_nu = synth.C_LIGHT / box.lam
+    oc, oS, _ = O.lte_ghost(lam_nu, h(chi[0]).reshape(nf, -1), h(box.g500_bot), h(box.chi500[0]),
+                            h(box.T_ghost_bot.float()))
+    assert np.array_equal(h(ghosts["lte"][0]).reshape(nf, -1), oc)
+    assert np.max(np.abs(h(ghosts["lte"][1]).reshape(nf, -1) / oS - 1)) < 1e-6
+    pc = np.maximum(O.pchip_ghost(h(chi[0]), h(chi[1]), h(chi[1]), box.ds[2]), 0)
+    assert np.array_equal(h(ghosts["pchip"][0]).ravel(), pc)
     for solver in OP.SOLVERS:
         print(f"  {solver}", flush=True)
+        gh = OP.ghost_of(solver, ghosts)
+        wchi, wS = work(c, gh[0]), work(s, gh[1])
+        wI = np.zeros_like(wchi)
+        wI[0] = np.moveaxis(h(gh[2]), 0, 2)
         I_out = torch.zeros_like(c)
         exit_I = torch.zeros_like(gh[0])
         kw = dict(dir=1, ds=box.ds, vec=vec, layout="native", I_out=I_out, exit_I=exit_I)
