@@ -126,10 +126,13 @@ def test_adjoint_accumulates_and_rejects(ctx):
                        chibar=cb.double(), Sbar=sb.double(), **geo)
 
 
-def test_formal_solution_autograd_matches_oracle(ctx):
+@pytest.mark.parametrize("patch", [23, 2])
+def test_formal_solution_autograd_matches_oracle(ctx, patch):
     """The torch.autograd front end (dispatch_amd.adjoint.formal_solution): the
     gradient of sum_r <W_r, exit_r> + <V, J> over four rays (two up, two down)
-    against the oracle's FP64 reverse sweeps summed over the rays."""
+    against the oracle's FP64 reverse sweeps summed over the rays; with one patch
+    of 5 planes, and with patches of 2, 2 and 1 planes (two checkpoint planes per
+    ray, the patch-boundary cotangents handed from patch to patch)."""
     from dispatch_amd.adjoint import RaySet, formal_solution
     vecs = np.array([(0.3, -0.4, 0.866), (-0.61, 0.05, 0.79), (-0.5, 0.2, -0.84), (0.45, 0.35, -0.82)])
     vecs /= np.linalg.norm(vecs, axis=1)[:, None]
@@ -164,7 +167,7 @@ def test_formal_solution_autograd_matches_oracle(ctx):
     tn = lambda a: dev(to_native(a))
     pn = lambda a: dev(plane_to_native(a))
     rs = RaySet(ctx, vecs, wj, base["ds"], dev(base["chi500"]), [dev(c) for c in cu],
-                dev(c5[0].astype(np.float32)), dev(c5[-1].astype(np.float32)))
+                dev(c5[0].astype(np.float32)), dev(c5[-1].astype(np.float32)), patch=patch)
     c = tn(chi[1:-1]).requires_grad_()
     s = tn(S[1:-1]).requires_grad_()
     gu = [pn(a[0]).requires_grad_() for a in (chi, S, Ig)]
@@ -181,3 +184,31 @@ def test_formal_solution_autograd_matches_oracle(ctx):
     close(pback(gd[1]), rsb[-1], "top ghost S grad")
     close(pback(gu[2]), rib[0], "bottom ghost I grad")
     close(pback(gd[2]), rib[-1], "top ghost I grad")
+
+
+def test_patch_sweeps_equal_the_column_sweep(ctx):
+    """The checkpointed forward (one launch per patch of 7 planes, each patch's
+    ghost plane the previous patch's last plane and exit intensity) gives the
+    single-launch column sweep's exit planes and J bit for bit, up and down."""
+    from dispatch_amd import synth
+    from dispatch_amd.adjoint import RaySet
+    box = synth.make_box(nx=24, ny=20, nz=30, nf=8, seed=31)
+    chi, S = synth.build_cubes(box, torch.float32)
+    gh = synth.ghost_planes(box, chi, torch.float32)
+    vecs, wj = synth.gauss_rays(6)
+    sel = [0, 5, 13, 20]
+    rs = RaySet(ctx, vecs, wj, box.ds, box.chi500, [synth.chi500_upwind(box, v) for v in vecs], box.g500_bot,
+                box.g500_top, patch=7)
+    ghosts = (gh["up"], gh["down"])
+    J1, J2 = torch.zeros_like(chi), torch.zeros_like(chi)
+    for r in sel:
+        print(f"ray {r}", flush=True)
+        e1 = torch.zeros_like(gh["up"][0])
+        e2 = torch.zeros_like(e1)
+        rs.sweep(r, chi, S, ghosts, exit_I=e1, J=J1)
+        ckpt = torch.empty((4,) + tuple(e1.shape), device="cuda")
+        rs.sweep_checkpointed(r, chi, S, ghosts, ckpt, e2, J=J2)
+        torch.cuda.synchronize()
+        assert torch.equal(e1, e2), r
+        assert float(e1.abs().max()) > 0
+    assert torch.equal(J1, J2)
