@@ -109,13 +109,21 @@ struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) Arr {
 // cache.  CPL: adjacent columns per lane (1, or 2 with the native layout: the
 // cube rows are then read and written 8 bytes per lane and the (dt500, 1/chi500)
 // pairs 16, so a pair of cells costs the vector-memory instructions of one).
+// threads of a workgroup with strip length L (fewer threads per workgroup leave
+// more registers per lane: 168 at 768 threads, 256 at 512)
+template <int L>
+constexpr int fast_max_threads() { return L >= 30 ? 512 : L >= 20 ? 768 : 1024; }
+
 template <int L, bool PJ, int MODE, int CPL>
-__global__ void __launch_bounds__(1024) sc2_fast(const FastArgs<float> a) {
+__global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs<float> a) {
   using namespace dev;
   using T = float;
   using V = Arr<T, CPL>;
   using DV = Arr<Pair<T>, CPL>;
-  constexpr int PD = CPL == 1 ? 3 : 1;  // two cells' loads per step in flight with CPL = 2
+  // rows of loads in flight ahead of the row being computed (two cells' loads per
+  // step with CPL = 2).  4 measured best for strips of 15 and 20 rows on the
+  // 120 x 120 plane (13.3 ms against 13.9 with 3 and 14.1 with 6 for strips of 20).
+  constexpr int PD = CPL == 1 ? (L >= 15 ? 4 : 3) : 1;
   constexpr uint32_t TB = sizeof(T);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, PW = a.PW, nxy = nx * ny;
@@ -383,8 +391,12 @@ int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int m
 }
 
 // strip lengths instantiated: one lane per column (60, 96, 120-cell planes), two
-// adjacent columns per lane (native layout, even nx)
-constexpr int kL1[] = {4, 12, 15};
+// adjacent columns per lane (native layout, even nx).  The first that fits is taken,
+// except that strips of 20 rows are preferred where they give 512 to 768 threads: on
+// the 120 x 120 plane (720 threads, up to 168 registers a lane) they measured 13.3 ms
+// per 1024-frequency direction against 13.7 for 15 (960 threads, 128 registers),
+// 14.8 for 24 and 14.5 for 30.
+constexpr int kL1[] = {4, 12, 15, 20, 24, 30};
 constexpr int kL2[] = {4, 6, 8};
 
 int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int cpl, int mode, size_t np) {
@@ -397,6 +409,9 @@ int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int 
   if (L == 4) return fast_variant<4, 1>(ctx, a, nf, nt, mode, np);
   if (L == 12) return fast_variant<12, 1>(ctx, a, nf, nt, mode, np);
   if (L == 15) return fast_variant<15, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 20) return fast_variant<20, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 24) return fast_variant<24, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 30) return fast_variant<30, 1>(ctx, a, nf, nt, mode, np);
   return -1;
 }
 
@@ -417,12 +432,22 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   const char *cenv = getenv("M3DRT_FAST_CPL");
   int cpl = 1;
   if (cenv && atoi(cenv) == 2 && d->cs_x == 1 && d->gs_x == 1 && nx % 2 == 0 && !(get_rates && b->PsiJ)) cpl = 2;
+  // the strip length: the first instantiated one that tiles ny into at least two
+  // strips within its workgroup's thread limit (M3DRT_FAST_L picks one of them)
+  const char *lenv = getenv("M3DRT_FAST_L");
+  const int want = lenv ? atoi(lenv) : 0;
+  auto max_threads = [](int l) { return l >= 30 ? 512 : l >= 20 ? 768 : 1024; };
   int L = 0;
+  if (!want && cpl == 1 && ny % 20 == 0 && ny / 20 >= 2 && (ny / 20) * nx >= 512 && (ny / 20) * nx <= 768) L = 20;
   for (int pass = 0; pass < 2 && !L; ++pass) {
     const int nxp = nx / cpl;
-    const int smax = 1024 / nxp;  // strips that fit the workgroup
-    for (int c : (cpl == 2 ? kL2 : kL1))
-      if (!L && ny % c == 0 && ny / c <= smax && ny / c >= 2) L = c;
+    const int *cand = cpl == 2 ? kL2 : kL1;
+    const int ncand = cpl == 2 ? (int)(sizeof(kL2) / sizeof(int)) : (int)(sizeof(kL1) / sizeof(int));
+    for (int i = 0; i < ncand && !L; ++i) {
+      const int c = cand[i];
+      if (want && c != want) continue;
+      if (ny % c == 0 && ny / c >= 2 && (ny / c) * nxp <= max_threads(c)) L = c;
+    }
     if (!L && cpl == 2) cpl = 1;
   }
   if (!L) return -1;
