@@ -37,12 +37,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+def cpus_visible() -> int:
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def host_cores() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS where the host sets it (a GPU
+    box shares its CPUs between its GPUs and sets it to its share), else every CPU
+    this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return min(int(env), cpus_visible())
+    return cpus_visible()
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def parse():
@@ -57,11 +76,16 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--no-batch1d", action="store_true")
     p.add_argument("--no-output-pass", action="store_true")
-    p.add_argument("--adjoint-fd", action="store_true",
-                   help="check the adjoint's abundance gradients against FP64 central differences")
+    p.add_argument("--no-adjoint-fd", action="store_true",
+                   help="skip the check of the abundance gradients against FP64 central differences")
     p.add_argument("--nf", type=int, default=1024)
     p.add_argument("--nz", type=int, default=230)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-nlte", action="store_true")
+    p.add_argument("--nlte-fp64", action="store_true",
+                   help="also run the NLTE iteration with the plain FP64 sweep and compare b")
+    p.add_argument("--nlte-max-iter", type=int, default=60)
+    p.add_argument("--no-voigt", action="store_true")
     return p.parse_args()
 
 
@@ -161,7 +185,6 @@ def cpu_baseline(box, chi32, S32, rays, weights, cu, seconds):
     """The oracle's short_char2 + integrate_intensity (OpenMP over frequency
     windows) on the host cores, on a sample of the workload's frequencies."""
     ncores = host_cores()
-    os.environ["OMP_NUM_THREADS"] = str(ncores)
     import numpy as np
     from oracle import oracle as O
     nfs = min(box.nf, 64)
@@ -195,18 +218,22 @@ def cpu_baseline(box, chi32, S32, rays, weights, cu, seconds):
         upd += nfs * nx * ny * nz
         if t_used >= seconds:
             break
-    return dict(value=upd / t_used, unit=UNIT, cores=ncores, kind="port",
+    return dict(value=upd / t_used, unit=UNIT, cores=ncores, kind="port", cpu_model=cpu_model(),
+                cpus_visible=cpus_visible(),
                 sample=f"{done} of {len(rays)} rays x {nfs} of {box.nf} frequencies on the full "
                        f"{nx}x{ny}x{nz} box ({upd:.3g} updates, {t_used:.1f} s); oracle short_char2 "
                        f"FP32 + integrate_intensity, OpenMP over {4 * ncores} frequency windows")
 
 
 def adjoint_leg(ctx, args):
-    """Adjoint / forward time on the abundance-fit workload (96 x 96 x 160, 24 rays,
-    2048 frequencies, 12 lines of 4 elements): the forward is the formal solution as
-    configured (24 sweeps, J and the emergent intensities); the adjoint is the
-    gradient of the flux chi^2 with respect to chi and S (the 12 upward rays, each
-    recomputed by the forward sweep and swept back), then to the 4 log abundances."""
+    """Gradient / forward cost on the abundance-fit workload (96 x 96 x 160, 24 rays,
+    2048 frequencies, 12 lines of 4 elements).  Forwards, one launch per ray and
+    nothing kept: the 12 upward rays' exit planes the flux loss needs, and the
+    24-ray formal solution with J.  Gradient of the flux chi^2: the checkpointed
+    forward of the 12 upward rays (patches of 23 planes, exit planes kept), the
+    backward (each patch recomputed from its checkpoint and swept in reverse), and
+    the chain to the 4 log abundances.  Checked against FP64 central differences
+    (h = 1e-3 dex) unless --no-adjoint-fd."""
     import torch
     from dispatch_amd.fit import FitProblem
     t0 = time.time()
@@ -216,30 +243,46 @@ def adjoint_leg(ctx, args):
     log(f"[bench] fit problem ready in {time.time() - t0:.1f}s")
     d0 = [0.0, 0.0, 0.0, 0.0]
     chi, S = p.cubes(d0)
-    fwd = []
-    for i in range(1 + max(1, args.steps // 2)):
-        torch.cuda.synchronize()
-        a = time.perf_counter()
-        with torch.no_grad():
+    full, flux = [], []
+    with torch.no_grad():
+        for i in range(1 + max(1, args.steps // 2)):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
             p.forward_full(chi, S)
-        torch.cuda.synchronize()
-        if i:
-            fwd.append(time.perf_counter() - a)
+            torch.cuda.synchronize()
+            b = time.perf_counter()
+            p.forward_flux(chi, S)
+            torch.cuda.synchronize()
+            if i:
+                full.append(b - a)
+                flux.append(time.perf_counter() - b)
     del chi, S
-    tm = {}
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    tms = []
     for i in range(2):
+        tm = {}
         loss, g, _ = p.loss_and_grad(d0, timing=tm)
-    t_fwd = min(fwd)
+        tms.append(tm)
+    tm = min(tms, key=lambda t: t["forward_s"] + t["adjoint_s"] + t["chain_s"])
+    t_flux, t_full = min(flux), min(full)
+    nup = len(p.up_rays())
+    grad_s = tm["forward_s"] + tm["adjoint_s"] + tm["chain_s"]
     out = {"workload": "abundance_fit", "box": [p.box.nx, p.box.ny, p.box.nz], "rays": len(p.vec),
-           "frequencies": p.box.nf, "lines": 12, "elements": 4,
-           "forward_ms": t_fwd * 1e3, "adjoint_sweeps_ms": tm["adjoint_s"] * 1e3,
-           "abundance_chain_ms": tm["chain_s"] * 1e3,
-           "adjoint_over_forward": tm["adjoint_s"] / t_fwd,
-           "gradient_over_forward": (tm["adjoint_s"] + tm["chain_s"]) / t_fwd,
-           "peak_hbm_GB": torch.cuda.max_memory_allocated() / 1e9,
+           "frequencies": p.box.nf, "lines": 12, "elements": 4, "patch_planes": p.rs.patch,
+           "forward_flux_ms": t_flux * 1e3, "forward_flux_rays": nup,
+           "forward_full_ms": t_full * 1e3, "forward_full_rays": len(p.vec),
+           "gradient_ms": grad_s * 1e3, "checkpointed_forward_ms": tm["forward_s"] * 1e3,
+           "backward_ms": tm["adjoint_s"] * 1e3, "abundance_chain_ms": tm["chain_s"] * 1e3,
+           "gradient_over_forward": grad_s / t_flux,
+           "backward_over_forward": tm["adjoint_s"] / t_flux,
+           "gradient_over_full_forward": grad_s / t_full,
+           "per_ray_ms": {"forward": t_flux * 1e3 / nup, "backward": tm["adjoint_s"] * 1e3 / nup},
+           "peak_hbm_GB_gradient": torch.cuda.max_memory_allocated() / 1e9,
            "loss": loss, "grad_log_abundance": [float(v) for v in g]}
-    log(f"[bench] adjoint {tm['adjoint_s'] * 1e3:.1f} ms vs forward {t_fwd * 1e3:.1f} ms")
-    if args.adjoint_fd:
+    log(f"[bench] gradient {grad_s * 1e3:.1f} ms (backward {tm['adjoint_s'] * 1e3:.1f}) vs forward "
+        f"{t_flux * 1e3:.1f} ms ({nup} rays)")
+    if not args.no_adjoint_fd:
         fd = []
         for e in range(4):
             dp, dm = list(d0), list(d0)
@@ -248,6 +291,8 @@ def adjoint_leg(ctx, args):
             fd.append((float(p.loss_of_flux(p.flux_fp64(dp))) - float(p.loss_of_flux(p.flux_fp64(dm)))) / 2e-3)
         out["fd_fp64_h1e-3"] = fd
         out["fd_max_rel_err"] = max(abs(float(a) - b) / abs(b) for a, b in zip(g, fd))
+        out["fd_tolerance"] = 1e-3
+        log(f"[bench] adjoint vs FP64 central differences: {out['fd_max_rel_err']:.2e}")
     del p
     return out
 
@@ -303,7 +348,128 @@ def batch1d_leg(ctx, args):
     return out
 
 
+def nlte_leg(ctx, args):
+    """The 3D NLTE ALI workload (60 x 60 x 120 box, 21-level atom with 80 lines and
+    20 continua, 16 rays x 120 frequencies per line): FP32 sweeps with Kahan-summed
+    rates, FP64 rate matrices and batched LU on the device, the reference's
+    automatic Ng trigger, iterated to max |dn/n| < 1e-4 from LTE.  With --nlte-fp64
+    the same iteration with the plain FP64 sweep beside it, and the departure
+    coefficients' largest relative difference between the two converged runs."""
+    import numpy as np
+    import torch
+    from dispatch_amd.ali import AliConfig, NlteProblem
+    cfg = AliConfig(max_iter=args.nlte_max_iter, graph=True)
+    t0 = time.perf_counter()
+    p = NlteProblem(ctx, cfg)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    n = p.solve(log=log)
+    its = [dt for _, dt in p.history]
+    out = {"workload": "3d_nlte_ali", "box": [cfg.nx, cfg.ny, cfg.nz], "levels": p.atom.nk,
+           "lines": p.atom.nline, "continua": len(p.atom.ti) - p.atom.nline, "rays": len(p.vec),
+           "frequencies_per_line": cfg.nq, "setup_s": setup, "iterations": n,
+           "converged": bool(abs(p.history[-1][0]) < cfg.conv), "iteration_bar": 40,
+           "s_per_iteration": float(np.median(its)), "first_iteration_s": its[0],
+           "maxerr_last": p.history[-1][0], "ng": "automatic (ng_step 0)",
+           "ng_fired": [i + 1 for i, h in enumerate(p.ng_history) if h],
+           "line_updates_per_iteration": p.ncell * p.atom.nline * cfg.nq * len(p.vec)}
+    out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
+    log(f"[bench] nlte: {n} iterations, {out['s_per_iteration']:.2f} s each")
+    if args.nlte_fp64:
+        b32 = p.b.clone()
+        del p
+        torch.cuda.empty_cache()
+        q = NlteProblem(ctx, cfg, sweep_dtype=torch.float64)
+        n64 = q.solve(log=log)
+        out["fp64_sweeps"] = {"iterations": n64, "s_per_iteration": float(np.median([dt for _, dt in q.history])),
+                              "b_max_rel_diff_fp32_vs_fp64": float((b32 / q.b - 1).abs().max()), "tolerance": 1e-4}
+        log(f"[bench] nlte fp64: {n64} iterations, b differs by {out['fp64_sweeps']['b_max_rel_diff_fp32_vs_fp64']:.2e}")
+        del q, b32
+    else:
+        del p
+    torch.cuda.empty_cache()
+    return out
+
+
+VOIGT_PROFILE = os.path.join(ROOT, "profiles", "line_opacity_equi_box120x120x230_nf1024_3lines.md")
+
+
+def voigt_leg(ctx, box):
+    """The cell x frequency Voigt summation of the box's three lines (QLG-bot
+    profiles, add_opacity_equi through add_linelist_opacities) over every cell and
+    frequency of the 120 x 120 x 230 x 1024 box: one species with the three lines
+    (5001, 5002, 5003 A; masses 56, 40, 24 u; centre line-to-continuum ratios 10,
+    10^2.5, 10^4; damping a = 0.02, 0.03, 0.05), densities proportional to chi500 so
+    the ratios hold at every depth, the continuum chi500 (0.7 + 0.3 (lam/lam0)^3)."""
+    import math
+    import numpy as np
+    import torch
+    from dispatch_amd import synth
+    from dispatch_amd.constants import BK, CC, UU
+    dev = "cuda"
+    nf, ncell = box.nf, box.nx * box.ny * box.nz
+    lam = torch.as_tensor(box.lam, dtype=torch.float64, device=dev)
+    llam = torch.log(lam * 1e8).contiguous()                       # ln(lambda / A)
+    T = box.T.reshape(-1).double()
+    c5 = box.chi500.reshape(-1).double()
+    cont = (c5[None, :] * (0.7 + 0.3 * (lam[:, None] / (5000e-8)) ** 3)).float().contiguous()
+    all_eta = torch.zeros_like(cont)
+    vmic = 1e5
+    lines = [(5001.0, 1e1, 0.02, 56.0), (5002.0, 10 ** 2.5, 0.03, 40.0), (5003.0, 1e4, 0.05, 24.0)]
+    isos = []
+    # one species, one isotope holding the three lines: n inv_df Rij = ratio * cont / voigt_peak
+    awgt = 40.0
+    inv_df = CC / torch.sqrt(2 * BK * T / (UU * awgt) + vmic ** 2)
+    d = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    vth_c = [math.sqrt(2 * BK * 5800.0 / (UU * m) + vmic ** 2) / CC for _, _, _, m in lines]
+    ll = dict(llam0=d([math.log(l0) for l0, _, _, _ in lines], torch.float64),
+              lower_ev=d([0.0] * 3, torch.float32),
+              gpref=d([math.log(r / 0.5641895835) for _, r, _, _ in lines], torch.float32),
+              grad=d([a * v for (_, _, a, _), v in zip(lines, vth_c)], torch.float32),
+              gw1=d([0.0] * 3, torch.float32), gw2=d([0.0] * 3, torch.float32), gstk=d([0.0] * 3, torch.float32),
+              unsold=d([1] * 3, torch.int32), lev_i=d([0] * 3, torch.int32), lev_j=d([0] * 3, torch.int32))
+    isos.append((ll, awgt, 1.0))
+    species = [dict(isotopes=isos, n=(c5 / inv_df).contiguous(), part=torch.zeros_like(c5), scale=1.0,
+                    scat_group=0)]
+    vz = box.vz.reshape(-1).double()
+    cells = dict(temp=T.float().contiguous(), vmic=torch.full((ncell,), vmic / 1e5, dtype=torch.float32, device=dev),
+                 vproj=(-vz / CC).float().contiguous(), ekt=(11604.518 / T).float().contiguous(),
+                 ne=torch.full((ncell,), 1e13, dtype=torch.float32, device=dev),
+                 tothi=torch.full((ncell,), 1e16, dtype=torch.float32, device=dev),
+                 spe=torch.full((ncell,), 1e13 * 5800 ** (1 / 6), dtype=torch.float32, device=dev))
+    all_chi = cont.clone()
+    ctx.add_linelist_opacities(species, llam, all_chi, all_eta, cells)      # warm-up (and scratch)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        all_chi.copy_(cont)
+        all_eta.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.add_linelist_opacities(species, llam, all_chi, all_eta, cells)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ratio = (all_chi / cont - 1.0)
+    pairs = 3 * nf * ncell
+    out = {"workload": "voigt_sum_3_lines", "cells": ncell, "frequencies": nf, "lines": 3,
+           "line_freq_cell_pairs": pairs, "ms": min(ts), "pairs_per_s": pairs / (min(ts) * 1e-3),
+           "centre_ratio_max": float(ratio.max()),
+           "deposited_fraction": float((all_eta > 0).float().mean())}
+    if os.path.exists(VOIGT_PROFILE):
+        for line in open(VOIGT_PROFILE):
+            if line.startswith("summary:"):
+                out["profile"] = line.split(":", 1)[1].strip() + f" (profiles/{os.path.basename(VOIGT_PROFILE)})"
+    log(f"[bench] voigt sum {min(ts):.1f} ms over {pairs:.3g} line-frequency-cell pairs")
+    del all_chi, all_eta, cont, ratio
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
+    # the oracle's OpenMP pool is sized when its library loads: fix the thread count
+    # before anything is imported
+    os.environ.setdefault("OMP_NUM_THREADS", str(host_cores()))
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -391,6 +557,10 @@ def main():
         res["output_pass"] = output_pass.run(nf=256, windows=1)
         log(f"[bench] output pass in {time.time() - t0:.1f}s: {res['output_pass']['ms_per_ray']}")
         torch.cuda.empty_cache()
+    if not args.no_voigt:
+        res["voigt"] = voigt_leg(ctx, box)
+    if not args.no_nlte:
+        res["nlte"] = nlte_leg(ctx, args)
     if not args.no_cpu:
         chi32, S32 = synth.build_cubes(box, torch.float32)
         res["cpu_baseline"] = cpu_baseline(box, chi32, S32, rays, weights, cu, args.cpu_seconds)

@@ -75,6 +75,8 @@ class AliConfig:
     ng_step: int = 0          # > 0: the reference's fixed schedule instead (&nlte_params ng_step,
     ng_start: int = 8         # ng_start; SRC/atom_mod.f90:278-287, 2500-2502); 1 or 2 is raised
                               # to 3 (SRC/m3d_patch_mod.f90:472-476)
+    sor_w: float = 1.0        # &nlte_params sor_w: the update factor e -> e**sor_w
+                              # (SRC/atom_mod.f90:255-266, 604-610); 1 is undamped
 
     def __post_init__(self):
         # Ng needs three iterates after an extrapolation before the next one
@@ -435,7 +437,7 @@ class NlteProblem:
     def update(self, gij, gji):
         """update_populations of every cell in place; returns the convergence measure."""
         return self.ctx.rate_solve(self.atom_dev, self.e_kT64, self.temp, self.totn, self.nstar, self.b, gij, gji,
-                                   self.cup)
+                                   self.cup, sor_w=self.cfg.sor_w)
 
     def iterate(self):
         """One ALI iteration: the rates, the statistical-equilibrium update and, when
@@ -497,11 +499,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch the rate pass eagerly every iteration")
     ap.add_argument("--ng-step", type=int, default=0, help="> 0: Ng on the reference's fixed schedule")
     ap.add_argument("--ng-start", type=int, default=8)
+    ap.add_argument("--sor-w", type=float, default=1.0, help="&nlte_params sor_w (1: undamped)")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
     cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph,
-                    ng_step=args.ng_step, ng_start=args.ng_start)
+                    ng_step=args.ng_step, ng_start=args.ng_start, sor_w=args.sor_w)
     t0 = time.perf_counter()
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
@@ -515,7 +518,7 @@ def main():
                line_updates_per_iteration=p.ncell * p.atom.nline * cfg.nq * len(p.vec),
                b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph,
                ng=("fixed schedule" if cfg.ng_step > 0 else "automatic") if cfg.ng else "off",
-               ng_step=cfg.ng_step, ng_start=cfg.ng_start,
+               ng_step=cfg.ng_step, ng_start=cfg.ng_start, sor_w=cfg.sor_w,
                ng_fired=[i + 1 for i, h in enumerate(p.ng_history) if h])
     out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
     if args.fp64_sweeps:

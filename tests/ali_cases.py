@@ -96,8 +96,9 @@ class OracleCtx:
         a, b = O.normalize_rates(atom, n(nstar), n(gij), n(gji))
         gij.copy_(torch.from_numpy(a)); gji.copy_(torch.from_numpy(b))
 
-    def rate_solve(self, atom, e_kT, temp, totn, nstar, b, gij, gji, cup):
-        bn, res = O.update_populations(atom, n(e_kT), n(temp), n(totn), n(nstar), n(b), n(gij), n(gji), n(cup))
+    def rate_solve(self, atom, e_kT, temp, totn, nstar, b, gij, gji, cup, **params):
+        bn, res = O.update_populations(atom, n(e_kT), n(temp), n(totn), n(nstar), n(b), n(gij), n(gji), n(cup),
+                                       **params)
         self.last = dict(gij=n(gij), gji=n(gji), b_in=n(b).copy())
         if res["status"] != 0:
             raise RuntimeError(f"rate solve status {res['status']} epos {res['epos']}")
