@@ -316,6 +316,8 @@ static int line_opacity_entry(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3
   if (!ll->llam0 || !ll->lower_ev || !ll->gpref || !ll->grad || !ll->gw1 || !ll->gw2 || !ll->gstk ||
       !ll->unsold || !ll->lev_i || !ll->lev_j)
     return set_error(M3DRT_EINVAL, w + "missing line-list array");
+  if ((ll->reach_in || ll->active) && !ll->sel_work)
+    return set_error(M3DRT_EINVAL, w + "a list with reaches or active flags needs sel_work (nline + 1 int32)");
   if (!o->llam || !o->vproj || !o->inv_df || !o->vbar2 || !o->e_kT || !o->ne || !o->tothi ||
       !o->stark_pe || !o->lnbase || !o->chi || !o->eta || !o->cont || !o->logref)
     return set_error(M3DRT_EINVAL, w + "missing cell array");

@@ -115,6 +115,9 @@ extern "C" int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist
       return set_error(M3DRT_EINVAL, w + "species " + std::to_string(s) + ": missing isotope list, n or part");
     if (sp.dep && sp.nlev < 1) return set_error(M3DRT_EINVAL, w + "species with departures needs nlev >= 1");
     if (sp.scat_group < 0 || sp.scat_group > 2) return set_error(M3DRT_EINVAL, w + "scat_group is 0, 1 or 2");
+    for (int k = 0; k < sp.niso; ++k)
+      if ((sp.iso[k].lines.reach_in || sp.iso[k].lines.active) && !sp.iso[k].lines.sel_work)
+        return set_error(M3DRT_EINVAL, w + "a list with reaches or active flags needs sel_work (nline + 1 int32)");
   }
   M3DRT_HIP(hipSetDevice(ctx->device));
   const int nf = a->nf;

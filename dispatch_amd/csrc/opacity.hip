@@ -39,6 +39,8 @@ struct OpacArgs {
   const int32_t *unsold, *lev_i, *lev_j, *ipart;
   int32_t *used;
   float *strength, *reach;
+  const float *reach_in;  // the list's reaches (has_reach), or NULL
+  const int32_t *sel;     // the selection: sel[0] lines, sel[1..] their 0-based indices; NULL: all
   int nf, ncell, npart, nlev, nlte, scat;
   const double *llam;
   const float *vproj, *idf, *vb2, *ekt, *ne, *tothi, *spe, *lnbase, *dep;
@@ -91,6 +93,101 @@ __device__ __forceinline__ int bisect_nearest(const double *llam, int ws, int we
   return lo;
 }
 
+// select_lines (HOP/linelist_mod.f90:1110-1142) in one workgroup: max |vproj| over
+// the call's cells and the list's largest reach (block maxima), the slice of line
+// centres within blue - reach_max .. red + reach_max (lower_bound8 / upper_bound8,
+// :1807-1844, on thread 0), then the active lines of the slice whose own reach
+// window touches blue..red, compacted in list order.  sel[0] = their count.
+__global__ void __launch_bounds__(1024) select_lines_kernel(const double *__restrict__ llam0,
+                                                            const float *__restrict__ reach_in,
+                                                            const int32_t *__restrict__ active, int nline,
+                                                            const double *__restrict__ llam, int nf,
+                                                            const float *__restrict__ vproj, int ncell,
+                                                            int32_t *__restrict__ sel) {
+  __shared__ float red_v[1024], red_r[1024];
+  __shared__ int scan[1024];
+  __shared__ int l0s, l1s, base;
+  __shared__ double blue_s, red_s;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  float v = 0.0f, r = -3.40282347e38f;
+  if (reach_in) {
+    for (int c = tid; c < ncell; c += nt) v = fmaxf(v, fabsf(vproj[c]));
+    for (int l = tid; l < nline; l += nt) r = fmaxf(r, reach_in[l]);
+  }
+  red_v[tid] = v;
+  red_r[tid] = r;
+  __syncthreads();
+  for (int h = nt / 2; h > 0; h /= 2) {
+    if (tid < h) {
+      red_v[tid] = fmaxf(red_v[tid], red_v[tid + h]);
+      red_r[tid] = fmaxf(red_r[tid], red_r[tid + h]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int l0 = 1, l1 = nline;
+    double blue = 0.0, redw = 0.0;
+    if (reach_in) {
+      double hb0 = 0.0, hb1 = 0.0;
+      if (nf > 1) {
+        hb0 = 0.5 * (llam[1] - llam[0]);
+        hb1 = 0.5 * (llam[nf - 1] - llam[nf - 2]);
+      }
+      const double vmax = (double)red_v[0];
+      blue = llam[0] - hb0 - vmax;
+      redw = llam[nf - 1] + hb1 + vmax;
+      const double rmax = (double)red_r[0];
+      // lower_bound8: first centre >= blue - rmax; upper_bound8: last <= red + rmax
+      int lo = 1, hi = nline + 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (llam0[mid - 1] >= blue - rmax) hi = mid;
+        else lo = mid + 1;
+      }
+      l0 = max(l0, lo);
+      lo = 1;
+      hi = nline + 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (llam0[mid - 1] > redw + rmax) hi = mid;
+        else lo = mid + 1;
+      }
+      l1 = min(l1, lo - 1);
+    }
+    l0s = l0;
+    l1s = l1;
+    blue_s = blue;
+    red_s = redw;
+    base = 0;
+  }
+  __syncthreads();
+  const int l0 = l0s, l1 = l1s;
+  for (int start = l0; start <= l1; start += nt) {
+    const int l = start + tid;  // 1-based
+    int keep = 0;
+    if (l <= l1) {
+      keep = !active || active[l - 1] != 0;
+      if (keep && reach_in) {
+        const double rr = (double)reach_in[l - 1];
+        keep = !(llam0[l - 1] + rr < blue_s || llam0[l - 1] - rr > red_s);
+      }
+    }
+    scan[tid] = keep;
+    __syncthreads();
+    for (int h = 1; h < nt; h *= 2) {  // inclusive scan
+      const int x = tid >= h ? scan[tid - h] : 0;
+      __syncthreads();
+      scan[tid] += x;
+      __syncthreads();
+    }
+    if (keep) sel[1 + base + scan[tid] - 1] = l - 1;
+    __syncthreads();
+    if (tid == nt - 1) base += scan[nt - 1];
+    __syncthreads();
+  }
+  if (tid == 0) sel[0] = base;
+}
+
 template <bool GENERAL>
 __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];  // TGX then TH2X
@@ -127,7 +224,9 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
 
   int f0_prev = -1, fc = 1;
   float refl_c = 0.0f, lref_c = 0.0f;
-  for (int l = 0; l < a.nline; ++l) {
+  const int nsel = a.sel ? a.sel[0] : a.nline;
+  for (int is = 0; is < nsel; ++is) {
+    const int l = a.sel ? a.sel[1 + is] : is;
     const double llc = a.llam0[l] - (double)vproj_c;
     const double pos = (llc - llam1) * inv_dllam;
     int f0;
@@ -143,21 +242,36 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     const int ip = a.ipart ? a.ipart[l] : 1;
     const float arg = a.lnbase[(int64_t)(ip - 1) * a.ncell + cidx] + a.gpref[l] - ex;
     if (arg < ex_cutoff) continue;
-    if (f0 != f0_prev) {
-      f0_prev = f0;
-      refl_c = a.tol * cont[(f0 - 1) * sf];
-      const float flo = fmaxf(a.tol, a.rej) * a.tol * chi[(f0 - 1) * sf];
-      if (flo > refl_c) {
-        refl_c = flo;
-        lref_c = (float)log((double)(flo * sqrt_pi));
-      } else {
-        lref_c = logref[(f0 - 1) * sf];
+    // a line centred beyond the grid, in a list with reaches: the reference is the
+    // continuum at the nearer grid end, without the running floor (:594-602, 844-851)
+    const bool outside = a.reach_in && (llc < a.llam[0] || llc > a.llam[nf - 1]);
+    float Rij, ref, lnq;
+    int fe = 1;
+    if (outside) {
+      fe = llc < a.llam[0] ? 1 : nf;
+      const float lref_o = logref[(fe - 1) * sf];
+      if (arg <= lref_o) continue;
+      Rij = (float)exp((double)arg);
+      ref = a.tol * cont[(fe - 1) * sf];
+      lnq = arg - lref_o;
+    } else {
+      if (f0 != f0_prev) {
+        f0_prev = f0;
+        refl_c = a.tol * cont[(f0 - 1) * sf];
+        const float flo = fmaxf(a.tol, a.rej) * a.tol * chi[(f0 - 1) * sf];
+        if (flo > refl_c) {
+          refl_c = flo;
+          lref_c = (float)log((double)(flo * sqrt_pi));
+        } else {
+          lref_c = logref[(f0 - 1) * sf];
+        }
       }
+      if (arg <= lref_c) continue;
+      Rij = (float)exp((double)arg);
+      ref = refl_c;
+      if (voigt_peak * Rij <= ref) continue;
+      lnq = arg - lref_c;
     }
-    if (arg <= lref_c) continue;
-    const float Rij = (float)exp((double)arg);
-    const float ref = refl_c;
-    if (voigt_peak * Rij <= ref) continue;
 
     const bool nlte = a.nlte && a.lev_i[l] > 0 && a.lev_j[l] > 0;
     float bi = 1.0f, bj = 1.0f;
@@ -176,23 +290,37 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     else
       gam = gam - a.gstk[l] * starkpe;
     if (!GENERAL) gam = gam * sdl;
-    if (a.strength) atomic_max_pos(&a.strength[l], voigt_peak * Rij / cont[(f0 - 1) * sf]);
-    const float lnq = arg - lref_c;
+    if (!outside && a.strength) atomic_max_pos(&a.strength[l], voigt_peak * Rij / cont[(f0 - 1) * sf]);
     const bool plain = !(a.scat && !nlte);
     if (GENERAL) {
       // qlg_bot_line on the ln(lambda) abscissa; the window by bisection unless the
-      // line covers the grid (HOP/linelist_mod.f90:660-705)
+      // line covers the grid (HOP/linelist_mod.f90:660-705); a line beyond the grid
+      // keeps the part of its window (capped by its reach) that reaches the grid
       const Coef c = bot_setup(gam, D, SQ, Rij * D);
       const float AL = 0.3183098862f * gam * (Rij * D) / ref;
       const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
                                          AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
-      if (a.used) a.used[l] = 1;
-      if (a.reach) atomic_max_pos(&a.reach[l], hw);
-      const double dl = (double)hw;
       int fa = 1, fb = nf;
-      if (!(dl >= fmax(a.llam[nf - 1] - llc, llc - a.llam[0]))) {
-        fa = bisect_nearest(a.llam, 1, f0, llc - dl);
-        fb = bisect_nearest(a.llam, f0, nf, llc + dl);
+      if (outside) {
+        if (a.reach) atomic_max_pos(&a.reach[l], hw);
+        const double dl = fmin((double)hw, (double)a.reach_in[l]);
+        double hb0 = 0.0, hb1 = 0.0;  // select_lines' half steps at the grid ends
+        if (nf > 1) {
+          hb0 = 0.5 * (a.llam[1] - a.llam[0]);
+          hb1 = 0.5 * (a.llam[nf - 1] - a.llam[nf - 2]);
+        }
+        if (llc + dl < a.llam[0] - hb0 || llc - dl > a.llam[nf - 1] + hb1) continue;
+        if (a.used) a.used[l] = 1;
+        fa = bisect_nearest(a.llam, 1, nf, llc - dl);
+        fb = bisect_nearest(a.llam, 1, nf, llc + dl);
+      } else {
+        if (a.used) a.used[l] = 1;
+        if (a.reach) atomic_max_pos(&a.reach[l], hw);
+        const double dl = (double)hw;
+        if (!(dl >= fmax(a.llam[nf - 1] - llc, llc - a.llam[0]))) {
+          fa = bisect_nearest(a.llam, 1, f0, llc - dl);
+          fb = bisect_nearest(a.llam, f0, nf, llc + dl);
+        }
       }
       for (int f = fa; f <= fb; ++f) {
         const float x = (float)(a.llam[f - 1] - llc);
@@ -228,7 +356,9 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
                                        AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
     if (a.reach) atomic_max_pos(&a.reach[l], (float)((double)hw * dllam));
-    const int noff = nint(fminf(fminf(hw, 3.40282347e38f), (float)(nf + abs(ic))));
+    // the window of a line beyond the grid is capped by its reach (:914-921)
+    const float cap = outside ? (float)((double)a.reach_in[l] * inv_dllam) : 3.40282347e38f;
+    const int noff = nint(fminf(fminf(hw, cap), (float)(nf + abs(ic))));
     const int fa = max(1, ic - noff);
     const int fb = min(nf, ic + noff);
     if (fa > fb) continue;
@@ -345,6 +475,14 @@ int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_a
   a.gw1 = ll->gw1; a.gw2 = ll->gw2; a.gstk = ll->gstk;
   a.unsold = ll->unsold; a.lev_i = ll->lev_i; a.lev_j = ll->lev_j; a.ipart = ll->ipart;
   a.used = ll->used; a.strength = ll->strength; a.reach = ll->reach;
+  a.reach_in = ll->reach_in;
+  a.sel = nullptr;
+  if (ll->reach_in || ll->active) {
+    hipLaunchKernelGGL(select_lines_kernel, dim3(1), dim3(1024), 0, ctx->stream, ll->llam0, ll->reach_in,
+                       ll->active, ll->nline, o->llam, o->nf, o->vproj, o->ncell, ll->sel_work);
+    M3DRT_HIP(hipGetLastError());
+    a.sel = ll->sel_work;
+  }
   a.nf = o->nf; a.ncell = o->ncell; a.npart = o->npart; a.nlev = o->nlev;
   a.nlte = (o->flags & M3DRT_OPAC_NLTE) ? 1 : 0;
   a.scat = (o->flags & M3DRT_OPAC_SCAT) ? 1 : 0;

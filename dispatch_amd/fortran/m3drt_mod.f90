@@ -92,6 +92,9 @@ module m3drt
     type(c_ptr) :: used = c_null_ptr
     type(c_ptr) :: strength = c_null_ptr
     type(c_ptr) :: reach = c_null_ptr
+    type(c_ptr) :: reach_in = c_null_ptr
+    type(c_ptr) :: active = c_null_ptr
+    type(c_ptr) :: sel_work = c_null_ptr
   end type m3drt_linelist
 
   type, bind(C), public :: m3drt_opacity_args

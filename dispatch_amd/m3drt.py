@@ -333,7 +333,7 @@ M3DRT_OPAC_NLTE, M3DRT_OPAC_SCAT = 1, 2
 class Linelist(C.Structure):
     _fields_ = [("nline", C.c_int32)] + [(n, C.c_void_p) for n in (
         "llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk", "unsold", "lev_i", "lev_j",
-        "ipart", "used", "strength", "reach")]
+        "ipart", "used", "strength", "reach", "reach_in", "active", "sel_work")]
 
 
 class OpacityArgs(C.Structure):
@@ -354,12 +354,27 @@ def _opacity_lib():
     return L
 
 
+def _sel_work(lines, nl):
+    """The selection scratch a list with reaches or active flags needs (nline + 1
+    int32 on the device), else None; reach_in float32 and active int32 of nline."""
+    ri, act = lines.get("reach_in"), lines.get("active")
+    if ri is None and act is None:
+        return None
+    if ri is not None and (ri.dtype != torch.float32 or ri.numel() != nl or not ri.is_cuda):
+        raise M3drtError("reach_in is a float32 device tensor of nline values")
+    if act is not None and (act.dtype != torch.int32 or act.numel() != nl or not act.is_cuda):
+        raise M3drtError("active is an int32 device tensor of nline values")
+    dev = (ri if ri is not None else act).device
+    return torch.empty(nl + 1, dtype=torch.int32, device=dev)
+
+
 def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None,
                        nlte=False, one_m_eps=None, chi_scat=None, layout="native", tolerance=1e-4,
                        line_reject_frac=0.0, used=None, strength=None, reach=None, general=False):
     """add_opacity_equi of one isotope's line list on the device.  lines: dict of
     device tensors (llam0 float64; lower_ev, gpref, grad, gw1, gw2, gstk float32;
-    unsold, lev_i, lev_j, optional ipart int32).  Spectra (chi, eta, cont, logref,
+    unsold, lev_i, lev_j, optional ipart int32; optional reach_in float32 and active
+    int32: select_lines with the list's reaches and active flags).  Spectra (chi, eta, cont, logref,
     one_m_eps, chi_scat) are (nf, ncell) in the native layout (cells at unit
     stride) or (ncell, nf) in the reference layout (f at unit stride).  general: the
     grid's spacing may vary (add_opacity, m3drt_line_opacity)."""
@@ -381,9 +396,10 @@ def _line_opacity_equi(self, lines, llam, chi, eta, cont, logref, cells, lnbase,
         raise M3drtError("llam is float64 with nf nodes")
     npart = lnbase.shape[0]
     p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    sel = _sel_work(lines, nl)
     ll = Linelist(nl, *[p(lines.get(k)) for k in ("llam0", "lower_ev", "gpref", "grad", "gw1", "gw2",
                                                   "gstk", "unsold", "lev_i", "lev_j", "ipart")],
-                  p(used), p(strength), p(reach))
+                  p(used), p(strength), p(reach), p(lines.get("reach_in")), p(lines.get("active")), p(sel))
     flags = (M3DRT_OPAC_NLTE if nlte else 0) | (M3DRT_OPAC_SCAT if chi_scat is not None else 0)
     o = OpacityArgs(nf, ncell, npart, 0 if dep is None else dep.shape[0], flags, tolerance,
                     line_reject_frac, sf, sc, p(llam), *[p(cells[k]) for k in (
@@ -1259,9 +1275,12 @@ def _add_linelist_opacities(self, species, llam, all_chi, all_eta, cells, equidi
         isos = (Isotope * max(1, len(sp["isotopes"])))()
         for k, (lines, awgt, frac) in enumerate(sp["isotopes"]):
             nl = int(lines["llam0"].numel())
+            sel = _sel_work(lines, nl)
+            keep.append(sel)
             isos[k] = Isotope(Linelist(nl, *[p(lines.get(n)) for n in (
                 "llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk", "unsold", "lev_i", "lev_j", "ipart")],
-                None, None, None), float(awgt), float(frac))
+                None, None, None, p(lines.get("reach_in")), p(lines.get("active")), p(sel)), float(awgt),
+                float(frac))
         keep.append(isos)
         dep = sp.get("dep")
         sps[s] = Species(len(sp["isotopes"]), C.cast(isos, C.c_void_p), p(sp["n"]), float(sp.get("scale", 1.0)),

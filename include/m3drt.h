@@ -175,6 +175,19 @@ typedef struct {
   const int32_t *ipart;               /* 1-based population group of each line; NULL: 1 */
   int32_t *used;                      /* out (may be NULL): set where the line deposits */
   float *strength, *reach;            /* out (may be NULL): maxima over the cells */
+  /* select_lines (HOP/linelist_mod.f90:1110-1142).  reach_in: every line's reach in
+   * ln(lambda) (the list's has_reach: pass it only when every line carries one, as
+   * finalize_linelist sets it, :375-378); then the lines are narrowed by bisection on
+   * the centres with the largest reach as margin and compared one by one with their
+   * own, each margin widened by max |vproj| of the call's cells and half a grid step,
+   * and a line centred beyond the grid takes its tolerance reference at the nearer
+   * grid end with its window capped by its reach (:594-602, 660-667, 844-858,
+   * 914-921).  active: nonzero for the lines to visit (NULL: all).  sel_work: device
+   * scratch of nline + 1 int32 the selection is written to, required with reach_in
+   * or active. */
+  const float *reach_in;
+  const int32_t *active;
+  int32_t *sel_work;
 } m3drt_linelist;
 
 /* The cells and spectra of one call.  (f, cell) arrays are addressed f*sf + cell*sc

@@ -223,11 +223,75 @@ float orc_qlg_bot_profile(float a, float q, float vpr, float idf)
   return K * ISQPIf * idf * (q + 1.0f);
 }
 
+/* lower_bound8 / upper_bound8, HOP/linelist_mod.f90:1807-1844: the first index
+ * (1-based) with a(i) >= x (n+1 if none), the last with a(i) <= x (0 if none). */
+static int lower_bound8(const double *a, int n, double x)
+{
+  int lo = 1, hi = n + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (a[mid - 1] >= x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+static int upper_bound8(const double *a, int n, double x)
+{
+  int lo = 1, hi = n + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (a[mid - 1] > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo - 1;
+}
+
+void orc_select_lines(const orc_linelist *ll, int nf, const double *llam, double vmax, int *l0, int *l1,
+                      int32_t *sel, int *nsel, double hb[2])
+{
+  double blue = 0, red = 0;
+  hb[0] = hb[1] = 0;
+  if (nf > 1) {
+    hb[0] = 0.5 * (llam[1] - llam[0]);
+    hb[1] = 0.5 * (llam[nf - 1] - llam[nf - 2]);
+  }
+  if (ll->reach_in) {
+    float rmax = ll->reach_in[0];
+    for (int l = 1; l < ll->nline; l++) rmax = ll->reach_in[l] > rmax ? ll->reach_in[l] : rmax;
+    blue = llam[0] - hb[0] - vmax;
+    red = llam[nf - 1] + hb[1] + vmax;
+    const int a = lower_bound8(ll->llam0, ll->nline, blue - (double)rmax);
+    const int b = upper_bound8(ll->llam0, ll->nline, red + (double)rmax);
+    *l0 = *l0 > a ? *l0 : a;
+    *l1 = *l1 < b ? *l1 : b;
+  }
+  *nsel = 0;
+  for (int l = *l0; l <= *l1; l++) {
+    if (ll->active && !ll->active[l - 1]) continue;
+    if (ll->reach_in) {
+      const double r = (double)ll->reach_in[l - 1];
+      if (ll->llam0[l - 1] + r < blue || ll->llam0[l - 1] - r > red) continue;
+    }
+    sel[(*nsel)++] = l;
+  }
+}
+
+/* max |vproj| over the call's cells, as real(4), widened (select_lines' vmax) */
+static double vproj_max(const float *vproj, int ncell)
+{
+  float v = 0.0f;
+  for (int c = 0; c < ncell; c++) v = fabsf(vproj[c]) > v ? fabsf(vproj[c]) : v;
+  return (double)v;
+}
+
 /*
  * add_opacity_equi, HOP/linelist_mod.f90:729-960, for one isotope's line list over
- * ncell cells (the cell loop flattened, Fortran order) without chunking, reaches or
- * per-line windows.  chi, eta, cont, logref, one_m_eps, chi_scat are (nf, ncell);
- * lnbase (ncell, np); dep (ncell, nlev).  Lines are visited in the list's order.
+ * ncell cells (the cell loop flattened, Fortran order) without chunking.
+ * chi, eta, cont, logref, one_m_eps, chi_scat are (nf, ncell); lnbase (ncell, np);
+ * dep (ncell, nlev).  The lines of select_lines are visited in the list's order; with
+ * reaches, a line centred beyond the grid takes its reference at the nearer grid end
+ * and its window is capped by its reach (:844-858, 914-921).
  */
 void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, int ncell,
                           float *chi, float *eta, const float *cont, const float *logref,
@@ -245,6 +309,10 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
   if (nl == 0 || nf < 2) return;
   orc_qlg_init();
   Kbuf = (float *)malloc(sizeof(float) * nf);
+  int32_t *sel = (int32_t *)malloc(sizeof(int32_t) * (nl > 0 ? nl : 1));
+  int l0 = 1, l1 = nl, nsel;
+  double hb[2];
+  orc_select_lines(ll, nf, llam, vproj_max(vproj, ncell), &l0, &l1, sel, &nsel, hb);
   llam1 = llam[0];
   dllam = (llam[nf - 1] - llam[0]) / (double)(nf - 1);
   inv_dllam = 1.0 / dllam;
@@ -261,7 +329,8 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
     const float lvb2 = logf(vbar2[cidx]);
     const float ekt = e_kT[cidx], ne_c = ne[cidx], tothi_c = tothi[cidx];
     const float starkpe = stark_pe[cidx], vproj_c = vproj[cidx];
-    for (int l = 0; l < nl; l++) {
+    for (int is = 0; is < nsel; is++) {
+      const int l = sel[is] - 1;
       const double llc = ll->llam0[l] - (double)vproj_c;
       const double pos = (llc - llam1) * inv_dllam;
       int f0 = nint_d(pos) + 1;
@@ -269,24 +338,33 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
       const float ex = ll->lower_ev[l] * ekt;
       const int ip = ll->ipart ? ll->ipart[l] : 1;
       const float arg = lnbase[cidx + (size_t)ncell * (ip - 1)] + ll->gpref[l] - ex;
-      float Rij, ref, a, hw, cap, lnq, bi, bj;
+      float Rij, ref, a, hw, cap, lnq, bi, bj, lref_o = 0;
       int fa, fb, nlte;
       if (arg < ex_cutoff) continue;
-      if (f0 != f0_prev) {
-        f0_prev = f0;
-        refl_c = tol * contc[f0 - 1];
-        const float flo = fmaxf(tol, line_reject_frac) * tol * chic[f0 - 1];
-        if (flo > refl_c) {
-          refl_c = flo;
-          lref_c = logf(flo * sqrt_pi);
-        } else {
-          lref_c = lrc[f0 - 1];
+      const int outside = ll->reach_in && (llc < llam[0] || llc > llam[nf - 1]);
+      if (outside) {
+        const int fe = llc < llam[0] ? 1 : nf;
+        lref_o = lrc[fe - 1];
+        if (arg <= lref_o) continue;
+        Rij = expf(arg);
+        ref = tol * contc[fe - 1];
+      } else {
+        if (f0 != f0_prev) {
+          f0_prev = f0;
+          refl_c = tol * contc[f0 - 1];
+          const float flo = fmaxf(tol, line_reject_frac) * tol * chic[f0 - 1];
+          if (flo > refl_c) {
+            refl_c = flo;
+            lref_c = logf(flo * sqrt_pi);
+          } else {
+            lref_c = lrc[f0 - 1];
+          }
         }
+        if (arg <= lref_c) continue;
+        Rij = expf(arg);
+        ref = refl_c;
+        if (voigt_peak * Rij <= ref) continue;
       }
-      if (arg <= lref_c) continue;
-      Rij = expf(arg);
-      ref = refl_c;
-      if (voigt_peak * Rij <= ref) continue;
       nlte = do_nlte && ll->lev_i[l] > 0 && ll->lev_j[l] > 0;
       if (nlte) {
         bi = dep[cidx + (size_t)ncell * (ll->lev_i[l] - 1)];
@@ -305,12 +383,15 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
       else
         a = a - ll->gstk[l] * starkpe;
       a = a * sdl;
-      {
+      if (outside) {
+        cap = (float)((double)ll->reach_in[l] * inv_dllam);
+        lnq = arg - lref_o;
+      } else {
         const float st = voigt_peak * Rij / contc[f0 - 1];
         if (st > ll->strength[l]) ll->strength[l] = st;
+        cap = 3.40282347e38f;
+        lnq = arg - lref_c;
       }
-      cap = 3.40282347e38f;
-      lnq = arg - lref_c;
       orc_qlg_bot_line_sweep(&qw, a, Rij * qw.D, ref, lnq, nf, llam, llc, pos, inv_dllam, cap,
                              &fa, &fb, &hw, Kbuf);
       {
@@ -338,6 +419,7 @@ void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, in
     }
   }
   free(Kbuf);
+  free(sel);
 }
 
 /* bisect_nearest, HOP/linelist_mod.f90:1075-1095: the first index of llam(ws:we)
@@ -377,6 +459,10 @@ void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int nce
   (void)np;
   if (nl == 0 || nf < 1) return;
   orc_qlg_init();
+  int32_t *sel = (int32_t *)malloc(sizeof(int32_t) * nl);
+  int l0 = 1, l1 = nl, nsel;
+  double hb[2];
+  orc_select_lines(ll, nf, llam, vproj_max(vproj, ncell), &l0, &l1, sel, &nsel, hb);
   for (int cidx = 0; cidx < ncell; cidx++) {
     float *chic = chi + (size_t)nf * cidx, *etac = eta + (size_t)nf * cidx;
     const float *contc = cont + (size_t)nf * cidx, *lrc = logref + (size_t)nf * cidx;
@@ -390,7 +476,8 @@ void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int nce
     const float lvb2 = logf(vbar2[cidx]);
     const float ekt = e_kT[cidx], ne_c = ne[cidx], tothi_c = tothi[cidx];
     const float starkpe = stark_pe[cidx], vproj_c = vproj[cidx];
-    for (int l = 0; l < nl; l++) {
+    for (int is = 0; is < nsel; is++) {
+      const int l = sel[is] - 1;
       const double llc = ll->llam0[l] - (double)vproj_c;
       while (fc < nf && !(llam[fc - 1] >= llc)) fc++;
       int f0;
@@ -399,24 +486,33 @@ void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int nce
       const float ex = ll->lower_ev[l] * ekt;
       const int ip = ll->ipart ? ll->ipart[l] : 1;
       const float arg = lnbase[cidx + (size_t)ncell * (ip - 1)] + ll->gpref[l] - ex;
-      float Rij, ref, a, hw, bi, bj;
+      float Rij, ref, a, hw, bi, bj, lref_o = 0;
       int fa, fb, nlte;
       if (arg < ex_cutoff) continue;
-      if (f0 != f0_prev) {
-        f0_prev = f0;
-        refl_c = tol * contc[f0 - 1];
-        const float flo = fmaxf(tol, line_reject_frac) * tol * chic[f0 - 1];
-        if (flo > refl_c) {
-          refl_c = flo;
-          lref_c = logf(flo * sqrt_pi);
-        } else {
-          lref_c = lrc[f0 - 1];
+      const int outside = ll->reach_in && (llc < llam[0] || llc > llam[nf - 1]);
+      if (outside) {
+        const int fe = llc < llam[0] ? 1 : nf;
+        lref_o = lrc[fe - 1];
+        if (arg <= lref_o) continue;
+        Rij = expf(arg);
+        ref = tol * contc[fe - 1];
+      } else {
+        if (f0 != f0_prev) {
+          f0_prev = f0;
+          refl_c = tol * contc[f0 - 1];
+          const float flo = fmaxf(tol, line_reject_frac) * tol * chic[f0 - 1];
+          if (flo > refl_c) {
+            refl_c = flo;
+            lref_c = logf(flo * sqrt_pi);
+          } else {
+            lref_c = lrc[f0 - 1];
+          }
         }
+        if (arg <= lref_c) continue;
+        Rij = expf(arg);
+        ref = refl_c;
+        if (voigt_peak * Rij <= ref) continue;
       }
-      if (arg <= lref_c) continue;
-      Rij = expf(arg);
-      ref = refl_c;
-      if (voigt_peak * Rij <= ref) continue;
       nlte = do_nlte && ll->lev_i[l] > 0 && ll->lev_j[l] > 0;
       if (nlte) {
         bi = dep[cidx + (size_t)ncell * (ll->lev_i[l] - 1)];
@@ -434,15 +530,24 @@ void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int nce
         a = a + ll->gstk[l] * ne_c;
       else
         a = a - ll->gstk[l] * starkpe;
-      ll->used[l] = 1;
-      {
-        const float st = voigt_peak * Rij / contc[f0 - 1];
-        if (st > ll->strength[l]) ll->strength[l] = st;
-      }
       orc_qlg_coef qc;
-      orc_qlg_bot_line(&qw, a, Rij * qw.D, ref, arg - lref_c, &qc, &hw);
-      if (hw > ll->reach[l]) ll->reach[l] = hw;
-      {
+      if (outside) {
+        orc_qlg_bot_line(&qw, a, Rij * qw.D, ref, arg - lref_o, &qc, &hw);
+        double dl = (double)hw;
+        if (hw > ll->reach[l]) ll->reach[l] = hw;
+        dl = fmin(dl, (double)ll->reach_in[l]);
+        if (llc + dl < llam[0] - hb[0] || llc - dl > llam[nf - 1] + hb[1]) continue;
+        ll->used[l] = 1;
+        fa = bisect_nearest(llam, ws, we, llc - dl);
+        fb = bisect_nearest(llam, ws, we, llc + dl);
+      } else {
+        ll->used[l] = 1;
+        {
+          const float st = voigt_peak * Rij / contc[f0 - 1];
+          if (st > ll->strength[l]) ll->strength[l] = st;
+        }
+        orc_qlg_bot_line(&qw, a, Rij * qw.D, ref, arg - lref_c, &qc, &hw);
+        if (hw > ll->reach[l]) ll->reach[l] = hw;
         const double dl = (double)hw;
         const double span = fmax(llam[we - 1] - llc, llc - llam[ws - 1]);
         if (dl >= span) {
@@ -468,4 +573,5 @@ void orc_add_opacity(const orc_linelist *ll, int nf, const double *llam, int nce
       f0_prev = -1;
     }
   }
+  free(sel);
 }

@@ -33,7 +33,18 @@ typedef struct {
   const int32_t *unsold, *lev_i, *lev_j, *ipart;  /* ipart NULL: every line in group 1 */
   int32_t *used;
   float *strength, *reach;
+  /* the list's reaches (has_reach: every line carries one, finalize_linelist,
+   * HOP/linelist_mod.f90:375-378) and active flags; NULL: none / every line */
+  const float *reach_in;
+  const int32_t *active;
 } orc_linelist;
+
+/* select_lines (HOP/linelist_mod.f90:1110-1142): the active lines of l0..l1
+ * (1-based, narrowed in place) whose reach window can touch llam(1:nf), into sel
+ * (1-based, nsel of them); hb the half grid steps at the two ends.  vmax is
+ * max |vproj| over the call's cells. */
+void orc_select_lines(const orc_linelist *ll, int nf, const double *llam, double vmax, int *l0, int *l1,
+                      int32_t *sel, int *nsel, double hb[2]);
 
 void orc_add_opacity_equi(const orc_linelist *ll, int nf, const double *llam, int ncell,
                           float *chi, float *eta, const float *cont, const float *logref,

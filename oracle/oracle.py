@@ -233,7 +233,7 @@ class Linelist(C.Structure):
     _fields_ = [("nline", C.c_int), ("llam0", C.c_void_p)] + \
                [(n, C.c_void_p) for n in ("lower_ev", "gpref", "grad", "gw1", "gw2", "gstk",
                                           "unsold", "lev_i", "lev_j", "ipart", "used", "strength",
-                                          "reach")]
+                                          "reach", "reach_in", "active")]
 
 
 def _qlg_declare():
@@ -315,7 +315,8 @@ def add_opacity(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None, do
 def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=None, do_nlte=False,
                      one_m_eps=None, chi_scat=None, tol=1e-4, line_reject_frac=0.0, _general=False):
     """Restated add_opacity_equi.  lines: dict of SoA arrays (llam0, lower_ev, gpref,
-    grad, gw1, gw2, gstk, unsold, lev_i, lev_j, optional ipart); chi/eta/cont/logref
+    grad, gw1, gw2, gstk, unsold, lev_i, lev_j, optional ipart, optional reach_in (every
+    line's reach: the list's has_reach) and active); chi/eta/cont/logref
     (ncell, nf) float32, updated copies returned; cells: dict of per-cell arrays
     (vproj, idf, vb2, ekt, ne, tothi, spe); lnbase (np, ncell); dep (nlev, ncell)."""
     L = _qlg_declare()
@@ -335,7 +336,9 @@ def add_opacity_equi(lines, llam, chi, eta, cont, logref, cells, lnbase, dep=Non
                   *[arr(lines[k], np.int32) for k in ("unsold", "lev_i", "lev_j")],
                   arr(lines["ipart"], np.int32) if lines.get("ipart") is not None else None,
                   used.ctypes.data_as(C.c_void_p), strength.ctypes.data_as(C.c_void_p),
-                  reach.ctypes.data_as(C.c_void_p))
+                  reach.ctypes.data_as(C.c_void_p),
+                  arr(lines["reach_in"], np.float32) if lines.get("reach_in") is not None else None,
+                  arr(lines["active"], np.int32) if lines.get("active") is not None else None)
     chi = np.array(chi, np.float32, copy=True, order="C")
     eta = np.array(eta, np.float32, copy=True, order="C")
     ncell, nf = chi.shape

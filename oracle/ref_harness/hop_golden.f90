@@ -131,7 +131,8 @@ contains
     logical, intent(in) :: general
     type(linelist_type) :: ll
     integer :: nl, nf, m(3), np, nlev, flags, l
-    logical :: do_nlte, has_ipart, do_scat
+    logical :: do_nlte, has_ipart, do_scat, has_reach
+    integer, allocatable :: active_i(:)
     real(8), allocatable :: llam(:)
     real(4), allocatable :: chi(:,:,:,:), eta(:,:,:,:), cont(:,:,:,:), logref(:,:,:,:)
     real(4), allocatable :: vproj(:,:,:), idf(:,:,:), vb2(:,:,:), ekt(:,:,:), ne(:,:,:)
@@ -142,6 +143,7 @@ contains
     do_nlte = iand(flags, 1) /= 0
     has_ipart = iand(flags, 2) /= 0
     do_scat = iand(flags, 4) /= 0
+    has_reach = iand(flags, 8) /= 0
     ll%nline = nl
     allocate(ll%llam0(nl), ll%lower_ev(nl), ll%gpref(nl), ll%grad(nl), ll%gw1(nl), ll%gw2(nl), &
              ll%gstk(nl), ll%unsold(nl), ll%lev_i(nl), ll%lev_j(nl), ll%active(nl), ll%used(nl), &
@@ -155,6 +157,15 @@ contains
     ll%reach = 0
     ll%reach_in = -1
     ll%has_reach = .false.
+    if (has_reach) then
+      ! a list whose every line carries a reach, as finalize_linelist marks it
+      ! (HOP/linelist_mod.f90:375-378), and its active flags
+      allocate(active_i(nl))
+      read(ui) ll%reach_in, active_i
+      ll%active = active_i /= 0
+      ll%has_reach = all(ll%reach_in(1:nl) >= 0)
+      if (ll%has_reach) ll%reach_in_max = maxval(ll%reach_in(1:nl))
+    end if
     if (has_ipart) then
       allocate(ll%ipart(nl))
       ll%ipart = ipart

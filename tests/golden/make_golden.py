@@ -141,7 +141,11 @@ def gen_lineq(rng):
 
 def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4, general=False):
     """general: add_opacity on a grid whose steps vary from 0.3 to 1.7 of the mean,
-    else add_opacity_equi on an equidistant grid."""
+    else add_opacity_equi on an equidistant grid.  flags & 8: a list with reaches
+    (has_reach) and active flags: the reaches are those a first run without them
+    reports, each scaled by 0.3-1.5 (so some windows are capped and some lines are
+    not selected), a small one for lines that run did not reach; one line in six is
+    inactive."""
     ncell = int(np.prod(m))
     dl = np.log(5010 / 5000) / (nf - 1)
     if general:
@@ -179,25 +183,38 @@ def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4, gene
     spe = (ne * 3.0).astype(np.float32)
     lnbase = rng.uniform(20, 36, (np_, ncell)).astype(np.float32)
     dep = rng.uniform(0.5, 2.0, (nlev, ncell)).astype(np.float32)
-    hdr = i32(nl, nf, *m, np_, nlev, flags)
-    payload = hdr + llam0.tobytes()
-    for a in (lower_ev, gpref, grad, gw1, gw2, gstk):
-        payload += a.tobytes()
-    for a in (unsold, lev_i, lev_j, ipart):
-        payload += a.tobytes()
-    payload += llam.tobytes()
-    for a in (chi, eta, cont, logref):
-        payload += a.T.tobytes()       # (nf, ncell) in Fortran order = (ncell, nf) C
-    for a in (vproj, idf, vb2, ekt, ne, tothi, spe):
-        payload += a.tobytes()
-    payload += lnbase.tobytes() + dep.tobytes()   # (ncell, np) Fortran = (np, ncell) C
     ome = csc = None
     if flags & 4:
         ome = rng.uniform(0, 1, (nf, ncell)).astype(np.float32)
         csc = (cont * 0.1).astype(np.float32)
-        payload += ome.T.tobytes() + csc.T.tobytes()
-    out = run("opac_gen" if general else "opac_equi", payload)
+
+    def payload_of(fl, reach_in=None, active=None):
+        pl = i32(nl, nf, *m, np_, nlev, fl) + llam0.tobytes()
+        for a in (lower_ev, gpref, grad, gw1, gw2, gstk):
+            pl += a.tobytes()
+        for a in (unsold, lev_i, lev_j, ipart):
+            pl += a.tobytes()
+        if fl & 8:
+            pl += reach_in.tobytes() + active.tobytes()
+        pl += llam.tobytes()
+        for a in (chi, eta, cont, logref):
+            pl += a.T.tobytes()       # (nf, ncell) in Fortran order = (ncell, nf) C
+        for a in (vproj, idf, vb2, ekt, ne, tothi, spe):
+            pl += a.tobytes()
+        pl += lnbase.tobytes() + dep.tobytes()   # (ncell, np) Fortran = (np, ncell) C
+        if fl & 4:
+            pl += ome.T.tobytes() + csc.T.tobytes()
+        return pl
+
+    kind = "opac_gen" if general else "opac_equi"
+    reach_in = active = None
     n2 = nf * ncell
+    if flags & 8:
+        first = run(kind, payload_of(flags & ~8))
+        r0 = np.frombuffer(first[8 * n2 + 8 * nl:8 * n2 + 12 * nl], np.float32)
+        reach_in = np.where(r0 > 0, r0 * rng.uniform(0.3, 1.5, nl), rng.uniform(0, 5 * dl, nl)).astype(np.float32)
+        active = (rng.random(nl) >= 1 / 6).astype(np.int32)
+    out = run(kind, payload_of(flags, reach_in, active))
     o = np.frombuffer(out, np.uint8)
     chi_o = np.frombuffer(out[:4 * n2], np.float32).reshape(ncell, nf)
     eta_o = np.frombuffer(out[4 * n2:8 * n2], np.float32).reshape(ncell, nf)
@@ -206,6 +223,9 @@ def gen_opac(rng, name, nl=40, nf=300, m=(4, 3, 2), flags=0, np_=1, nlev=4, gene
     strength = np.frombuffer(out[p:p + 4 * nl], np.float32); p += 4 * nl
     reach = np.frombuffer(out[p:p + 4 * nl], np.float32); p += 4 * nl
     extra = {}
+    if flags & 8:
+        extra["in_reach_in"] = reach_in
+        extra["in_active"] = active
     if flags & 4:
         extra["chi_scat"] = np.frombuffer(out[p:p + 4 * n2], np.float32).reshape(ncell, nf)
         extra["in_one_m_eps"] = ome.T.copy()
@@ -356,6 +376,12 @@ def main():
                                     ("hop_opac_gen_scat.npz", dict(flags=4, nl=30, nf=200, m=(3, 2, 2)))]):
         if not only or name in only:
             gen_opac(np.random.default_rng(20261210 + i), name, general=True, **kw)
+    for i, (name, kw) in enumerate([("hop_opac_equi_reach.npz", dict(flags=8)),
+                                    ("hop_opac_equi_reach_scat.npz", dict(flags=8 | 4, nl=30, nf=200, m=(3, 2, 2))),
+                                    ("hop_opac_gen_reach.npz", dict(flags=8, general=True)),
+                                    ("hop_opac_gen_reach_nlte.npz", dict(flags=8 | 1 | 2, np_=2, general=True))]):
+        if not only or name in only:
+            gen_opac(np.random.default_rng(20261230 + i), name, **kw)
     gen_quad()
     gen_pchip(np.random.default_rng(20261220))
     gen_optab(np.random.default_rng(20261221))

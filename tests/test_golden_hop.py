@@ -95,18 +95,35 @@ def test_lineq_no_pivot_reference_solution():
         assert np.allclose(np.linalg.solve(A, b), x, rtol=1e-9, atol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["hop_opac_equi_lte.npz", "hop_opac_equi_nlte.npz",
-                                  "hop_opac_equi_scat.npz", "hop_opac_gen_lte.npz", "hop_opac_gen_nlte.npz",
-                                  "hop_opac_gen_scat.npz"])
+OPAC_GOLDENS = ["hop_opac_equi_lte.npz", "hop_opac_equi_nlte.npz", "hop_opac_equi_scat.npz",
+                "hop_opac_gen_lte.npz", "hop_opac_gen_nlte.npz", "hop_opac_gen_scat.npz",
+                "hop_opac_equi_reach.npz", "hop_opac_equi_reach_scat.npz", "hop_opac_gen_reach.npz",
+                "hop_opac_gen_reach_nlte.npz"]
+
+
+@pytest.mark.parametrize("name", OPAC_GOLDENS)
 def test_add_opacity_equi_matches_reference_build(name):
     """add_opacity_equi, and add_opacity on grids whose steps vary by a factor 5
-    (the hop_opac_gen fixtures), against the reference's compiled outputs."""
+    (the hop_opac_gen fixtures), against the reference's compiled outputs; the
+    _reach fixtures hold lists with reaches and active flags (select_lines'
+    narrowing and per-line test, the lines centred beyond the grid with the grid
+    end's reference and their windows capped by the reach)."""
     g = load(name)
     fn = O.add_opacity if "_gen_" in name else O.add_opacity_equi
     flags = int(g["in_flags"])
     lines = {k: g["in_" + k] for k in ("llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk",
                                         "unsold", "lev_i", "lev_j")}
     lines["ipart"] = g["in_ipart"] if flags & 2 else None
+    if flags & 8:
+        lines["reach_in"] = g["in_reach_in"]
+        lines["active"] = g["in_active"]
+        # the case exercises what the reaches change: inactive lines, lines outside
+        # the grid that deposit, lines the selection leaves out
+        llam = g["in_llam"]
+        out = (g["in_llam0"] < llam[0]) | (g["in_llam0"] > llam[-1])
+        assert np.any(g["in_active"] == 0) and not np.any(g["used"][g["in_active"] == 0])
+        assert np.any(g["used"][out] == 1)
+        assert np.any((g["reach"] == 0) & (g["in_active"] == 1) & out)
     cells = {k: g["in_" + k] for k in ("vproj", "idf", "vb2", "ekt", "ne", "tothi", "spe")}
     r = fn(lines, g["in_llam"], g["in_chi"], g["in_eta"], g["in_cont"], g["in_logref"],
            cells, g["in_lnbase"], dep=g["in_dep"], do_nlte=bool(flags & 1),

@@ -31,6 +31,9 @@ def run(ctx, g, layout, flags, general=False):
     lines.update({k: dev(g["in_" + k], torch.int32) for k in LINE_KEYS_I})
     if flags & 2:
         lines["ipart"] = dev(g["in_ipart"], torch.int32)
+    if flags & 8:
+        lines["reach_in"] = dev(g["in_reach_in"], torch.float32)
+        lines["active"] = dev(g["in_active"], torch.int32)
     sp = (lambda a: dev(a.T)) if layout == "native" else (lambda a: dev(a))
     chi, eta = sp(g["in_chi"]), sp(g["in_eta"])
     ome = sp(g["in_one_m_eps"]) if flags & 4 else None
@@ -55,11 +58,15 @@ def run(ctx, g, layout, flags, general=False):
 
 @pytest.mark.parametrize("name", ["hop_opac_equi_lte.npz", "hop_opac_equi_nlte.npz",
                                   "hop_opac_equi_scat.npz", "hop_opac_gen_lte.npz", "hop_opac_gen_nlte.npz",
-                                  "hop_opac_gen_scat.npz"])
+                                  "hop_opac_gen_scat.npz", "hop_opac_equi_reach.npz",
+                                  "hop_opac_equi_reach_scat.npz", "hop_opac_gen_reach.npz",
+                                  "hop_opac_gen_reach_nlte.npz"])
 @pytest.mark.parametrize("layout", ["native", "reference"])
 def test_opacity_matches_reference_build(ctx, name, layout):
     """m3drt_line_opacity_equi on the equidistant fixtures, m3drt_line_opacity
-    (add_opacity) on the hop_opac_gen ones, whose grid steps vary by a factor 5."""
+    (add_opacity) on the hop_opac_gen ones, whose grid steps vary by a factor 5; the
+    _reach fixtures are lists with reaches and active flags (select_lines, the lines
+    beyond the grid with their capped windows)."""
     g = np.load(os.path.join(GOLD, name))
     flags = int(g["in_flags"])
     print(f"opacity {name} {layout}", flush=True)
@@ -93,3 +100,21 @@ def test_opacity_matches_oracle_many_cells(ctx):
     for k in ("chi", "eta"):
         err = np.max(np.abs(r[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-30))
         assert err < 2e-6, f"{k}: {err:.2e}"
+
+
+def test_missing_selection_scratch_is_an_error(ctx):
+    from dispatch_amd import m3drt
+    g = np.load(os.path.join(GOLD, "hop_opac_equi_reach.npz"))
+    nl = len(g["in_llam0"])
+    lines = {"llam0": dev(g["in_llam0"], torch.float64)}
+    lines.update({k: dev(g["in_" + k], torch.float32) for k in LINE_KEYS_F})
+    lines.update({k: dev(g["in_" + k], torch.int32) for k in LINE_KEYS_I})
+    lines["active"] = dev(g["in_active"], torch.int32)
+    L = m3drt.lib()
+    ll = m3drt.Linelist(nl, *[m3drt.C.c_void_p(lines[k].data_ptr()) for k in (
+        "llam0", "lower_ev", "gpref", "grad", "gw1", "gw2", "gstk", "unsold", "lev_i", "lev_j")],
+        None, None, None, None, None, m3drt.C.c_void_p(lines["active"].data_ptr()), None)
+    o = m3drt.OpacityArgs()
+    o.nf, o.ncell, o.npart, o.sf, o.sc = 4, 1, 1, 1, 4
+    assert L.m3drt_line_opacity_equi(ctx._h, m3drt.C.byref(ll), m3drt.C.byref(o)) != 0
+    assert "sel_work" in m3drt.last_error()
