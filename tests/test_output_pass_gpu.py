@@ -1,8 +1,9 @@
 """The output pass at the workload's full box (120 x 120 x 230, 8 frequencies, one
 inclined ray): the column solvers on the rotated cubes against the oracle run on
 the same rotated inputs copied to the host (I on every plane through the exit
-plane), and the emergent fluxes of the three solvers agree to the accuracy of
-their source integrations."""
+plane), the result files written from the GPU's exit planes against the oracle's
+emergent products, and the emergent fluxes of the three solvers agree to the
+accuracy of their source integrations."""
 import numpy as np
 import pytest
 import torch
@@ -10,10 +11,14 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def test_output_pass_columns_match_oracle(ctx):
-    from dispatch_amd import output_pass as OP, synth
+def test_output_pass_columns_match_oracle(ctx, tmp_path):
+    """... and each solver's emergent plane written through save.save_intensity
+    (ie, flux of the window) reads back, as tumult's io_binary reads it, as the
+    emergent products of the oracle's exit plane to 1e-5."""
+    from dispatch_amd import output_pass as OP, save, synth
     from oracle import oracle as O
     from sweep_cases import max_rel
+    from test_save_cpu import read_like_tumult
     nf = 8
     box = synth.make_box(nf=nf, seed=20261203)
     chi, S = synth.build_cubes(box)
@@ -67,6 +72,17 @@ def test_output_pass_columns_match_oracle(ctx):
         err = max_rel(ref(I_out), Iref, 1e-6 * np.abs(Iref).max())
         assert err < 1e-5, (solver, err)
         assert max_rel(np.moveaxis(h(exit_I), 0, 2), Iref[nz - 1]) < 1e-5
+        # the result files of this window (one output ray of flux weight wmu)
+        wmu = [0.25]
+        run = str(tmp_path / solver)
+        save.save_intensity(run, exit_I[None], lam_nu, wmu, key=(1, 1))
+        ie = read_like_tumult(f"{run}/save", "ie")[(1, 1)]
+        flux = read_like_tumult(f"{run}/save", "flux")[(1, 1)]
+        ref_exit = torch.from_numpy(np.ascontiguousarray(np.moveaxis(I[nz].astype(np.float32), 2, 0)))[None]
+        rie, rflux, _ = save.emergent_products(ref_exit, lam_nu, wmu)
+        assert ie.shape == (nf, 1, 1, 1, 1) and flux.shape == (nf, 1, 1, 1, 1)
+        assert np.max(np.abs(np.asarray(ie) / rie - 1)) < 1e-5, solver
+        assert np.max(np.abs(np.asarray(flux) / rflux - 1)) < 1e-5, solver
 
 
 def test_output_pass_fluxes_agree():
