@@ -466,9 +466,10 @@ class NlteProblem:
                 ng.accumulate(self.b, self._oldb, res["nclamp"])
         return res
 
-    def solve(self, log=None):
-        """Iterate to max |dn/n| < conv or max_iter; returns the iteration count."""
-        for it in range(1, self.cfg.max_iter + 1):
+    def solve(self, log=None, iterations=None):
+        """Iterate to max |dn/n| < conv or max_iter; returns the iteration count.
+        iterations: run exactly that many instead (the Ng gate still sees conv)."""
+        for it in range(1, (iterations or self.cfg.max_iter) + 1):
             t0 = time.perf_counter()
             res = self.iterate()
             if self.device == "cuda":
@@ -478,9 +479,9 @@ class NlteProblem:
             self.ng_history.append("ng" in res)
             if log:
                 log(f"[ali] iteration {it}: max|dn/n| = {res['maxerr']:.3e}  ({dt:.2f} s)")
-            if abs(res["maxerr"]) < self.cfg.conv:
+            if iterations is None and abs(res["maxerr"]) < self.cfg.conv:
                 return it
-        return self.cfg.max_iter
+        return iterations or self.cfg.max_iter
 
 
 def main():

@@ -188,6 +188,60 @@ __global__ void __launch_bounds__(1024) select_lines_kernel(const double *__rest
   if (tid == 0) sel[0] = base;
 }
 
+// qlg_bot_line_eval (HOP/qlg_mod.f90:542-556) at abscissa x
+__device__ __forceinline__ float bot_eval(const Coef &c, float x, const float *TGX, const float *TH2X) {
+  const float xs = fabsf(x) * c.ih;
+  const float u = xs * xs;
+  const float L = __builtin_amdgcn_rcpf(1.0f + u);
+  const int ig = nint(fminf(xs * 800.0f, (float)QLG_NX));
+  const float G = TGX[ig], W = TH2X[ig];
+  float p = c.cL2 + c.cZH * W;
+  p = c.bL + L * p;
+  return G * c.gG + L * p;
+}
+
+// One line's deposits over its window fa..fb (1-based) of one cell's spectrum, in
+// batches of eight frequencies: every load of a batch is issued before its sums and
+// stores, so a lane keeps eight read-modify-writes in flight instead of one.  The
+// arithmetic and the order of the sums at each frequency are those of the plain loop.
+template <typename XF>
+__device__ __forceinline__ void deposit(float *__restrict__ chi, float *__restrict__ eta,
+                                        float *__restrict__ csc, const float *__restrict__ ome, int64_t sf,
+                                        int fa, int fb, const Coef &c, const float *TGX, const float *TH2X,
+                                        bool plain, float bi, float bj, XF xof) {
+  constexpr int U = 8;
+  for (int f = fa; f <= fb; f += U) {
+    float cv[U], ev[U], sv[U], ov[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (f + j <= fb) {
+        const int64_t o = (int64_t)(f + j - 1) * sf;
+        cv[j] = chi[o];
+        ev[j] = eta[o];
+        if (!plain) {
+          sv[j] = csc[o];
+          ov[j] = ome[o];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (f + j <= fb) {
+        const int64_t o = (int64_t)(f + j - 1) * sf;
+        const float lte = bot_eval(c, xof(f + j), TGX, TH2X);
+        if (plain) {
+          chi[o] = cv[j] + lte * bi;
+          eta[o] = ev[j] + lte * bj;
+        } else {
+          chi[o] = cv[j] + lte;
+          eta[o] = ev[j] + lte;
+          csc[o] = sv[j] + ov[j] * lte;
+        }
+      }
+    }
+  }
+}
+
 template <bool GENERAL>
 __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];  // TGX then TH2X
@@ -208,6 +262,8 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   const float sdl = (float)inv_dllam;
 
   float *chi = a.chi + cidx * a.sc, *eta = a.eta + cidx * a.sc;
+  float *csc_c = a.csc ? a.csc + cidx * a.sc : nullptr;
+  const float *ome_c = a.ome ? a.ome + cidx * a.sc : nullptr;
   const float *cont = a.cont + cidx * a.sc, *logref = a.logref + cidx * a.sc;
   const int64_t sf = a.sf;
 
@@ -322,26 +378,9 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
           fb = bisect_nearest(a.llam, f0, nf, llc + dl);
         }
       }
-      for (int f = fa; f <= fb; ++f) {
-        const float x = (float)(a.llam[f - 1] - llc);
-        const float xs = fabsf(x) * c.ih;
-        const float u = xs * xs;
-        const float L = __builtin_amdgcn_rcpf(1.0f + u);
-        const int ig = nint(fminf(xs * 800.0f, (float)QLG_NX));
-        const float G = TGX[ig], W = TH2X[ig];
-        float p = c.cL2 + c.cZH * W;
-        p = c.bL + L * p;
-        const float lte = G * c.gG + L * p;
-        const int64_t o = (int64_t)(f - 1) * sf;
-        if (plain) {
-          chi[o] = chi[o] + lte * bi;
-          eta[o] = eta[o] + lte * bj;
-        } else {
-          chi[o] = chi[o] + lte;
-          eta[o] = eta[o] + lte;
-          a.csc[cidx * a.sc + o] = a.csc[cidx * a.sc + o] + a.ome[cidx * a.sc + o] * lte;
-        }
-      }
+      const double *llam = a.llam;
+      deposit(chi, eta, csc_c, ome_c, sf, fa, fb, c, TGX, TH2X, plain, bi, bj,
+              [llam, llc](int f) { return (float)(llam[f - 1] - llc); });
       f0_prev = -1;
       continue;
     }
@@ -364,27 +403,8 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     if (fa > fb) continue;
     if (a.used) a.used[l] = 1;
     const float x0 = x1 - (float)(i1 - fa);
-    for (int f = fa; f <= fb; ++f) {
-      // qlg_bot_line_eval (HOP/qlg_mod.f90:542-556)
-      const float x = x0 + (float)(f - fa);
-      const float xs = fabsf(x) * c.ih;
-      const float u = xs * xs;
-      const float L = __builtin_amdgcn_rcpf(1.0f + u);
-      const int ig = nint(fminf(xs * 800.0f, (float)QLG_NX));
-      const float G = TGX[ig], W = TH2X[ig];
-      float p = c.cL2 + c.cZH * W;
-      p = c.bL + L * p;
-      const float lte = G * c.gG + L * p;
-      const int64_t o = (int64_t)(f - 1) * sf;
-      if (plain) {
-        chi[o] = chi[o] + lte * bi;
-        eta[o] = eta[o] + lte * bj;
-      } else {
-        chi[o] = chi[o] + lte;
-        eta[o] = eta[o] + lte;
-        a.csc[cidx * a.sc + o] = a.csc[cidx * a.sc + o] + a.ome[cidx * a.sc + o] * lte;
-      }
-    }
+    deposit(chi, eta, csc_c, ome_c, sf, fa, fb, c, TGX, TH2X, plain, bi, bj,
+            [x0, fa](int f) { return x0 + (float)(f - fa); });
     f0_prev = -1;
   }
 }
