@@ -3,10 +3,16 @@
 ! dispatch.x's rt_patch_t would: reference-layout work arrays on the host, device
 ! copies through the library's helpers, short_char2_gpu per direction.
 !
-! Case: a 3D column with random opacity and a constant source function S0, the
+! Case 1: a 3D column with random opacity and a constant source function S0, the
 ! upstream ghost plane at I = S0.  Every linear step keeps I = S0 exactly (its
 ! weights sum to one, the upwind interpolation reproduces constants), so J must
 ! come back as wmu2*S0 everywhere and the exit plane as S0.  Prints PASS or FAIL.
+!
+! Case 2: the same column with a random source function; the inputs and the J and
+! exit plane the GPU returns are written, as raw Fortran-order streams, to the file
+! named by the first argument (when given), for the oracle to check:
+!   nf nx ny nz (int32); ds(3) vec(3) wmu2 (real(8)); chi S (nf,nx,ny,nz);
+!   ghost (nf,nx,ny,3: chi, S, I); c500 cup (nx,ny,nz); g500 (nx,ny); J; exit (nf,nx,ny)
 !===============================================================================
 program sweep_driver
   use iso_c_binding
@@ -18,8 +24,8 @@ program sweep_driver
   type(c_ptr) :: ctx, dchi, dsrc, dgh, dJ, dex, dc5, dcu, dg5
   real(8) :: ds(3), vec(3), wmu2
   real(4) :: S0, err
-  integer :: st, z
-  real(4) :: r
+  integer :: st, z, uo
+  character(len=512) :: fout
   S0 = 0.0372
   allocate(chi(nf,nx,ny,nz), S(nf,nx,ny,nz), ghost(nf,nx,ny,3), J(nf,nx,ny,nz), ex(nf,nx,ny))
   allocate(c500(nx,ny,nz), cup(nx,ny,nz), g500(nx,ny))
@@ -63,6 +69,33 @@ program sweep_driver
     print '(a)', 'PASS'
   else
     print '(a)', 'FAIL'
+  end if
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, fout)
+    call random_number(S)
+    S = S0 * (0.5 + S)
+    ghost(:,:,:,2) = S(:,:,:,1) * 1.1
+    ghost(:,:,:,3) = ghost(:,:,:,2)
+    J = 0
+    st = m3drt_copy_to_device(ctx, dsrc, c_loc(S), int(size(S), c_size_t) * 4)
+    call check('copy S')
+    st = m3drt_copy_to_device(ctx, dgh, c_loc(ghost), int(size(ghost), c_size_t) * 4)
+    call check('copy ghost')
+    st = m3drt_copy_to_device(ctx, dJ, c_loc(J), int(size(J), c_size_t) * 4)
+    call check('copy J')
+    st = short_char2_gpu(ctx, nf, nx, ny, nz, 1, ds, vec, dchi, dsrc, dgh, dc5, dcu, dg5, wmu2, dJ, dex, 2d0, 1d30)
+    call check('short_char2_gpu, random source')
+    st = m3drt_synchronize(ctx)
+    st = m3drt_copy_to_host(ctx, c_loc(J), dJ, int(size(J), c_size_t) * 4)
+    call check('copy J')
+    st = m3drt_copy_to_host(ctx, c_loc(ex), dex, int(size(ex), c_size_t) * 4)
+    call check('copy exit')
+    open(newunit=uo, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(uo) nf, nx, ny, nz
+    write(uo) ds, vec, wmu2
+    write(uo) chi, S, ghost, c500, cup, g500, J, ex
+    close(uo)
+    print '(a)', 'WROTE'
   end if
   st = m3drt_destroy(ctx)
 contains

@@ -139,3 +139,29 @@ def test_single_atmosphere_synthesis_matches_oracle(ctx):
     host = {k: v.cpu().numpy() for k, v in arrays.items()}
     check_oracle(arrays, meta, [O.b1d_atmosphere(host, 0, B.LAM_REF)])
     plan.close()
+
+
+def test_full_workload_sample_matches_oracle(ctx):
+    """The 8192-atmosphere workload itself (depths 48-256, 200-6000 frequencies, 3-7
+    mu, 20-200 lines) through the autotuned persistent plan, and the oracle on its
+    atmospheres with the most depth points, frequencies, mu points and
+    lines, the most (depth, frequency, mu) updates, and 24 more drawn at random."""
+    arrays, meta = B.make_batch(B.B1dConfig(), device="cuda")
+    print("b1d full workload, 8192 atmospheres", flush=True)
+    plan = ctx.b1d_plan(arrays, B.LAM_REF)
+    plan.run()
+    torch.cuda.synchronize()
+    plan.close()
+    upd = meta["nd"].astype(np.int64) * meta["nf"] * meta["nmu"]
+    pick = {int(np.argmax(meta[k])) for k in ("nd", "nf", "nmu", "nline")} | {int(np.argmax(upd))}
+    pick |= set(int(m) for m in np.random.default_rng(3).choice(len(meta["nd"]), 24, replace=False))
+    host = {k: v.cpu().numpy() for k, v in arrays.items()}
+    worst = 0.0
+    for m in sorted(pick):
+        print(f"  atmosphere {m}: nd {meta['nd'][m]} nf {meta['nf'][m]} nmu {meta['nmu'][m]} "
+              f"lines {meta['nline'][m]}", flush=True)
+        Ir, Fr = O.b1d_atmosphere(host, m, B.LAM_REF)
+        I, F = split(arrays, meta, m)
+        worst = max(worst, float(np.max(np.abs(I - Ir) / np.abs(Ir))), float(np.max(np.abs(F - Fr) / np.abs(Fr))))
+    assert worst < TOL, worst
+    assert max(meta["nd"][m] for m in pick) > 200 and max(meta["nf"][m] for m in pick) > 5000
