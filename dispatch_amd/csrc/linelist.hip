@@ -1,7 +1,9 @@
 // The line-list opacity of a composition (composition_type%add_linelist_opacities,
 // HOP/composition_mod.f90:1663-1954, without its hydrogen Stark branch): the
 // tolerance reference logref = ln(tol/voigt_peak) + ln chi from the working
-// opacity before any line (:1766-1772); per species and isotope the population
+// opacity before any line (:1766-1772), formed by the summation kernel at the
+// nodes it reads (the working opacity is only added to at the end); per species
+// and isotope the population
 // base and Doppler width of add_particle_opacity (HOP/linelist_mod.f90:970-1070)
 // and the Voigt summation (line_opacity, the kernels of m3drt_line_opacity[_equi])
 // into the line increments; the increments added to the working xhat/ehat
@@ -52,17 +54,6 @@ __global__ void __launch_bounds__(256) particle_base_kernel(int64_t ncell, const
   lnbase[c] = (float)((((double)lb - part[c]) + (double)(float)log((double)id)) + (double)(float)log(frac));
 }
 
-__global__ void __launch_bounds__(256) logref_kernel(int nf, int64_t ncell, int64_t sf, int64_t sc, float lt,
-                                                     const float *__restrict__ chi, float *__restrict__ logref) {
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int f = blockIdx.y;
-  if (c >= ncell) return;
-  const int64_t o = f * sf + c * sc;
-  const float x = chi[o];
-  logref[(int64_t)f * ncell + c] = x > 0.0f ? lt + (float)log((double)x) : -FLT_MAX;
-}
-
-// a fill kernel rather than a memset node, so that a caller may capture the call
 __global__ void __launch_bounds__(256) zero_kernel(int64_t n, float *__restrict__ p) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0.0f;
@@ -123,18 +114,17 @@ extern "C" int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist
   const int nf = a->nf;
   const int64_t nc = a->ncell, n2 = (int64_t)nf * nc;
   const int64_t sf = nc, sc = 1;
-  // scratch: logref, chi_l, eta_l (f, cell) and idf, vb2, lnbase (cell)
-  const size_t bytes = sizeof(float) * (size_t)(3 * n2 + 3 * nc);
+  // scratch: chi_l, eta_l (f, cell) and idf, vb2, lnbase (cell).  logref =
+  // ln(tol/voigt_peak) + ln all_chi is formed by the opacity kernel at the nodes it
+  // needs: all_chi is not touched until every list has been summed.
+  const size_t bytes = sizeof(float) * (size_t)(2 * n2 + 3 * nc);
   int rc = m3drt::ensure_scratch(ctx, bytes);
   if (rc) return rc;
-  float *logref = static_cast<float *>(ctx->scratch);
-  float *chi_l = logref + n2, *eta_l = chi_l + n2, *idf = eta_l + n2, *vb2 = idf + nc, *lnb = vb2 + nc;
+  float *chi_l = static_cast<float *>(ctx->scratch);
+  float *eta_l = chi_l + n2, *idf = eta_l + n2, *vb2 = idf + nc, *lnb = vb2 + nc;
   hipStream_t st = ctx->stream;
   const dim3 g2((unsigned)((nc + 255) / 256), (unsigned)nf);
   const unsigned g1 = (unsigned)((nc + 255) / 256);
-  const float voigt_peak = 0.5641895835f;  // 1/sqrt(pi), bbtrans_mod
-  const float lt = (float)log((double)(a->tolerance / voigt_peak));
-  hipLaunchKernelGGL(m3drt::logref_kernel, g2, dim3(256), 0, st, nf, nc, sf, sc, lt, a->all_chi, logref);
   hipLaunchKernelGGL(m3drt::zero_kernel, dim3((unsigned)std::min<int64_t>((2 * n2 + 255) / 256, 1 << 20)), dim3(256),
                      0, st, 2 * n2, chi_l);
   M3DRT_HIP(hipGetLastError());
@@ -169,10 +159,10 @@ extern "C" int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist
       o.chi = chi_l;
       o.eta = eta_l;
       o.cont = a->all_chi;
-      o.logref = logref;
+      o.logref = nullptr;
       o.one_m_eps = (o.flags & M3DRT_OPAC_SCAT) ? a->one_m_eps + (int64_t)(sp.scat_group - 1) * n2 : nullptr;
       o.chi_scat = (o.flags & M3DRT_OPAC_SCAT) ? a->chi_scat : nullptr;
-      rc = m3drt::line_opacity(ctx, &iso.lines, &o, a->equidistant == 0);
+      rc = m3drt::line_opacity(ctx, &iso.lines, &o, a->equidistant == 0, true);
       if (rc) return rc;
     }
   }

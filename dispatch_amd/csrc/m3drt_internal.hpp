@@ -46,7 +46,11 @@ void *noalloc_scratch(int device, size_t *bytes);
 int sweep_sc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int sweep_lc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *b);
 int qlg_tables(m3drt_ctx *ctx);
-int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general);
+// lref_from_cont: the tolerance reference logref is ln(tol/voigt_peak) + ln cont,
+// formed at the nodes the lines need instead of read from o->logref (the line-list
+// composition, whose cont is the working opacity the reference forms logref from)
+int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general,
+                 bool lref_from_cont = false);
 int rate_solve(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res);
 int normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi);
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);

@@ -12,7 +12,10 @@
 // one cell and the line loop is uniform across a wavefront: lanes are consecutive
 // cells, and with cells at unit stride (the native layout) every deposit of a
 // wavefront at one frequency is one contiguous run.  QLG-bot's two 9020-entry
-// tables (72 KB) are staged in LDS once per workgroup.
+// tables (72 KB) are staged in LDS once per workgroup.  Up to four lines that pass
+// are deposited together, eight frequencies at a time (one read-modify-write of
+// the spectrum per frequency for the chunk), with every decision still taken on the
+// opacity the sequential order would have at that point.
 //
 // Arithmetic.  FP32 as the reference, each operation separately rounded (no fused
 // multiply-add) so that the window edges and the snapped table index, integer
@@ -48,11 +51,25 @@ struct OpacArgs {
   const float *cont, *logref, *ome;
   int64_t sf, sc;
   float tol, rej;
+  int lref_from_cont;  // logref = lt + ln cont at the nodes used, -FLT_MAX where cont <= 0
+  float lt;            // ln(tol / voigt_peak)
   const float *tgx, *th2x;
 };
 
 struct Coef {
   float ih, bL, gG, cZH, cL2;
+};
+
+// a line that passed, waiting to be deposited: its profile constants, its window
+// fa..fb (1-based), its abscissa (the node-fa offset x0 in grid steps on an
+// equidistant grid, the shifted centre llc in ln(lambda) on any other), its
+// departure factors and whether it also feeds chi_scat (plain: it does not)
+struct Pend {
+  Coef c;
+  double llc;
+  float x0, bi, bj;
+  int fa, fb;
+  bool plain;
 };
 
 // bot_setup (HOP/qlg_mod.f90:524-540): QLG-bot's per-line constants, the paper's fit
@@ -200,48 +217,6 @@ __device__ __forceinline__ float bot_eval(const Coef &c, float x, const float *T
   return G * c.gG + L * p;
 }
 
-// One line's deposits over its window fa..fb (1-based) of one cell's spectrum, in
-// batches of eight frequencies: every load of a batch is issued before its sums and
-// stores, so a lane keeps eight read-modify-writes in flight instead of one.  The
-// arithmetic and the order of the sums at each frequency are those of the plain loop.
-template <typename XF>
-__device__ __forceinline__ void deposit(float *__restrict__ chi, float *__restrict__ eta,
-                                        float *__restrict__ csc, const float *__restrict__ ome, int64_t sf,
-                                        int fa, int fb, const Coef &c, const float *TGX, const float *TH2X,
-                                        bool plain, float bi, float bj, XF xof) {
-  constexpr int U = 8;
-  for (int f = fa; f <= fb; f += U) {
-    float cv[U], ev[U], sv[U], ov[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (f + j <= fb) {
-        const int64_t o = (int64_t)(f + j - 1) * sf;
-        cv[j] = chi[o];
-        ev[j] = eta[o];
-        if (!plain) {
-          sv[j] = csc[o];
-          ov[j] = ome[o];
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (f + j <= fb) {
-        const int64_t o = (int64_t)(f + j - 1) * sf;
-        const float lte = bot_eval(c, xof(f + j), TGX, TH2X);
-        if (plain) {
-          chi[o] = cv[j] + lte * bi;
-          eta[o] = ev[j] + lte * bj;
-        } else {
-          chi[o] = cv[j] + lte;
-          eta[o] = ev[j] + lte;
-          csc[o] = sv[j] + ov[j] * lte;
-        }
-      }
-    }
-  }
-}
-
 template <bool GENERAL>
 __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];  // TGX then TH2X
@@ -278,8 +253,95 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   const float ekt = a.ekt[cidx], ne_c = a.ne[cidx], tothi_c = a.tothi[cidx];
   const float starkpe = a.spe[cidx], vproj_c = a.vproj[cidx];
 
-  int f0_prev = -1, fc = 1;
-  float refl_c = 0.0f, lref_c = 0.0f;
+  // Lines that pass are kept pending, up to K, and deposited together: one batched
+  // read-modify-write of the spectrum per frequency for the chunk instead of one per
+  // line, the sums at each frequency still in line order.  A new line's running floor
+  // needs chi at its centre node as the sequential order has it there: the global
+  // value (every earlier chunk deposited) plus the pending lines' deposits at that
+  // node, added in line order with the same FP32 operations.
+  constexpr int K = 4;
+  Pend pend[K];
+  int np = 0, fc = 1;
+  const double *llam = a.llam;
+  auto val = [&](const Pend &q, int f) -> float {
+    const float x = GENERAL ? (float)(llam[f - 1] - q.llc) : q.x0 + (float)(f - q.fa);
+    return bot_eval(q.c, x, TGX, TH2X);
+  };
+  auto flush = [&]() {
+    if (np == 0) return;
+    int Fa = nf + 1, Fb = 0;
+    bool anyscat = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k < np) {
+        Fa = min(Fa, pend[k].fa);
+        Fb = max(Fb, pend[k].fb);
+        anyscat = anyscat || !pend[k].plain;
+      }
+    }
+    constexpr int U = 8;
+    for (int f = Fa; f <= Fb; f += U) {
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) any = any || (k < np && pend[k].fa <= f + U - 1 && pend[k].fb >= f);
+      if (!any) continue;
+      float cv[U], ev[U], sv[U], ov[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (f + j <= Fb) {
+          const int64_t o = (int64_t)(f + j - 1) * sf;
+          cv[j] = chi[o];
+          ev[j] = eta[o];
+          if (anyscat) {
+            sv[j] = csc_c[o];
+            ov[j] = ome_c[o];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k < np) {
+#pragma unroll
+          for (int j = 0; j < U; ++j) {
+            const int g = f + j;
+            if (g >= pend[k].fa && g <= pend[k].fb) {
+              const float lte = val(pend[k], g);
+              if (pend[k].plain) {
+                cv[j] = cv[j] + lte * pend[k].bi;
+                ev[j] = ev[j] + lte * pend[k].bj;
+              } else {
+                cv[j] = cv[j] + lte;
+                ev[j] = ev[j] + lte;
+                sv[j] = sv[j] + ov[j] * lte;
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (f + j <= Fb) {
+          const int64_t o = (int64_t)(f + j - 1) * sf;
+          chi[o] = cv[j];
+          eta[o] = ev[j];
+          if (anyscat) csc_c[o] = sv[j];
+        }
+      }
+    }
+    np = 0;
+  };
+  auto queue = [&](const Pend &q) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (k == np) pend[k] = q;
+    if (++np == K) flush();
+  };
+
+  auto lref_at = [&](int f) -> float {
+    if (!a.lref_from_cont) return logref[(f - 1) * sf];
+    const float x = cont[(f - 1) * sf];
+    return x > 0.0f ? a.lt + (float)log((double)x) : -3.40282347e38f;
+  };
   const int nsel = a.sel ? a.sel[0] : a.nline;
   for (int is = 0; is < nsel; ++is) {
     const int l = a.sel ? a.sel[1 + is] : is;
@@ -302,25 +364,32 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     // continuum at the nearer grid end, without the running floor (:594-602, 844-851)
     const bool outside = a.reach_in && (llc < a.llam[0] || llc > a.llam[nf - 1]);
     float Rij, ref, lnq;
-    int fe = 1;
     if (outside) {
-      fe = llc < a.llam[0] ? 1 : nf;
-      const float lref_o = logref[(fe - 1) * sf];
+      const int fe = llc < a.llam[0] ? 1 : nf;
+      const float lref_o = lref_at(fe);
       if (arg <= lref_o) continue;
       Rij = (float)exp((double)arg);
       ref = a.tol * cont[(fe - 1) * sf];
       lnq = arg - lref_o;
     } else {
-      if (f0 != f0_prev) {
-        f0_prev = f0;
-        refl_c = a.tol * cont[(f0 - 1) * sf];
-        const float flo = fmaxf(a.tol, a.rej) * a.tol * chi[(f0 - 1) * sf];
-        if (flo > refl_c) {
-          refl_c = flo;
-          lref_c = (float)log((double)(flo * sqrt_pi));
-        } else {
-          lref_c = logref[(f0 - 1) * sf];
+      // the tolerance reference: tol * continuum, floored by tol^2 * the opacity
+      // already at the centre node (:853-870; the reference caches it per node until
+      // a line deposits, which is the same value)
+      float chi_f0 = chi[(f0 - 1) * sf];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k < np && pend[k].fa <= f0 && f0 <= pend[k].fb) {
+          const float v = val(pend[k], f0);
+          chi_f0 = pend[k].plain ? chi_f0 + v * pend[k].bi : chi_f0 + v;
         }
+      }
+      float refl_c = a.tol * cont[(f0 - 1) * sf], lref_c;
+      const float flo = fmaxf(a.tol, a.rej) * a.tol * chi_f0;
+      if (flo > refl_c) {
+        refl_c = flo;
+        lref_c = (float)log((double)(flo * sqrt_pi));
+      } else {
+        lref_c = lref_at(f0);
       }
       if (arg <= lref_c) continue;
       Rij = (float)exp((double)arg);
@@ -347,16 +416,22 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       gam = gam - a.gstk[l] * starkpe;
     if (!GENERAL) gam = gam * sdl;
     if (!outside && a.strength) atomic_max_pos(&a.strength[l], voigt_peak * Rij / cont[(f0 - 1) * sf]);
-    const bool plain = !(a.scat && !nlte);
+    Pend q;
+    q.bi = bi;
+    q.bj = bj;
+    q.plain = !(a.scat && !nlte);
+    q.llc = llc;
+    q.x0 = 0.0f;
     if (GENERAL) {
       // qlg_bot_line on the ln(lambda) abscissa; the window by bisection unless the
       // line covers the grid (HOP/linelist_mod.f90:660-705); a line beyond the grid
       // keeps the part of its window (capped by its reach) that reaches the grid
-      const Coef c = bot_setup(gam, D, SQ, Rij * D);
+      q.c = bot_setup(gam, D, SQ, Rij * D);
       const float AL = 0.3183098862f * gam * (Rij * D) / ref;
       const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
                                          AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
-      int fa = 1, fb = nf;
+      q.fa = 1;
+      q.fb = nf;
       if (outside) {
         if (a.reach) atomic_max_pos(&a.reach[l], hw);
         const double dl = fmin((double)hw, (double)a.reach_in[l]);
@@ -367,21 +442,19 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
         }
         if (llc + dl < a.llam[0] - hb0 || llc - dl > a.llam[nf - 1] + hb1) continue;
         if (a.used) a.used[l] = 1;
-        fa = bisect_nearest(a.llam, 1, nf, llc - dl);
-        fb = bisect_nearest(a.llam, 1, nf, llc + dl);
+        q.fa = bisect_nearest(a.llam, 1, nf, llc - dl);
+        q.fb = bisect_nearest(a.llam, 1, nf, llc + dl);
       } else {
         if (a.used) a.used[l] = 1;
         if (a.reach) atomic_max_pos(&a.reach[l], hw);
         const double dl = (double)hw;
         if (!(dl >= fmax(a.llam[nf - 1] - llc, llc - a.llam[0]))) {
-          fa = bisect_nearest(a.llam, 1, f0, llc - dl);
-          fb = bisect_nearest(a.llam, f0, nf, llc + dl);
+          q.fa = bisect_nearest(a.llam, 1, f0, llc - dl);
+          q.fb = bisect_nearest(a.llam, f0, nf, llc + dl);
         }
       }
-      const double *llam = a.llam;
-      deposit(chi, eta, csc_c, ome_c, sf, fa, fb, c, TGX, TH2X, plain, bi, bj,
-              [llam, llc](int f) { return (float)(llam[f - 1] - llc); });
-      f0_prev = -1;
+      if (q.fa > q.fb) continue;
+      queue(q);
       continue;
     }
 
@@ -390,7 +463,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     const int ic = nint(pos) + 1;
     const int i1 = min(nf, max(1, ic));
     const float x1 = (float)((a.llam[i1 - 1] - llc) * inv_dllam);
-    const Coef c = bot_setup(gam, D, SQ, S);
+    q.c = bot_setup(gam, D, SQ, S);
     const float AL = 0.3183098862f * gam * S / ref;
     const float hw = sqrtf(fmaxf(fmaxf(D2 * lnq + fminf(AL, D2G),
                                        AL - fmaxf(gam * gam - D2h, 0.0f) + fminf(AL, D2L)), 0.0f));
@@ -398,15 +471,14 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
     // the window of a line beyond the grid is capped by its reach (:914-921)
     const float cap = outside ? (float)((double)a.reach_in[l] * inv_dllam) : 3.40282347e38f;
     const int noff = nint(fminf(fminf(hw, cap), (float)(nf + abs(ic))));
-    const int fa = max(1, ic - noff);
-    const int fb = min(nf, ic + noff);
-    if (fa > fb) continue;
+    q.fa = max(1, ic - noff);
+    q.fb = min(nf, ic + noff);
+    if (q.fa > q.fb) continue;
     if (a.used) a.used[l] = 1;
-    const float x0 = x1 - (float)(i1 - fa);
-    deposit(chi, eta, csc_c, ome_c, sf, fa, fb, c, TGX, TH2X, plain, bi, bj,
-            [x0, fa](int f) { return x0 + (float)(f - fa); });
-    f0_prev = -1;
+    q.x0 = x1 - (float)(i1 - q.fa);
+    queue(q);
   }
+  flush();
 }
 
 // ------------------------------------------------------------------ tables
@@ -485,7 +557,8 @@ int qlg_tables(m3drt_ctx *ctx) {
   return M3DRT_OK;
 }
 
-int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general) {
+int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general,
+                 bool lref_from_cont) {
   int rc = qlg_tables(ctx);
   if (rc) return rc;
   OpacArgs a;
@@ -513,6 +586,8 @@ int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_a
   a.cont = o->cont; a.logref = o->logref; a.ome = o->one_m_eps;
   a.sf = o->sf; a.sc = o->sc;
   a.tol = o->tolerance; a.rej = o->line_reject_frac;
+  a.lref_from_cont = lref_from_cont ? 1 : 0;
+  a.lt = (float)log((double)(o->tolerance / 0.5641895835f));
   a.tgx = ctx->qlg_tab;
   a.th2x = ctx->qlg_tab + QLG_NX + 1;
   const int threads = 256;
