@@ -106,7 +106,12 @@ struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) Arr {
 };
 
 // MODE 2: (I, x) pairs and S in LDS; 1: (I, x) pairs in LDS, upwind S from the
-// cache.  CPL: adjacent columns per lane (1, or 2 with the native layout: the
+// cache; 4: (I, x) pairs in LDS and, for planes where a whole S plane no longer
+// fits beside them, the S of the first SR of every strip's L+1 tap rows in LDS
+// (SR rows of nx per strip, then a spare row), the other tap rows from the cache.
+// Every strip takes the same share from the cache: the workgroup waits at each
+// plane for its slowest wave, so holding whole strips in LDS and leaving others
+// on the cache gains nothing.  CPL: adjacent columns per lane (1, or 2 with the native layout: the
 // cube rows are then read and written 8 bytes per lane and the (dt500, 1/chi500)
 // pairs 16, so a pair of cells costs the vector-memory instructions of one).
 // threads of a workgroup with strip length L (fewer threads per workgroup leave
@@ -114,7 +119,7 @@ struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) Arr {
 template <int L>
 constexpr int fast_max_threads() { return L >= 30 ? 512 : L >= 20 ? 768 : 1024; }
 
-template <int L, bool PJ, int MODE, int CPL>
+template <int L, bool PJ, int MODE, int CPL, int SR = 0>
 __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs<float> a) {
   using namespace dev;
   using T = float;
@@ -125,6 +130,7 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
   // 120 x 120 plane (13.3 ms against 13.9 with 3 and 14.1 with 6 for strips of 20).
   constexpr int PD = CPL == 1 ? (L >= 15 ? 4 : 3) : 1;
   constexpr uint32_t TB = sizeof(T);
+  static_assert(MODE != 4 || (CPL == 1 && SR >= 1 && SR <= L), "S rows in LDS: one column per lane, SR <= L");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, PW = a.PW, nxy = nx * ny;
   const int np = PW * (ny + 4);
@@ -198,6 +204,15 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
     sP[c] = p;
     if (MODE == 2) sS[c] = ldo(gS, go);
   }
+  if (MODE == 4) {
+    // slot s*SR + r-1 holds window row r (row s*L+yo+r) of strip s, r = 1..SR
+    for (int c = tid; c < a.nstrips * SR * nx; c += blockDim.x) {
+      const int q = c / nx, cx = c - q * nx;
+      const int sq = q / SR;
+      const int cy = wrap(sq * L + a.yo + 1 + (q - sq * SR), ny);
+      sS[c] = ldo(gS, (cx * a.gs_x + cy * a.gs_y) * TB);
+    }
+  }
   lds_barrier();
 
   for (int step = 0; step < a.nz; ++step) {
@@ -212,6 +227,14 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
     const Rsrc up_S = rsrc(up_ghost ? gS : a.S + (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f, ub);
     const uint32_t uy = up_ghost ? a.gs_y : a.cs_y;
     const uint32_t ux = up_ghost ? a.gs_x : a.cs_x;
+    // MODE 4: the taps' LDS slots less r*nx (window row r in slot strip*SR + r-1),
+    // opaque per plane like e0
+    int sa[CPL + 1];
+#pragma unroll
+    for (int j = 0; j <= CPL; ++j) {
+      sa[j] = (strip * SR - 1) * nx + xc[j];
+      if (MODE == 4) asm volatile("" : "+v"(sa[j]));
+    }
 
     // one upwind row for the CPL cells: I at CPL+3 columns, x and S at CPL+1
     struct RawRow { T i[CPL + 3], x[CPL + 1], s[CPL + 1]; };
@@ -227,6 +250,13 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
       if (MODE == 2) {
 #pragma unroll
         for (int j = 0; j <= CPL; ++j) q.s[j] = sS[e + 1 + j];
+      } else if (MODE == 4 && (r < 1 || r > L + 1)) {
+#pragma unroll
+        for (int j = 0; j <= CPL; ++j) q.s[j] = (T)0;  // rows whose S no step uses
+      } else if (MODE == 4 && r <= SR) {
+        // (r is a constant of the unrolled loop: LDS or cache is decided at compile time)
+#pragma unroll
+        for (int j = 0; j <= CPL; ++j) q.s[j] = sS[sa[j] + r * nx];
       } else {
         int g = g0 + r;
         g = g >= ny ? g - ny : g;
@@ -250,7 +280,7 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
       return w;
     };
 
-    T newI[CPL][L], newX[CPL][L], newS[CPL][MODE == 2 ? L : 1];
+    T newI[CPL][L], newX[CPL][L], newS[CPL][MODE == 2 || MODE == 4 ? L : 1];
     Rows w0 = reduce_row(read_row(0));
     Rows w1 = reduce_row(read_row(1));
     Rows w2 = reduce_row(read_row(2));
@@ -282,7 +312,7 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
         Inu = fmax(-a.imax, fmin(a.imax, Inu));
         newI[c][k] = Inu;
         if (MODE >= 1) newX[c][k] = xx;
-        if (MODE == 2) newS[c][k] = ss;
+        if (MODE == 2 || (MODE == 4 && (k < SR || k == L - 1))) newS[c][k] = ss;
         Jn.v[c] = in.J.v[c] + a.wj * Inu;
         if (PJ) {
           const T sgnchi = copysign(fmax(fabs(chiv), (T)1.17549435e-38f), chiv);
@@ -347,6 +377,21 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
         if (xedge) put(wb + (k - ny) * PW + dh, k);
       }
     }
+    if (MODE == 4) {
+      // row yb+k is window row k-yo of this strip (slot strip*SR + k-yo-1 while
+      // k-yo <= SR) and, for k = L-1 with yo = -2, window row 1 of the next strip
+      // (periodically: strip 0 after the last); the other rows to the spare row
+      // (selects, not branches: a branch per row costs registers)
+      const int c1 = -a.yo - 1;
+      const int spare = a.nstrips * SR;
+      const int nxt = strip + 1 == a.nstrips ? 0 : (strip + 1) * SR;
+#pragma unroll
+      for (int k = 0; k < SR; ++k) {  // rows SR .. L-2 are never held
+        const int rr = k + c1;
+        sS[(rr < SR ? strip * SR + rr : spare) * nx + x0] = newS[0][k];
+      }
+      sS[(c1 ? nxt : spare) * nx + x0] = newS[0][L - 1];
+    }
     if (a.exitI && step == a.nz - 1) {
       const Rsrc ex = rsrc(a.exitI + f * a.gs_f, a.gplane_bytes);
       uint32_t eo = (x0 * a.gs_x + yb * a.gs_y) * TB;
@@ -367,18 +412,22 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
   }
 }
 
-template <int L, bool PJ, int MODE, int CPL>
+template <int L, bool PJ, int MODE, int CPL, int SR = 0>
 int launch_fast(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, size_t lds) {
-  auto kern = sc2_fast<L, PJ, MODE, CPL>;
+  auto kern = sc2_fast<L, PJ, MODE, CPL, SR>;
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(nf), dim3(nt), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
   return M3DRT_OK;
 }
 
+// MODE 4's rows per strip held in LDS, by strip length (the largest that fits is taken)
+constexpr int kSR20[] = {8, 14};
+
 template <int L, int CPL>
-int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int mode, size_t np) {
+int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int mode, size_t np, int sr) {
   const size_t l2 = np * 2 * sizeof(float), l3 = np * 3 * sizeof(float);
+  const size_t l4 = l2 + (size_t)(a.nstrips * sr + 1) * a.nx * sizeof(float);
   if (a.PsiJ) {
     if constexpr (CPL == 1) {
       if (mode == 2) return launch_fast<L, true, 2, CPL>(ctx, a, nf, nt, l3);
@@ -387,6 +436,10 @@ int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int m
     return -1;
   }
   if (mode == 2) return launch_fast<L, false, 2, CPL>(ctx, a, nf, nt, l3);
+  if constexpr (CPL == 1 && L == 20) {
+    if (mode == 4 && sr == 14) return launch_fast<L, false, 4, CPL, 14>(ctx, a, nf, nt, l4);
+    if (mode == 4 && sr == 8) return launch_fast<L, false, 4, CPL, 8>(ctx, a, nf, nt, l4);
+  }
   return launch_fast<L, false, 1, CPL>(ctx, a, nf, nt, l2);
 }
 
@@ -399,19 +452,20 @@ int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int m
 constexpr int kL1[] = {4, 12, 15, 20, 24, 30};
 constexpr int kL2[] = {4, 6, 8};
 
-int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int cpl, int mode, size_t np) {
+int fast_dispatch(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int L, int cpl, int mode, size_t np,
+                  int sr) {
   if (cpl == 2) {
-    if (L == 4) return fast_variant<4, 2>(ctx, a, nf, nt, mode, np);
-    if (L == 6) return fast_variant<6, 2>(ctx, a, nf, nt, mode, np);
-    if (L == 8) return fast_variant<8, 2>(ctx, a, nf, nt, mode, np);
+    if (L == 4) return fast_variant<4, 2>(ctx, a, nf, nt, mode, np, sr);
+    if (L == 6) return fast_variant<6, 2>(ctx, a, nf, nt, mode, np, sr);
+    if (L == 8) return fast_variant<8, 2>(ctx, a, nf, nt, mode, np, sr);
     return -1;
   }
-  if (L == 4) return fast_variant<4, 1>(ctx, a, nf, nt, mode, np);
-  if (L == 12) return fast_variant<12, 1>(ctx, a, nf, nt, mode, np);
-  if (L == 15) return fast_variant<15, 1>(ctx, a, nf, nt, mode, np);
-  if (L == 20) return fast_variant<20, 1>(ctx, a, nf, nt, mode, np);
-  if (L == 24) return fast_variant<24, 1>(ctx, a, nf, nt, mode, np);
-  if (L == 30) return fast_variant<30, 1>(ctx, a, nf, nt, mode, np);
+  if (L == 4) return fast_variant<4, 1>(ctx, a, nf, nt, mode, np, sr);
+  if (L == 12) return fast_variant<12, 1>(ctx, a, nf, nt, mode, np, sr);
+  if (L == 15) return fast_variant<15, 1>(ctx, a, nf, nt, mode, np, sr);
+  if (L == 20) return fast_variant<20, 1>(ctx, a, nf, nt, mode, np, sr);
+  if (L == 24) return fast_variant<24, 1>(ctx, a, nf, nt, mode, np, sr);
+  if (L == 30) return fast_variant<30, 1>(ctx, a, nf, nt, mode, np, sr);
   return -1;
 }
 
@@ -454,13 +508,31 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   const int nstrips = ny / L;
   const int nt = (nx / cpl) * nstrips;
   const size_t np = (size_t)(nx + 4) * (ny + 4);
-  int mode = -1;
-  if (np * 3 * sizeof(T) <= (size_t)ctx->lds_max) mode = 2;
-  else if (np * 2 * sizeof(T) <= (size_t)ctx->lds_max) mode = 1;
-  if (mode < 0) return -1;
   const uint64_t span = (uint64_t)(nx - 1) * d->cs_x + (uint64_t)(ny - 1) * d->cs_y;
   const uint64_t gspan = (uint64_t)(nx - 1) * d->gs_x + (uint64_t)(ny - 1) * d->gs_y;
   if ((span + 1) * sizeof(T) >= (1ull << 32) || (gspan + 1) * sizeof(T) >= (1ull << 32)) return -1;
+  int mode = -1, sr = 0;
+  if (np * 3 * sizeof(T) <= (size_t)ctx->lds_max) {
+    mode = 2;
+  } else if (np * 2 * sizeof(T) <= (size_t)ctx->lds_max) {
+    mode = 1;
+    // S of the first sr tap rows of every strip (nstrips * sr rows of nx, and a
+    // spare row) in the LDS the pairs leave: J only, one column per lane, strips of
+    // 20 rows (the form that holds it without spilling registers)
+    const size_t rows = ((size_t)ctx->lds_max - np * 2 * sizeof(T)) / ((size_t)nx * sizeof(T));
+    const bool j_only = !(get_rates && b->PsiJ);
+    if (j_only && cpl == 1 && L == 20 && rows >= 1 && !getenv("M3DRT_FAST_NO_SROWS")) {
+      // the most rows per strip to hold: 8 unless M3DRT_FAST_SR says otherwise (on
+      // the 120 x 120 plane 13.35 ms per 1024-frequency direction with 8, 13.45 with
+      // 14, 13.79 with none)
+      const char *senv = getenv("M3DRT_FAST_SR");
+      const int smax = senv ? atoi(senv) : 8;
+      for (int c : kSR20)
+        if ((size_t)nstrips * c + 1 <= rows && c <= smax) sr = c;
+      if (sr) mode = 4;
+    }
+  }
+  if (mode < 0) return -1;
 
   const Sc2Geom g = sc2_geometry(d);
   const int64_t nxy = (int64_t)nx * ny;
@@ -504,9 +576,11 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   a.wj = (T)d->wj;
   if (getenv("M3DRT_VERBOSE"))
     fprintf(stderr, "[m3drt] sc2 fast %s nf=%d plane=%dx%d nz=%d: %d threads, %d column(s) per lane, strip %d, "
-            "LDS mode %d (%zu B)%s\n", f64 ? "f64" : "f32", d->nf, nx, ny, nz, nt, cpl, L, mode,
-            np * (mode == 2 ? 3 : 2) * sizeof(T), a.PsiJ ? ", PsiJ" : "");
-  return fast_dispatch(ctx, a, d->nf, nt, L, cpl, mode, np);
+            "LDS mode %d (%zu B, %d S rows of %d per strip in LDS)%s\n", f64 ? "f64" : "f32", d->nf, nx, ny, nz,
+            nt, cpl, L, mode,
+            np * (mode == 2 ? 3 : 2) * sizeof(T) + (mode == 4 ? (size_t)(nstrips * sr + 1) * nx * sizeof(T) : 0),
+            mode == 2 ? L + 1 : sr, L + 1, a.PsiJ ? ", PsiJ" : "");
+  return fast_dispatch(ctx, a, d->nf, nt, L, cpl, mode, np, sr);
 }
 
 }  // namespace

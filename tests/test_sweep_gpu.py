@@ -173,6 +173,26 @@ def test_sweep_exact_strips(ctx, nx, ny, fp64, psij, layout):
     (FP64): J, PsiJ and the exit plane against the oracle."""
     case = make_case(nf=3, nx=nx, ny=ny, nz=4, seed=nx + ny + int(fp64), vec=(0.61481017, 0.0, 0.78867513))
     print(f"exact strips {nx}x{ny} fp64={fp64} psij={psij} {layout}", flush=True)
+    check_exact_strips(ctx, case, fp64, psij, layout)
+
+
+@pytest.mark.parametrize("vec", [(0.31, 0.42, 0.85), (-0.31, 0.42, 0.85), (0.31, -0.42, 0.85), (-0.31, -0.42, 0.85),
+                                 (0.25, -0.35, -0.90)])
+def test_sweep_s_rows_in_lds(ctx, vec, monkeypatch, capfd):
+    """120 x 120 planes with J and the exit plane as the only outputs: the form that
+    holds the upwind S of the first strips' rows in LDS (beside the (I, x) plane)
+    and reads the other strips' S from the cache.  Every sign of the horizontal
+    step (both row offsets of the taps, so both placements of the periodic row),
+    upward and downward, over 9 planes so the held rows are rewritten several
+    times: J and the exit plane against the oracle."""
+    case = make_case(nf=2, nx=120, ny=120, nz=9, seed=77, vec=vec)
+    print(f"S rows in LDS 120x120 vec={vec}", flush=True)
+    monkeypatch.setenv("M3DRT_VERBOSE", "1")
+    check_exact_strips(ctx, case, False, False, "native")
+    assert "LDS mode 4" in capfd.readouterr().err
+
+
+def check_exact_strips(ctx, case, fp64, psij, layout):
     I, psi, _, _ = run_oracle(case, fp64=fp64)
     dt = torch.float64 if fp64 else torch.float32
     gz = case["gz"]
