@@ -77,6 +77,10 @@ class AliConfig:
                               # to 3 (SRC/m3d_patch_mod.f90:472-476)
     sor_w: float = 1.0        # &nlte_params sor_w: the update factor e -> e**sor_w
                               # (SRC/atom_mod.f90:255-266, 604-610); 1 is undamped
+    # the automatic Ng gate's &nlte_params (SRC/m3d_patch_mod.f90:405-406): overrides of
+    # NgAcceleration's defaults by name (rate_min, rate_tol, clamp_max, min_iters_left,
+    # cond_min, max_change, min_gap)
+    ng_gate: dict = field(default_factory=dict)
 
     def __post_init__(self):
         # Ng needs three iterates after an extrapolation before the next one
@@ -119,8 +123,13 @@ class NgAcceleration:
     11.  The gate's constants are the reference's &nlte_params defaults."""
     rate_min, rate_tol, clamp_max, min_iters_left = 0.5, 0.15, 1e-3, 3
     cond_min, max_change, min_gap = 1e-6, 10.0, 4
+    GATE = ("rate_min", "rate_tol", "clamp_max", "min_iters_left", "cond_min", "max_change", "min_gap")
 
-    def __init__(self):
+    def __init__(self, **gate):
+        for k, v in gate.items():
+            if k not in self.GATE:
+                raise ValueError(f"unknown Ng gate parameter {k!r} (one of {', '.join(self.GATE)})")
+            setattr(self, k, type(getattr(self, k))(v))
         self.d_cur = np.zeros((3, 3))
         self.d_done = np.zeros((3, 3))
         self.s_cur = self.s_done = 0.0
@@ -244,7 +253,7 @@ class NlteProblem:
         self._graph_out = None
         self.history = []
         self.ng_history = []
-        self.ng = NgAcceleration() if cfg.ng else None
+        self.ng = NgAcceleration(**cfg.ng_gate) if cfg.ng else None
         self._oldb = []
         self._iter = 0
         self._continuum()
@@ -501,11 +510,14 @@ def main():
     ap.add_argument("--ng-step", type=int, default=0, help="> 0: Ng on the reference's fixed schedule")
     ap.add_argument("--ng-start", type=int, default=8)
     ap.add_argument("--sor-w", type=float, default=1.0, help="&nlte_params sor_w (1: undamped)")
+    ap.add_argument("--ng-gate", action="append", default=[], metavar="NAME=VALUE",
+                    help="an automatic Ng gate parameter (&nlte_params ng_<NAME>), e.g. min_gap=3")
     args = ap.parse_args()
+    gate = dict(kv.split("=", 1) for kv in args.ng_gate)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
     cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph,
-                    ng_step=args.ng_step, ng_start=args.ng_start, sor_w=args.sor_w)
+                    ng_step=args.ng_step, ng_start=args.ng_start, sor_w=args.sor_w, ng_gate=gate)
     t0 = time.perf_counter()
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
@@ -519,7 +531,7 @@ def main():
                line_updates_per_iteration=p.ncell * p.atom.nline * cfg.nq * len(p.vec),
                b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph,
                ng=("fixed schedule" if cfg.ng_step > 0 else "automatic") if cfg.ng else "off",
-               ng_step=cfg.ng_step, ng_start=cfg.ng_start, sor_w=cfg.sor_w,
+               ng_step=cfg.ng_step, ng_start=cfg.ng_start, sor_w=cfg.sor_w, ng_gate=cfg.ng_gate,
                ng_fired=[i + 1 for i, h in enumerate(p.ng_history) if h])
     out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
     if args.fp64_sweeps:

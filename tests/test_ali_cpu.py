@@ -61,3 +61,19 @@ def test_ng_step_below_three_is_raised_to_three():
         assert cfg.ng_step == 3 and any("set to 3" in str(x.message) for x in w)
         assert [i for i in range(1, 20) if ng_fixed_fires(i, cfg.ng_step, cfg.ng_start)] == [9, 12, 15, 18]
     assert AliConfig(ng_step=0).ng_step == 0 and AliConfig(ng_step=4).ng_step == 4
+
+
+def test_ng_gate_parameters_reach_the_gate():
+    """The automatic gate's &nlte_params (SRC/m3d_patch_mod.f90:405-406) set through
+    AliConfig.ng_gate: a contraction-rate floor no iteration reaches keeps Ng from
+    firing at all, where the defaults fire; an unknown name is an error."""
+    import pytest
+    from dispatch_amd.ali import NgAcceleration
+    p = NlteProblem(OracleCtx(), small(max_iter=15), device="cpu")
+    p.solve()
+    q = NlteProblem(OracleCtx(), small(max_iter=15, ng_gate={"rate_min": 0.9999}), device="cpu")
+    q.solve()
+    assert any(p.ng_history) and not any(q.ng_history)
+    assert NgAcceleration(min_gap="3").min_gap == 3 and NgAcceleration.min_gap == 4
+    with pytest.raises(ValueError, match="unknown Ng gate parameter"):
+        NgAcceleration(gap=3)
