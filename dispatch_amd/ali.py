@@ -512,6 +512,9 @@ def main():
     ap.add_argument("--sor-w", type=float, default=1.0, help="&nlte_params sor_w (1: undamped)")
     ap.add_argument("--ng-gate", action="append", default=[], metavar="NAME=VALUE",
                     help="an automatic Ng gate parameter (&nlte_params ng_<NAME>), e.g. min_gap=3")
+    ap.add_argument("--lockstep", action="store_true",
+                    help="with --fp64-sweeps: step both runs together for --max-iter iterations and report "
+                         "their departure coefficients' difference at every iteration")
     args = ap.parse_args()
     gate = dict(kv.split("=", 1) for kv in args.ng_gate)
     log = lambda m: print(m, file=sys.stderr, flush=True)
@@ -522,6 +525,21 @@ def main():
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
     log(f"[ali] set-up {time.perf_counter() - t0:.1f} s")
+    if args.lockstep:
+        if not args.fp64_sweeps:
+            ap.error("--lockstep compares with the FP64-sweep run: give --fp64-sweeps")
+        q = NlteProblem(ctx, cfg, sweep_dtype=torch.float64)
+        rows = []
+        for it in range(1, cfg.max_iter + 1):
+            r32, r64 = p.iterate(), q.iterate()
+            d = float((p.b / q.b - 1).abs().max())
+            rows.append(dict(iteration=it, maxerr_fp32=r32["maxerr"], maxerr_fp64=r64["maxerr"], ng_fp32="ng" in r32,
+                             ng_fp64="ng" in r64, b_max_rel_diff=d))
+            log(f"[ali] iteration {it}: max|dn/n| {r32['maxerr']:.3e} (FP32 sweeps) {r64['maxerr']:.3e} (FP64); "
+                f"b differ by {d:.3e}")
+        ctx.close()
+        print(json.dumps(dict(workload="3d_nlte_ali", box=[cfg.nx, cfg.ny, cfg.nz], lockstep=rows)), flush=True)
+        return
     n = p.solve(log=log)
     its = [dt for _, dt in p.history]
     out = dict(workload="3d_nlte_ali", box=[cfg.nx, cfg.ny, cfg.nz], levels=p.atom.nk, lines=p.atom.nline,
