@@ -77,6 +77,10 @@ class AliConfig:
                               # to 3 (SRC/m3d_patch_mod.f90:472-476)
     sor_w: float = 1.0        # &nlte_params sor_w: the update factor e -> e**sor_w
                               # (SRC/atom_mod.f90:255-266, 604-610); 1 is undamped
+    conv_pop_floor: float = 0.0  # &nlte_params conv_pop_floor: a level holding less than this share
+                              # of the atoms counts in the convergence measure with its change
+                              # scaled by its share over the floor (SRC/atom_mod.f90:233-236,
+                              # 673-686); 0 weights every level fully
     # the automatic Ng gate's &nlte_params (SRC/m3d_patch_mod.f90:405-406): overrides of
     # NgAcceleration's defaults by name (rate_min, rate_tol, clamp_max, min_iters_left,
     # cond_min, max_change, min_gap)
@@ -446,7 +450,7 @@ class NlteProblem:
     def update(self, gij, gji):
         """update_populations of every cell in place; returns the convergence measure."""
         return self.ctx.rate_solve(self.atom_dev, self.e_kT64, self.temp, self.totn, self.nstar, self.b, gij, gji,
-                                   self.cup, sor_w=self.cfg.sor_w)
+                                   self.cup, sor_w=self.cfg.sor_w, conv_pop_floor=self.cfg.conv_pop_floor)
 
     def iterate(self):
         """One ALI iteration: the rates, the statistical-equilibrium update and, when
@@ -512,6 +516,9 @@ def main():
     ap.add_argument("--sor-w", type=float, default=1.0, help="&nlte_params sor_w (1: undamped)")
     ap.add_argument("--ng-gate", action="append", default=[], metavar="NAME=VALUE",
                     help="an automatic Ng gate parameter (&nlte_params ng_<NAME>), e.g. min_gap=3")
+    ap.add_argument("--conv-pop-floor", type=float, default=0.0,
+                    help="&nlte_params conv_pop_floor: levels below this share of the atoms weigh in the "
+                         "convergence measure by their share (0: every level fully)")
     ap.add_argument("--lockstep", action="store_true",
                     help="with --fp64-sweeps: step both runs together for --max-iter iterations and report "
                          "their departure coefficients' difference at every iteration")
@@ -520,7 +527,8 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = m3drt.Context(0)
     cfg = AliConfig(nx=args.nx, ny=args.nx, nz=args.nz, max_iter=args.max_iter, graph=not args.no_graph,
-                    ng_step=args.ng_step, ng_start=args.ng_start, sor_w=args.sor_w, ng_gate=gate)
+                    ng_step=args.ng_step, ng_start=args.ng_start, sor_w=args.sor_w, ng_gate=gate,
+                    conv_pop_floor=args.conv_pop_floor)
     t0 = time.perf_counter()
     p = NlteProblem(ctx, cfg)
     torch.cuda.synchronize()
@@ -550,6 +558,7 @@ def main():
                b_min=float(p.b.min()), b_max=float(p.b.max()), rate_pass_graph=cfg.graph,
                ng=("fixed schedule" if cfg.ng_step > 0 else "automatic") if cfg.ng else "off",
                ng_step=cfg.ng_step, ng_start=cfg.ng_start, sor_w=cfg.sor_w, ng_gate=cfg.ng_gate,
+               conv_pop_floor=cfg.conv_pop_floor,
                ng_fired=[i + 1 for i, h in enumerate(p.ng_history) if h])
     out["line_updates_per_s"] = out["line_updates_per_iteration"] / out["s_per_iteration"]
     if args.fp64_sweeps:
