@@ -59,17 +59,20 @@ def test_ali_strong_collisions_give_lte(ctx):
     assert np.abs(b[:, :cfg.nz // 2] - 1).max() < 1e-3, np.abs(b[:, :cfg.nz // 2] - 1).max()
 
 
-def test_window_opacity_equals_the_sequence(ctx):
+@pytest.mark.parametrize("kw", [{}, dict(nq=50, lines_per_window=4)])
+def test_window_opacity_equals_the_sequence(ctx, kw):
     """m3drt_window_opacity against the sequence it fuses (the continuum product,
     convert_to_work_units, the seeded split tracking, bb_opacity, work_to_transfer) on
     every window and direction of the small problem at non-LTE populations: chi, S and
-    the six tracking arrays bit for bit (both round every product and sum apart)."""
+    the six tracking arrays bit for bit (both round every product and sum apart).  With
+    50 frequencies per line a window of 4 lines has 200 frequencies, which the kernel
+    splits into runs of 67 per lane, their ends inside line windows."""
     from dispatch_amd.ali import NlteProblem
     from dispatch_amd import synth
-    p = NlteProblem(ctx, small())
+    p = NlteProblem(ctx, small(**kw))
     g = torch.Generator(device="cuda").manual_seed(3)
     p.b.mul_(torch.exp(0.3 * torch.randn(p.b.shape, generator=g, dtype=torch.float64, device="cuda")))
-    print("window opacity fused vs sequence", flush=True)
+    print(f"window opacity fused vs sequence {kw}", flush=True)
     for wi, w in enumerate(p.windows):
         l0, l1 = w.lines[0], w.lines[-1] + 1
         nf, nc = w.nf, p.ncell
