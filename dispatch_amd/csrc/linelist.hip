@@ -15,6 +15,8 @@
 #include "m3drt_internal.hpp"
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <string>
 
@@ -114,20 +116,44 @@ extern "C" int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist
   const int nf = a->nf;
   const int64_t nc = a->ncell, n2 = (int64_t)nf * nc;
   const int64_t sf = nc, sc = 1;
-  // scratch: chi_l, eta_l (f, cell) and idf, vb2, lnbase (cell).  logref =
-  // ln(tol/voigt_peak) + ln all_chi is formed by the opacity kernel at the nodes it
-  // needs: all_chi is not touched until every list has been summed.
-  const size_t bytes = sizeof(float) * (size_t)(2 * n2 + 3 * nc);
+  // One list of at most 4 lines and nothing else (no other list, no scattering split,
+  // no copy of the increments asked for): the summation kernel deposits each cell's
+  // lines once, so it adds their sums straight to the working opacity, with the
+  // increments' rounding, and the increments with their zeroing and adding passes
+  // fall away.
+  int nlists = 0, nl_direct = 0;
+  bool scat_direct = false;
+  for (int s = 0; s < a->nsp; ++s)
+    for (int k = 0; k < a->sp[s].niso; ++k)
+      if (a->sp[s].iso[k].lines.nline > 0) {
+        ++nlists;
+        nl_direct = a->sp[s].iso[k].lines.nline;
+        scat_direct = !a->sp[s].dep && a->sp[s].scat_group && a->chi_scat;
+      }
+  const bool direct = nlists == 1 && nl_direct <= 4 && !scat_direct && !a->chi_lines && !a->eta_lines &&
+                      !getenv("M3DRT_NO_DIRECT_LINES");
+  // scratch: chi_l, eta_l (f, cell) unless direct, and idf, vb2, lnbase (cell).
+  // logref = ln(tol/voigt_peak) + ln all_chi is formed by the opacity kernel at the
+  // nodes it needs: all_chi is not otherwise touched until every list has been summed.
+  const int64_t nl2 = direct ? 0 : n2;
+  const size_t bytes = sizeof(float) * (size_t)(2 * nl2 + 3 * nc);
   int rc = m3drt::ensure_scratch(ctx, bytes);
   if (rc) return rc;
   float *chi_l = static_cast<float *>(ctx->scratch);
-  float *eta_l = chi_l + n2, *idf = eta_l + n2, *vb2 = idf + nc, *lnb = vb2 + nc;
+  float *eta_l = chi_l + nl2, *idf = eta_l + nl2, *vb2 = idf + nc, *lnb = vb2 + nc;
   hipStream_t st = ctx->stream;
   const dim3 g2((unsigned)((nc + 255) / 256), (unsigned)nf);
   const unsigned g1 = (unsigned)((nc + 255) / 256);
-  hipLaunchKernelGGL(m3drt::zero_kernel, dim3((unsigned)std::min<int64_t>((2 * n2 + 255) / 256, 1 << 20)), dim3(256),
-                     0, st, 2 * n2, chi_l);
-  M3DRT_HIP(hipGetLastError());
+  if (direct) {
+    chi_l = a->all_chi;
+    eta_l = a->all_eta;
+    if (getenv("M3DRT_VERBOSE"))
+      fprintf(stderr, "[m3drt] add_linelist_opacities: one list of %d lines, deposited directly\n", nl_direct);
+  } else {
+    hipLaunchKernelGGL(m3drt::zero_kernel, dim3((unsigned)std::min<int64_t>((2 * n2 + 255) / 256, 1 << 20)),
+                       dim3(256), 0, st, 2 * n2, chi_l);
+    M3DRT_HIP(hipGetLastError());
+  }
   for (int s = 0; s < a->nsp; ++s) {
     const m3drt_species &sp = a->sp[s];
     for (int k = 0; k < sp.niso; ++k) {
@@ -162,13 +188,15 @@ extern "C" int m3drt_add_linelist_opacities(m3drt_ctx *ctx, const m3drt_linelist
       o.logref = nullptr;
       o.one_m_eps = (o.flags & M3DRT_OPAC_SCAT) ? a->one_m_eps + (int64_t)(sp.scat_group - 1) * n2 : nullptr;
       o.chi_scat = (o.flags & M3DRT_OPAC_SCAT) ? a->chi_scat : nullptr;
-      rc = m3drt::line_opacity(ctx, &iso.lines, &o, a->equidistant == 0, true);
+      rc = m3drt::line_opacity(ctx, &iso.lines, &o, a->equidistant == 0, true, direct);
       if (rc) return rc;
     }
   }
-  hipLaunchKernelGGL(m3drt::add_lines_kernel, g2, dim3(256), 0, st, nf, nc, sf, sc, chi_l, eta_l, a->all_chi,
-                     a->all_eta, a->chi_lines, a->eta_lines);
-  M3DRT_HIP(hipGetLastError());
+  if (!direct) {
+    hipLaunchKernelGGL(m3drt::add_lines_kernel, g2, dim3(256), 0, st, nf, nc, sf, sc, chi_l, eta_l, a->all_chi,
+                       a->all_eta, a->chi_lines, a->eta_lines);
+    M3DRT_HIP(hipGetLastError());
+  }
   if (a->chi_scat && a->beta) {
     hipLaunchKernelGGL(m3drt::scat_physical_kernel, g2, dim3(256), 0, st, nf, nc, sf, sc, a->beta, a->chi_scat);
     M3DRT_HIP(hipGetLastError());

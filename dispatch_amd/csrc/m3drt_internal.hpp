@@ -48,9 +48,13 @@ int sweep_lc2(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs 
 int qlg_tables(m3drt_ctx *ctx);
 // lref_from_cont: the tolerance reference logref is ln(tol/voigt_peak) + ln cont,
 // formed at the nodes the lines need instead of read from o->logref (the line-list
-// composition, whose cont is the working opacity the reference forms logref from)
+// composition, whose cont is the working opacity the reference forms logref from).
+// direct: o->chi and o->eta are that working opacity (the same array as o->cont), for
+// a list of at most 4 lines alone in its composition and without a scattering split:
+// each point's line sum, formed from zero, is added to it once, as the composition's
+// zeroed increments would be.
 int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general,
-                 bool lref_from_cont = false);
+                 bool lref_from_cont = false, bool direct = false);
 int rate_solve(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, m3drt_rate_solve_result *res);
 int normalize_rates(m3drt_ctx *ctx, const m3drt_rate_solve_args *r, const float *wphi);
 int bb_rates(m3drt_ctx *ctx, const m3drt_bb_rates_args *r);

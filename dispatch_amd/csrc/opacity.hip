@@ -217,7 +217,11 @@ __device__ __forceinline__ float bot_eval(const Coef &c, float x, const float *T
   return G * c.gG + L * p;
 }
 
-template <bool GENERAL>
+// DIRECT: chi and eta are the working opacity itself, for a list of at most K lines
+// alone in its composition (one deposit per cell, after every read of cont): each
+// point's line sum is formed from zero in line order and added to the working value
+// at the deposit, as the increments of the composition would be added.
+template <bool GENERAL, bool DIRECT>
 __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
   extern __shared__ float tab[];  // TGX then TH2X
   for (int i = threadIdx.x; i <= QLG_NX; i += blockDim.x) {
@@ -285,13 +289,20 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
 #pragma unroll
       for (int k = 0; k < K; ++k) any = any || (k < np && pend[k].fa <= f + U - 1 && pend[k].fb >= f);
       if (!any) continue;
-      float cv[U], ev[U], sv[U], ov[U];
+      float cv[U], ev[U], sv[U], ov[U], cb[DIRECT ? U : 1], eb[DIRECT ? U : 1];
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         if (f + j <= Fb) {
           const int64_t o = (int64_t)(f + j - 1) * sf;
-          cv[j] = chi[o];
-          ev[j] = eta[o];
+          if (DIRECT) {
+            cb[j] = chi[o];
+            eb[j] = eta[o];
+            cv[j] = 0.0f;
+            ev[j] = 0.0f;
+          } else {
+            cv[j] = chi[o];
+            ev[j] = eta[o];
+          }
           if (anyscat) {
             sv[j] = csc_c[o];
             ov[j] = ome_c[o];
@@ -322,8 +333,13 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       for (int j = 0; j < U; ++j) {
         if (f + j <= Fb) {
           const int64_t o = (int64_t)(f + j - 1) * sf;
-          chi[o] = cv[j];
-          eta[o] = ev[j];
+          if (DIRECT) {
+            chi[o] = cb[j] + cv[j];
+            eta[o] = eb[j] + ev[j];
+          } else {
+            chi[o] = cv[j];
+            eta[o] = ev[j];
+          }
           if (anyscat) csc_c[o] = sv[j];
         }
       }
@@ -375,7 +391,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       // the tolerance reference: tol * continuum, floored by tol^2 * the opacity
       // already at the centre node (:853-870; the reference caches it per node until
       // a line deposits, which is the same value)
-      float chi_f0 = chi[(f0 - 1) * sf];
+      float chi_f0 = DIRECT ? 0.0f : chi[(f0 - 1) * sf];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (k < np && pend[k].fa <= f0 && f0 <= pend[k].fb) {
@@ -558,7 +574,9 @@ int qlg_tables(m3drt_ctx *ctx) {
 }
 
 int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_args *o, bool general,
-                 bool lref_from_cont) {
+                 bool lref_from_cont, bool direct) {
+  if (direct && (ll->nline > 4 || (o->flags & M3DRT_OPAC_SCAT)))
+    return set_error(M3DRT_EINVAL, "line_opacity: the direct deposit takes at most 4 lines and no scattering split");
   int rc = qlg_tables(ctx);
   if (rc) return rc;
   OpacArgs a;
@@ -593,7 +611,8 @@ int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_a
   const int threads = 256;
   const int blocks = (o->ncell + threads - 1) / threads;
   const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
-  auto kern = general ? opacity_kernel<true> : opacity_kernel<false>;
+  auto kern = direct ? (general ? opacity_kernel<true, true> : opacity_kernel<false, true>)
+                    : (general ? opacity_kernel<true, false> : opacity_kernel<false, false>);
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());

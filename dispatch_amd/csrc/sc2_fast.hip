@@ -421,7 +421,8 @@ int launch_fast(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, size_t
   return M3DRT_OK;
 }
 
-// MODE 4's rows per strip held in LDS, by strip length (the largest that fits is taken)
+// MODE 4's rows per strip held in LDS for strips of 20 rows (the largest that fits
+// and does not exceed the cap in run_fast is taken)
 constexpr int kSR20[] = {8, 14};
 
 template <int L, int CPL>
@@ -522,9 +523,10 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
     const size_t rows = ((size_t)ctx->lds_max - np * 2 * sizeof(T)) / ((size_t)nx * sizeof(T));
     const bool j_only = !(get_rates && b->PsiJ);
     if (j_only && cpl == 1 && L == 20 && rows >= 1 && !getenv("M3DRT_FAST_NO_SROWS")) {
-      // the most rows per strip to hold: 8 unless M3DRT_FAST_SR says otherwise (on
-      // the 120 x 120 plane 13.35 ms per 1024-frequency direction with 8, 13.45 with
-      // 14, 13.79 with none)
+      // the most rows per strip to hold: 8 unless M3DRT_FAST_SR says otherwise.  On
+      // the 120 x 120 plane, per 1024-frequency direction: 13.35 ms with 8 and 13.45
+      // with 14 over all 24 directions, 13.32 and 13.10 over four of them on another
+      // box; 13.5-13.8 with none.
       const char *senv = getenv("M3DRT_FAST_SR");
       const int smax = senv ? atoi(senv) : 8;
       for (int c : kSR20)

@@ -66,3 +66,46 @@ def test_composition_matches_oracle(ctx, equidistant, scat):
         got = back(dkw["chi_scat"])
         assert np.max(np.abs(got - csc_o)) <= 2e-6 * max(np.abs(csc_o).max(), 1e-30)
         assert np.abs(csc_o).max() > 0
+
+
+@pytest.mark.parametrize("equidistant,nl,nlte", [(True, 3, False), (False, 4, False), (True, 4, True)])
+def test_short_list_deposits_directly(ctx, equidistant, nl, nlte, monkeypatch, capfd):
+    """A composition of one list of at most 4 lines (no other list, no scattering
+    split, no copy of the increments): the summation kernel adds each point's line sum
+    straight to the working opacity.  Against the oracle's composition, and bit for
+    bit against the path through zeroed increments that are added at the end."""
+    from linelist_cases import lines_of
+    c = make_case(seed=17 + nl, equidistant=equidistant, scat=False)
+    rng = np.random.default_rng(nl)
+    llam = c["llam"]
+    dl = (llam[-1] - llam[0]) / (len(llam) - 1)
+    ncell = c["all_chi"].shape[0]
+    sp = dict(isotopes=[(lines_of(rng, nl, llam, dl, 4 if nlte else 0), 40.0, 1.0)],
+              n=10 ** rng.uniform(8, 12, ncell), part=rng.uniform(0.5, 4, ncell), scale=1.0, scat_group=0)
+    if nlte:
+        sp["dep"] = rng.uniform(0.5, 2.0, (4, ncell)).astype(np.float32)
+        sp["nlev"] = 4
+    species = [sp]
+    print(f"direct deposit equidistant={equidistant} lines={nl} nlte={nlte}", flush=True)
+    chi_o, eta_o, _, dchi_o, _ = O.add_linelist_opacities(species, llam, c["all_chi"], c["all_eta"], c["cells"],
+                                                          equidistant=equidistant)
+    T = lambda a: d(np.ascontiguousarray(np.asarray(a, np.float32).T))
+    cells = {k: d(v, np.float32) for k, v in c["cells"].items()}
+    out = []
+    for env in (None, "1"):
+        if env:
+            monkeypatch.setenv("M3DRT_NO_DIRECT_LINES", env)
+        monkeypatch.setenv("M3DRT_VERBOSE", "1")
+        chi, eta = T(c["all_chi"]), T(c["all_eta"])
+        ctx.add_linelist_opacities(dev_species(species), d(llam, np.float64), chi, eta, cells,
+                                   equidistant=equidistant)
+        torch.cuda.synchronize()
+        err = capfd.readouterr().err
+        assert ("deposited directly" in err) == (env is None)
+        out.append((chi, eta))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    back = lambda t: t.cpu().numpy().T
+    scale = np.abs(dchi_o).max()
+    assert scale > 0
+    assert np.max(np.abs(back(out[0][0]) - chi_o)) <= 2e-6 * scale
+    assert np.max(np.abs(back(out[0][1]) - eta_o)) <= 2e-6 * scale
