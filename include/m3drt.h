@@ -633,7 +633,7 @@ int m3drt_reference_strides(m3drt_sweep_desc *desc);
  * A composition of one list of at most 4 lines (no other list, no scattering split,
  * chi_lines and eta_lines NULL) is summed without the increments: each point's line
  * sum, formed from zero in line order, is added to all_chi / all_eta where the lines
- * are deposited, with the same result bit for bit and no (f, cell) scratch. */
+ * are deposited, with the same result bit for bit and no (f, cell) scratch.
  * All (f, cell) arrays have cells fastest (f*ncell + cell): all_chi, all_eta,
  * chi_scat, beta, chi_lines, eta_lines, and one_m_eps (f, cell, 2).  Species and
  * isotope descriptors are HOST arrays, their members DEVICE arrays. */
