@@ -192,6 +192,44 @@ def test_sweep_s_rows_in_lds(ctx, vec, monkeypatch, capfd):
     assert "LDS mode 4" in capfd.readouterr().err
 
 
+@pytest.mark.parametrize("nx,ny,vec,maser,layout", [(60, 60, (0.3, -0.4, 0.866), False, "native"),
+                                                     (60, 60, (-0.25, 0.3, -0.92), True, "native"),
+                                                     (120, 120, (0.3, 0.2, 0.93), False, "reference")])
+def test_sweep_split_outputs_fast_form(ctx, nx, ny, vec, maser, layout, monkeypatch, capfd):
+    """The ALI's split set alone (I, Psi, Irem and escape with rates, no J): the
+    straight-line kernel's split form (S in LDS on the 60 x 60 plane, from the cache on
+    120 x 120), with masing cells (negative chi) on one case, against the oracle's
+    short_char2 with split rates: I, Psi, Irem, escape and the exit plane."""
+    case = make_case(nf=3, nx=nx, ny=ny, nz=6, seed=nx + int(maser), vec=vec, maser=maser)
+    print(f"split outputs fast form {nx}x{ny} maser={maser} {layout}", flush=True)
+    I, psi, irem, esc = run_oracle(case, split_rates=True)
+    gz = case["gz"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(interior(a)))
+        pla = lambda a: dev(plane_to_native(a[gz]))
+        nat = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pnat = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        cub = lambda a: dev(interior(a))
+        pla = lambda a: dev(a[gz])
+        nat = pnat = lambda t: t.double().cpu().numpy()
+    c, s = cub(case["chi"]), cub(case["S"])
+    outs = {k: torch.full_like(c, 7.0) for k in ("I_out", "Psi_out", "Irem_out", "escape_out")}
+    exitI = torch.zeros_like(pla(case["chi"]))
+    monkeypatch.setenv("M3DRT_VERBOSE", "1")
+    ctx.sc_sweep(c, s, pla(case["chi"]), pla(case["S"]), pla(case["I"]), dev(case["chi500"]),
+                 dev(case["chi500_up"]), dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"],
+                 split_rates=True, get_rates=True, exit_I=exitI, layout=layout, **outs)
+    torch.cuda.synchronize()
+    assert "split outputs" in capfd.readouterr().err
+    check32(nat(outs["I_out"]), interior(I), "I")
+    check32(nat(outs["Psi_out"]), psi, "Psi")
+    check32(nat(outs["Irem_out"]), irem, "Irem")
+    check32(nat(outs["escape_out"]), esc, "escape")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(pnat(exitI), I[last], "exit plane")
+
+
 def check_exact_strips(ctx, case, fp64, psij, layout):
     I, psi, _, _ = run_oracle(case, fp64=fp64)
     dt = torch.float64 if fp64 else torch.float32
