@@ -36,7 +36,7 @@ struct FastArgs {
   const T *gchi, *gS, *gI;
   const Pair<T> *dr;  // per cell (dt500, 1/chi500), planes 0..nz-1, then the ghost plane (0, 1/g500)
   T *J, *PsiJ, *exitI;
-  T *Iout, *Psiout, *Irem, *esc;  // SPL: the ALI's split outputs instead of J
+  T *Iout, *Psiout, *Irem, *esc;  // OM 1: the ALI's split outputs instead of J; OM 2: Iout alone
   int64_t cs_f, cs_z, gs_f;
   uint32_t cs_x, cs_y, gs_x, gs_y;
   int nx, ny, nz, dir, xo, yo, PW, nstrips;
@@ -120,10 +120,12 @@ struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) Arr {
 template <int L>
 constexpr int fast_max_threads() { return L >= 30 ? 512 : L >= 20 ? 768 : 1024; }
 
-// SPL: no J; every cell's I, Psi (= Lambda*/chi), Irem and escape written instead, as
-// the general kernel's split-rate form writes them for a cell without eta flags
-// (SRC/rt_solvers_mod.f90:419-440, 465-475; the maser limit on |I| before the clamp).
-template <int L, bool PJ, int MODE, int CPL, int SR = 0, bool SPL = false>
+// OM (outputs): 0, J (and PsiJ) and the exit plane; 1, no J: every cell's I, Psi
+// (= Lambda*/chi), Irem and escape written instead, as the general kernel's split-rate
+// form writes them for a cell without eta flags (SRC/rt_solvers_mod.f90:419-440,
+// 465-475; the maser limit on |I| before the clamp); 2, no J: every cell's I (the
+// intensity cube an adjoint sweep recomputes).
+template <int L, bool PJ, int MODE, int CPL, int SR = 0, int OM = 0>
 __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs<float> a) {
   using namespace dev;
   using T = float;
@@ -135,7 +137,8 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
   constexpr int PD = CPL == 1 ? (L >= 15 ? 4 : 3) : 1;
   constexpr uint32_t TB = sizeof(T);
   static_assert(MODE != 4 || (CPL == 1 && SR >= 1 && SR <= L), "S rows in LDS: one column per lane, SR <= L");
-  static_assert(!SPL || (CPL == 1 && !PJ && MODE != 4), "split outputs: one column per lane, no PsiJ, S rows whole");
+  constexpr bool SPL = OM == 1, IOUT = OM >= 1;
+  static_assert(OM == 0 || (CPL == 1 && !PJ && MODE != 4), "cell outputs: one column per lane, no PsiJ, S rows whole");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, PW = a.PW, nxy = nx * ny;
   const int np = PW * (ny + 4);
@@ -179,8 +182,8 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
     P.J = a.J ? rsrc(a.J + o, a.plane_bytes) : rsrc(a.chi + o, 0u);
     P.PsJ = PJ ? rsrc(a.PsiJ + o, a.plane_bytes) : P.J;
     P.dr = rsrc(a.dr + (int64_t)z * nxy, (uint32_t)(nxy * sizeof(Pair<T>)));
+    if (IOUT) P.Io = rsrc(a.Iout + o, a.plane_bytes);
     if (SPL) {
-      P.Io = rsrc(a.Iout + o, a.plane_bytes);
       P.Po = rsrc(a.Psiout + o, a.plane_bytes);
       P.Ro = rsrc(a.Irem + o, a.plane_bytes);
       P.Eo = rsrc(a.esc + o, a.plane_bytes);
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
     In v;
     v.chi = bld<V>(P.chi, voff, k * rowb);
     v.S = bld<V>(P.S, voff, k * rowb);
-    if (!SPL) v.J = bld<V>(P.J, voff, k * rowb);
+    if (!IOUT) v.J = bld<V>(P.J, voff, k * rowb);
     if (PJ) v.PsJ = bld<V>(P.PsJ, voff, k * rowb);
     v.dr = bld<DV>(P.dr, doff, k * drowb);
     return v;
@@ -331,14 +334,13 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
         newI[c][k] = Inu;
         if (MODE >= 1) newX[c][k] = xx;
         if (MODE == 2 || (MODE == 4 && (k < SR || k == L - 1))) newS[c][k] = ss;
+        if (IOUT) Io.v[c] = Inu;
         if (SPL) {
-          Io.v[c] = Inu;
           Po.v[c] = psio;
           Ro.v[c] = soff ? Inu : Ir;
           Eo.v[c] = soff ? (T)1 : Es;
-        } else {
-          Jn.v[c] = in.J.v[c] + a.wj * Inu;
         }
+        if (!IOUT) Jn.v[c] = in.J.v[c] + a.wj * Inu;
         if (PJ) {
           const T sgnchi = copysign(fmax(fabs(chiv), (T)1.17549435e-38f), chiv);
           const bool off = fmin(dt, chiv) <= (T)0;
@@ -346,14 +348,13 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
           PJn.v[c] = in.PsJ.v[c] + a.wj * psiout;
         }
       }
+      if (IOUT) bst<V>(P.Io, voff, k * rowb, Io);
       if (SPL) {
-        bst<V>(P.Io, voff, k * rowb, Io);
         bst<V>(P.Po, voff, k * rowb, Po);
         bst<V>(P.Ro, voff, k * rowb, Ro);
         bst<V>(P.Eo, voff, k * rowb, Eo);
-      } else {
-        bst<V>(P.J, voff, k * rowb, Jn);
       }
+      if (!IOUT) bst<V>(P.J, voff, k * rowb, Jn);
       if (PJ) bst<V>(P.PsJ, voff, k * rowb, PJn);
       w0 = w1;
       w1 = w2;
@@ -444,9 +445,9 @@ __global__ void __launch_bounds__(fast_max_threads<L>()) sc2_fast(const FastArgs
   }
 }
 
-template <int L, bool PJ, int MODE, int CPL, int SR = 0, bool SPL = false>
+template <int L, bool PJ, int MODE, int CPL, int SR = 0, int OM = 0>
 int launch_fast(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, size_t lds) {
-  auto kern = sc2_fast<L, PJ, MODE, CPL, SR, SPL>;
+  auto kern = sc2_fast<L, PJ, MODE, CPL, SR, OM>;
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(nf), dim3(nt), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
@@ -461,10 +462,15 @@ template <int L, int CPL>
 int fast_variant(m3drt_ctx *ctx, const FastArgs<float> &a, int nf, int nt, int mode, size_t np, int sr) {
   const size_t l2 = np * 2 * sizeof(float), l3 = np * 3 * sizeof(float);
   const size_t l4 = l2 + (size_t)(a.nstrips * sr + 1) * a.nx * sizeof(float);
-  if (a.Iout) {  // the split outputs, for the strip lengths the NLTE planes take
+  if (a.Iout) {  // per-cell outputs (the split set, or I alone), for the strip lengths 4 to 20
     if constexpr (CPL == 1 && (L == 4 || L == 12 || L == 15 || L == 20)) {
-      if (mode == 2) return launch_fast<L, false, 2, CPL, 0, true>(ctx, a, nf, nt, l3);
-      if (mode == 1) return launch_fast<L, false, 1, CPL, 0, true>(ctx, a, nf, nt, l2);
+      if (a.Psiout) {
+        if (mode == 2) return launch_fast<L, false, 2, CPL, 0, 1>(ctx, a, nf, nt, l3);
+        if (mode == 1) return launch_fast<L, false, 1, CPL, 0, 1>(ctx, a, nf, nt, l2);
+      } else {
+        if (mode == 2) return launch_fast<L, false, 2, CPL, 0, 2>(ctx, a, nf, nt, l3);
+        if (mode == 1) return launch_fast<L, false, 1, CPL, 0, 2>(ctx, a, nf, nt, l2);
+      }
     }
     return -1;
   }
@@ -519,8 +525,11 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   // split set (I, Psi, Irem, escape with rates, no J), with or without the exit plane
   const bool spl = (d->flags & M3DRT_SPLIT_RATES) && get_rates && b->I_out && b->Psi_out && b->Irem_out &&
                    b->escape_out && !b->J && !b->PsiJ;
+  // or the intensity cube alone (no J, no rates), with or without the exit plane
+  const bool iout = !spl && b->I_out && !b->J && !b->PsiJ && !b->Psi_out && !b->Irem_out && !b->escape_out &&
+                    !(d->flags & M3DRT_SPLIT_RATES);
   if (b->eta_row) return -1;
-  if (!spl) {
+  if (!spl && !iout) {
     if (b->I_out || (d->flags & M3DRT_SPLIT_RATES) || (get_rates && b->Psi_out)) return -1;
     if (!b->J && (!b->exit_I || (get_rates && b->PsiJ))) return -1;
   }
@@ -531,7 +540,8 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   // 120 x 120 plane (14.9 against 14.3 ms per 1024-frequency direction).
   const char *cenv = getenv("M3DRT_FAST_CPL");
   int cpl = 1;
-  if (cenv && atoi(cenv) == 2 && d->cs_x == 1 && d->gs_x == 1 && nx % 2 == 0 && !(get_rates && b->PsiJ) && !spl)
+  if (cenv && atoi(cenv) == 2 && d->cs_x == 1 && d->gs_x == 1 && nx % 2 == 0 && !(get_rates && b->PsiJ) && !spl &&
+      !iout)
     cpl = 2;
   // the strip length: the first instantiated one that tiles ny into at least two
   // strips within its workgroup's thread limit (M3DRT_FAST_L picks one of them)
@@ -567,7 +577,7 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
     // spare row) in the LDS the pairs leave: J only, one column per lane, strips of
     // 20 rows (the form that holds it without spilling registers)
     const size_t rows = ((size_t)ctx->lds_max - np * 2 * sizeof(T)) / ((size_t)nx * sizeof(T));
-    const bool j_only = !(get_rates && b->PsiJ) && !spl;
+    const bool j_only = !(get_rates && b->PsiJ) && !spl && !iout;
     if (j_only && cpl == 1 && L == 20 && rows >= 1 && !getenv("M3DRT_FAST_NO_SROWS")) {
       // the most rows per strip to hold: 8 unless M3DRT_FAST_SR says otherwise.  On
       // the 120 x 120 plane, per 1024-frequency direction: 13.35 ms with 8 and 13.45
@@ -604,7 +614,7 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   a.J = static_cast<T *>(b->J);
   a.PsiJ = get_rates ? static_cast<T *>(b->PsiJ) : nullptr;
   a.exitI = static_cast<T *>(b->exit_I);
-  a.Iout = spl ? static_cast<T *>(b->I_out) : nullptr;
+  a.Iout = spl || iout ? static_cast<T *>(b->I_out) : nullptr;
   a.Psiout = spl ? static_cast<T *>(b->Psi_out) : nullptr;
   a.Irem = spl ? static_cast<T *>(b->Irem_out) : nullptr;
   a.esc = spl ? static_cast<T *>(b->escape_out) : nullptr;
@@ -631,7 +641,7 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
             "LDS mode %d (%zu B, %d S rows of %d per strip in LDS)%s\n", f64 ? "f64" : "f32", d->nf, nx, ny, nz,
             nt, cpl, L, mode,
             np * (mode == 2 ? 3 : 2) * sizeof(T) + (mode == 4 ? (size_t)(nstrips * sr + 1) * nx * sizeof(T) : 0),
-            mode == 2 ? L + 1 : sr, L + 1, a.PsiJ ? ", PsiJ" : spl ? ", split outputs" : "");
+            mode == 2 ? L + 1 : sr, L + 1, a.PsiJ ? ", PsiJ" : spl ? ", split outputs" : iout ? ", I out" : "");
   return fast_dispatch(ctx, a, d->nf, nt, L, cpl, mode, np, sr);
 }
 

@@ -230,6 +230,39 @@ def test_sweep_split_outputs_fast_form(ctx, nx, ny, vec, maser, layout, monkeypa
     check32(pnat(exitI), I[last], "exit plane")
 
 
+@pytest.mark.parametrize("nx,ny,vec,layout", [(96, 96, (0.35, 0.3, 0.887), "native"),
+                                              (120, 120, (-0.2, -0.3, -0.93), "reference")])
+def test_sweep_intensity_cube_fast_form(ctx, nx, ny, vec, layout, monkeypatch, capfd):
+    """The intensity cube alone (I_out, no J, no rates: what an adjoint sweep
+    recomputes), in the straight-line kernel's I-only form, against the oracle; the
+    exit plane with it."""
+    case = make_case(nf=3, nx=nx, ny=ny, nz=5, seed=nx + 3, vec=vec)
+    print(f"intensity cube fast form {nx}x{ny} {layout}", flush=True)
+    I, _, _, _ = run_oracle(case)
+    gz = case["gz"]
+    if layout == "native":
+        cub = lambda a: dev(to_native(interior(a)))
+        pla = lambda a: dev(plane_to_native(a[gz]))
+        nat = lambda t: np.moveaxis(t.double().cpu().numpy(), 1, 3)
+        pnat = lambda t: np.moveaxis(t.double().cpu().numpy(), 0, 2)
+    else:
+        cub = lambda a: dev(interior(a))
+        pla = lambda a: dev(a[gz])
+        nat = pnat = lambda t: t.double().cpu().numpy()
+    c, s = cub(case["chi"]), cub(case["S"])
+    Iout = torch.full_like(c, 7.0)
+    exitI = torch.zeros_like(pla(case["chi"]))
+    monkeypatch.setenv("M3DRT_VERBOSE", "1")
+    ctx.sc_sweep(c, s, pla(case["chi"]), pla(case["S"]), pla(case["I"]), dev(case["chi500"]),
+                 dev(case["chi500_up"]), dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"],
+                 I_out=Iout, exit_I=exitI, layout=layout)
+    torch.cuda.synchronize()
+    assert "I out" in capfd.readouterr().err
+    check32(nat(Iout), interior(I), "I")
+    last = case["nz"] if case["dir"] > 0 else 1
+    check32(pnat(exitI), I[last], "exit plane")
+
+
 def check_exact_strips(ctx, case, fp64, psij, layout):
     I, psi, _, _ = run_oracle(case, fp64=fp64)
     dt = torch.float64 if fp64 else torch.float32
