@@ -181,3 +181,24 @@ def test_bf_rates_match_oracle(ctx):
         # the same FP32 sums in the same order: bit for bit
         np.testing.assert_array_equal(gij.cpu().numpy(), ref[0])
         np.testing.assert_array_equal(gji.cpu().numpy(), ref[1])
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(precond=True, own_norm=True)])
+def test_bb_rates_line_runs_equal_single_lines(ctx, kw):
+    """Over 300 000 cells the kernel gives each lane a run of lines (8 lines in runs of
+    2); each (line, cell) sum is its own, so the call equals one call per line bit for
+    bit."""
+    d = make_rates_case(ncell=300_000, nf=64, nline=8, seed=5)
+    print(f"bb_rates line runs {kw}", flush=True)
+    per_line = ("iblue", "ired", "lev_i", "lev_j", "Aij", "Bij", "lower_off", "q", "wnu", "wq", "adamp")
+    dd = to_dev(d, "native")
+    g0 = torch.zeros_like(dd["adamp"])
+    full = (g0.clone(), g0.clone())
+    ctx.bb_rates(dd, WMU, 0.9, full[0], full[1], **kw)
+    each = (g0.clone(), g0.clone())
+    for l in range(8):
+        dl = {k: (v[l:l + 1] if k in per_line else v) for k, v in dd.items()}
+        ctx.bb_rates(dl, WMU, 0.9, each[0][l:l + 1], each[1][l:l + 1], **kw)
+    torch.cuda.synchronize()
+    assert float(full[0].abs().max()) > 0
+    assert torch.equal(full[0], each[0]) and torch.equal(full[1], each[1])
