@@ -66,6 +66,17 @@ __device__ __forceinline__ void lin_weights_deriv(float t, float &E, float &W, f
   dW = E - dPs;
 }
 
+// A float at a byte offset from a wave-uniform base: the offset is a 32-bit lane
+// value, so the access takes the scalar-base + 32-bit-offset form and no 64-bit
+// address is formed per lane and row.  The host takes the kernels that use it only
+// where a plane's byte span fits 32 bits.
+__device__ __forceinline__ float ldb(const float *base, uint32_t off) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void stb(float *base, uint32_t off, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // MAXC: cells per thread.  FULL: every thread owns exactly MAXC cells (nx*ny =
 // MAXC * blockDim), so the per-cell loops carry no guards and the loads of all of a
 // thread's cells can be in flight together.  SEED: a J or intensity-cube cotangent
@@ -305,6 +316,8 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
   const float *gchi = a.gchi + f * a.gs_f, *gS = a.gS + f * a.gs_f, *gI = a.gI + f * a.gs_f;
   const float *ghost_rc = a.rc + (int64_t)a.nz * nxy;
   auto plane_of = [&](int step) { return a.dir > 0 ? step : a.nz - 1 - step; };
+  // byte steps of one row in a cube plane, in the ghost plane, in a per-cell plane
+  const uint32_t rowb = (uint32_t)a.cs_y * 4u, growb = (uint32_t)a.gs_y * 4u, crowb = (uint32_t)nx * 4u;
   int xs[4], tx[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -315,11 +328,15 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
   // per owned cell, carried from one step's (d) to the next step's (b): the
   // intensity cotangent and the chi (before its 1/chi500) and S shares sent upwind
   float acc[L], ub[L], vb[L];
+  {
+    const uint32_t go0 = (uint32_t)(gox + (int64_t)ybeg * a.gs_y) * 4u;
+    const float *eb = a.exitbar ? a.exitbar + f * a.gs_f : nullptr;
 #pragma unroll
-  for (int k = 0; k < L; ++k) {
-    acc[k] = a.exitbar ? a.exitbar[f * a.gs_f + gox + (int64_t)(ybeg + k) * a.gs_y] : 0.0f;
-    ub[k] = 0.0f;
-    vb[k] = 0.0f;
+    for (int k = 0; k < L; ++k) {
+      acc[k] = eb ? ldb(eb, go0 + (uint32_t)k * growb) : 0.0f;
+      ub[k] = 0.0f;
+      vb[k] = 0.0f;
+    }
   }
 
   struct URow { float rI, i1, i2, rX, rS; };
@@ -333,26 +350,31 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
     const int z = plane_of(step);
     const int zu = GHOST ? 0 : plane_of(step - 1);
     const float *rcu = GHOST ? ghost_rc : a.rc + (int64_t)zu * nxy;
-    const int64_t bz = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f + ox;
-    const int64_t bu = (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f + ox;
+    // wave-uniform bases of the target plane z and the upwind plane zu, and this
+    // thread's byte offsets of its first row (in a cube plane, the ghost plane, a
+    // per-cell plane)
+    const int64_t pz = (int64_t)z * a.cs_z + (int64_t)f * a.cs_f, pu = (int64_t)zu * a.cs_z + (int64_t)f * a.cs_f;
+    const uint32_t vo = (uint32_t)(ox + (int64_t)yb * a.cs_y) * 4u;
+    const uint32_t go = (uint32_t)(gox + (int64_t)yb * a.gs_y) * 4u;
+    const uint32_t co = (uint32_t)(yb * nx + x) * 4u;
     // (a) the upwind plane
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       const int y = yb + k, c = y * nx + x;
       float iv, cv, sv;
       if (GHOST) {
-        const int64_t g = gox + (int64_t)y * a.gs_y;
-        cv = gchi[g];
-        sv = gS[g];
-        iv = gI[g];
+        const uint32_t g = go + (uint32_t)k * growb;
+        cv = ldb(gchi, g);
+        sv = ldb(gS, g);
+        iv = ldb(gI, g);
       } else {
-        const int64_t o = bu + (int64_t)y * a.cs_y;
-        cv = a.chi[o];
-        sv = a.S[o];
-        iv = a.I[o];
+        const uint32_t o = vo + (uint32_t)k * rowb;
+        cv = ldb(a.chi + pu, o);
+        sv = ldb(a.S + pu, o);
+        iv = ldb(a.I + pu, o);
       }
       sI[c] = iv;
-      sX[c] = cv * rcu[c];
+      sX[c] = cv * ldb(rcu, co + (uint32_t)k * crowb);
       sS[c] = sv;
     }
     dev::lds_barrier();
@@ -373,16 +395,15 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
     const float *dtp = a.dt500 + (int64_t)z * nxy, *rcp = a.rc + (int64_t)z * nxy;
     auto tload = [&](int k) -> TIn {
       TIn t;
-      const int y = yb + k, c = y * nx + x;
-      const int64_t o = bz + (int64_t)y * a.cs_y;
-      t.chv = a.chi[o];
-      t.ss = a.S[o];
-      t.dt500 = dtp[c];
-      t.rc = rcp[c];
-      t.cb = a.chibar[o];
-      t.sb = a.Sbar[o];
-      t.jb = (SEED && a.Jbar) ? a.Jbar[o] : 0.0f;
-      t.ib = (SEED && a.Ibar) ? a.Ibar[o] : 0.0f;
+      const uint32_t o = vo + (uint32_t)k * rowb, oc = co + (uint32_t)k * crowb;
+      t.chv = ldb(a.chi + pz, o);
+      t.ss = ldb(a.S + pz, o);
+      t.dt500 = ldb(dtp, oc);
+      t.rc = ldb(rcp, oc);
+      t.cb = ldb(a.chibar + pz, o);
+      t.sb = ldb(a.Sbar + pz, o);
+      t.jb = (SEED && a.Jbar) ? ldb(a.Jbar + pz, o) : 0.0f;
+      t.ib = (SEED && a.Ibar) ? ldb(a.Ibar + pz, o) : 0.0f;
       return t;
     };
     float gi[L], gx[L], gs[L];
@@ -417,9 +438,9 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
       g = (Inu >= -a.imax && Inu <= a.imax) ? g : 0.0f;
       const float gdt = dtr > a.mdtau ? g * (Iup * dE + ssup * dW + t.ss * dPs) : 0.0f;
       const float gxx = 0.5f * gdt * t.dt500;
-      const int64_t o = bz + (int64_t)y * a.cs_y;
-      a.chibar[o] = t.cb + (ub[k] + gxx) * t.rc;
-      a.Sbar[o] = t.sb + (vb[k] + g * Ps);
+      const uint32_t o = vo + (uint32_t)k * rowb;
+      stb(a.chibar + pz, o, t.cb + (ub[k] + gxx) * t.rc);
+      stb(a.Sbar + pz, o, t.sb + (vb[k] + g * Ps));
       gi[k] = g * E;
       gx[k] = gxx;
       gs[k] = g * W;
@@ -480,10 +501,11 @@ __global__ void __launch_bounds__(L >= 8 ? 768 : 1024) sc2_adjoint_strip(const A
       xb += a.wy[1] * t2.rX;
       sb += a.wy[1] * t2.rS;
       if (GHOST) {
-        const int64_t g = f * a.gs_f + gox + (int64_t)y * a.gs_y;
-        if (a.gIbar) a.gIbar[g] = a.gIbar[g] + ib;
-        if (a.gchibar) a.gchibar[g] = a.gchibar[g] + xb * rcu[y * nx + x];
-        if (a.gSbar) a.gSbar[g] = a.gSbar[g] + sb;
+        const uint32_t g = go + (uint32_t)k * growb;
+        const int64_t gf = f * a.gs_f;
+        if (a.gIbar) stb(a.gIbar + gf, g, ldb(a.gIbar + gf, g) + ib);
+        if (a.gchibar) stb(a.gchibar + gf, g, ldb(a.gchibar + gf, g) + xb * ldb(rcu, co + (uint32_t)k * crowb));
+        if (a.gSbar) stb(a.gSbar + gf, g, ldb(a.gSbar + gf, g) + sb);
       } else {
         acc[k] = ib;
         ub[k] = xb;
@@ -589,8 +611,13 @@ int sweep_adjoint(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_adjoint
   a.imax = (float)d->maser_Inu_max;
   a.wj = (float)d->wj;
   // the strip form where an instantiated strip length divides ny and the strips fit
-  // one workgroup; the cell-owner form otherwise
-  if (!getenv("M3DRT_ADJ_CELLS")) {
+  // one workgroup, and where a plane of the cubes and of the ghost planes spans less
+  // than 4 GiB (the strip kernel addresses within a plane by 32-bit byte offsets);
+  // the cell-owner form otherwise
+  const auto span32 = [&](int64_t sx, int64_t sy) {
+    return ((int64_t)(nx - 1) * sx + (int64_t)(ny - 1) * sy + 1) * 4 <= (int64_t)UINT32_MAX;
+  };
+  if (!getenv("M3DRT_ADJ_CELLS") && span32(a.cs_x, a.cs_y) && span32(a.gs_x, a.gs_y)) {
     for (int Ls : kStripL) {
       if (ny % Ls || (int64_t)nx * (ny / Ls) > strip_max_threads(Ls)) continue;
       const int nts = nx * (ny / Ls);
