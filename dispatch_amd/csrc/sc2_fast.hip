@@ -550,6 +550,15 @@ int run_fast(m3drt_ctx *ctx, const m3drt_sweep_desc *d, const m3drt_sweep_bufs *
   auto max_threads = [](int l) { return l >= 30 ? 512 : l >= 20 ? 768 : 1024; };
   int L = 0;
   if (!want && cpl == 1 && ny % 20 == 0 && ny / 20 >= 2 && (ny / 20) * nx >= 512 && (ny / 20) * nx <= 768) L = 20;
+  // Where the pairs and S of three workgroups fit one CU's LDS, a strip of 15 (or 20)
+  // rows that still gives a workgroup more than three waves: four rows of loads in
+  // flight instead of three, three workgroups per CU.  On the 60 x 60 plane the
+  // split-output sweep took 6.47 ms per 2400-frequency direction with strips of 15,
+  // 6.55 with 20, 7.40 with 12 and 7.05 with 4 (900 threads, one workgroup per CU).
+  const size_t np_all = (size_t)(nx + 4) * (ny + 4);
+  if (!want && !L && cpl == 1 && 3 * np_all * 3 * sizeof(T) <= (size_t)ctx->lds_max)
+    for (int c : {15, 20})
+      if (!L && ny % c == 0 && ny / c >= 2 && (ny / c) * nx > 192 && (ny / c) * nx <= max_threads(c)) L = c;
   for (int pass = 0; pass < 2 && !L; ++pass) {
     const int nxp = nx / cpl;
     const int *cand = cpl == 2 ? kL2 : kL1;

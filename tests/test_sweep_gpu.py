@@ -192,16 +192,21 @@ def test_sweep_s_rows_in_lds(ctx, vec, monkeypatch, capfd):
     assert "LDS mode 4" in capfd.readouterr().err
 
 
-@pytest.mark.parametrize("nx,ny,vec,maser,layout", [(60, 60, (0.3, -0.4, 0.866), False, "native"),
-                                                     (60, 60, (-0.25, 0.3, -0.92), True, "native"),
-                                                     (120, 120, (0.3, 0.2, 0.93), False, "reference")])
-def test_sweep_split_outputs_fast_form(ctx, nx, ny, vec, maser, layout, monkeypatch, capfd):
+@pytest.mark.parametrize("nx,ny,vec,maser,layout,strip", [(60, 60, (0.3, -0.4, 0.866), False, "native", 15),
+                                                           (60, 60, (-0.25, 0.3, -0.92), True, "native", None),
+                                                           (60, 60, (0.3, -0.4, 0.866), True, "native", 4),
+                                                           (120, 120, (0.3, 0.2, 0.93), False, "reference", None)])
+def test_sweep_split_outputs_fast_form(ctx, nx, ny, vec, maser, layout, strip, monkeypatch, capfd):
     """The ALI's split set alone (I, Psi, Irem and escape with rates, no J): the
     straight-line kernel's split form (S in LDS on the 60 x 60 plane, from the cache on
-    120 x 120), with masing cells (negative chi) on one case, against the oracle's
-    short_char2 with split rates: I, Psi, Irem, escape and the exit plane."""
+    120 x 120), with masing cells (negative chi) on two cases, against the oracle's
+    short_char2 with split rates: I, Psi, Irem, escape and the exit plane.  On 60 x 60
+    the default strip is 15 rows (checked on the first case); the third case takes
+    strips of 4."""
     case = make_case(nf=3, nx=nx, ny=ny, nz=6, seed=nx + int(maser), vec=vec, maser=maser)
-    print(f"split outputs fast form {nx}x{ny} maser={maser} {layout}", flush=True)
+    print(f"split outputs fast form {nx}x{ny} maser={maser} {layout} strip={strip}", flush=True)
+    if strip == 4:
+        monkeypatch.setenv("M3DRT_FAST_L", "4")
     I, psi, irem, esc = run_oracle(case, split_rates=True)
     gz = case["gz"]
     if layout == "native":
@@ -221,7 +226,10 @@ def test_sweep_split_outputs_fast_form(ctx, nx, ny, vec, maser, layout, monkeypa
                  dev(case["chi500_up"]), dev(case["g500"]), dir=case["dir"], ds=case["ds"], vec=case["vec"],
                  split_rates=True, get_rates=True, exit_I=exitI, layout=layout, **outs)
     torch.cuda.synchronize()
-    assert "split outputs" in capfd.readouterr().err
+    err = capfd.readouterr().err
+    assert "split outputs" in err
+    if strip:
+        assert f"strip {strip}," in err
     check32(nat(outs["I_out"]), interior(I), "I")
     check32(nat(outs["Psi_out"]), psi, "Psi")
     check32(nat(outs["Irem_out"]), irem, "Irem")
