@@ -9,6 +9,7 @@
 #include "m3drt_internal.hpp"
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -66,6 +67,66 @@ __global__ void __launch_bounds__(256) rotate_kernel(const RotArgs a) {
   }
 }
 
+// The same gather where x is the unit-stride axis (the native layout): each wave
+// takes RW consecutive output rows (k, z, f, y) per pass, its lanes along x.  A row's
+// shift, weights and both input rows are wave-uniform, so no index is recovered by
+// division per element; a lane's four taps are two adjacent elements of each of two
+// input rows.  The taps of all RW rows are loaded before any is stored, so a wave
+// has RW rows of loads in flight.  Rows are taken in memory order: the input row y2
+// of one output row is the y1 of the next, read from the cache.  The weighted sum is
+// formed as in rotate_kernel.
+template <int RW>
+__global__ void __launch_bounds__(256) rotate_rows_kernel(const RotArgs a, int nrows) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int r0 = (blockIdx.x * (blockDim.x >> 6) + wave) * RW; r0 < nrows; r0 += nwaves * RW) {
+    const float *p1[RW], *p2[RW];
+    float *o[RW];
+    PlaneShift s[RW];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+      // r = ((k * nz + z) * nf + f) * ny + y; rows past the end repeat the last one
+      const int r = min(r0 + j, nrows - 1);
+      const int y = r % a.ny;
+      int q = r / a.ny;
+      const int f = q % a.nf;
+      q /= a.nf;
+      const int z = q % a.nz, k = q / a.nz;
+      s[j] = a.sh[z];
+      int y1 = y + s[j].up2;
+      y1 -= y1 >= a.ny ? a.ny : 0;
+      const int y2 = y1 + 1 == a.ny ? 0 : y1 + 1;
+      const int64_t base = (int64_t)z * a.cs_z + (int64_t)k * a.cs_k + (int64_t)f * a.cs_f;
+      p1[j] = a.in + base + (int64_t)y1 * a.cs_y;
+      p2[j] = a.in + base + (int64_t)y2 * a.cs_y;
+      o[j] = a.out + base + (int64_t)y * a.cs_y;
+    }
+    for (int x = lane; x < a.nx; x += 64) {
+      float t[RW][4];
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        int x1 = x + s[j].up1;
+        x1 -= x1 >= a.nx ? a.nx : 0;
+        const int x2 = x1 + 1 == a.nx ? 0 : x1 + 1;
+        t[j][0] = p1[j][x1 * a.cs_x];
+        t[j][1] = p1[j][x2 * a.cs_x];
+        t[j][2] = p2[j][x1 * a.cs_x];
+        t[j][3] = p2[j][x2 * a.cs_x];
+      }
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        if (r0 + j >= nrows) break;
+        float v = s[j].a * t[j][0];
+        v = s[j].b * t[j][1] + v;
+        v = s[j].c * t[j][2] + v;
+        v = s[j].d * t[j][3] + v;
+        o[j][x * a.cs_x] = v;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // the per-plane shifts, as rotate_array forms them (real(4) but for the real(8)
@@ -117,7 +178,14 @@ int rotate_array(m3drt_ctx *ctx, const m3drt_rotate_args *r) {
   a.fast_f = r->cs_f <= r->cs_x ? 1 : 0;
   const int64_t n = (int64_t)r->nf * r->nx * r->ny;
   const int bx = (int)std::min<int64_t>((n + 255) / 256, 4096);
-  if (n + (int64_t)bx * 256 <= INT32_MAX)
+  // rows of x when x runs fastest: four rows a wave per pass, four waves a
+  // workgroup (the row count and the waves' stride over it fit int32)
+  constexpr int RW = 4;
+  const int64_t nrows = (int64_t)r->nk * r->nz * r->nf * r->ny;
+  const int rb = (int)std::min<int64_t>((nrows + 4 * RW - 1) / (4 * RW), 8192);
+  if (!a.fast_f && nrows + (int64_t)rb * 4 * RW <= INT32_MAX && !getenv("M3DRT_ROTATE_FLAT"))
+    hipLaunchKernelGGL(rotate_rows_kernel<RW>, dim3(rb), dim3(256), 0, ctx->stream, a, (int)nrows);
+  else if (n + (int64_t)bx * 256 <= INT32_MAX)
     hipLaunchKernelGGL(rotate_kernel<int>, dim3(bx, r->nz, r->nk), dim3(256), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL(rotate_kernel<int64_t>, dim3(bx, r->nz, r->nk), dim3(256), 0, ctx->stream, a);

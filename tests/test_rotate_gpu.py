@@ -37,6 +37,25 @@ def test_rotate_matches_oracle(ctx, layout, vec, off):
     np.testing.assert_array_equal(back(out), ref)
 
 
+@pytest.mark.parametrize("flat", [False, True])
+def test_rotate_rows_wider_than_a_wave(ctx, flat, monkeypatch):
+    """The native layout's row kernel on a plane of 130 columns (two passes of a
+    wave's lanes along x, the wrap in the second) and 7 rows, three frequencies, two
+    fields; and the flat-index kernel (M3DRT_ROTATE_FLAT) on the same case."""
+    rng = np.random.default_rng(11)
+    x = rng.random((2, 5, 7, 130, 3)).astype(np.float32)       # (k, z, y, x, f)
+    vec, off = (0.52, -0.31, 0.8), (0.75, -0.25)
+    print(f"rotate rows 130x7 flat={flat}", flush=True)
+    if flat:
+        monkeypatch.setenv("M3DRT_ROTATE_FLAT", "1")
+    ref = O.rotate_array(x, vec, DS, 2.5, off)
+    xi = dev(np.moveaxis(x, 4, 2))                               # (k, z, f, y, x)
+    out = torch.full_like(xi, -1.0)
+    ctx.rotate_array(xi, out, vec, DS, 2.5, off, layout="native")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(np.moveaxis(out.cpu().numpy(), 2, 4), ref)
+
+
 def test_output_pass_on_rotated_columns(ctx):
     vec = (0.3, -0.25, 0.92)
     case = make_case(nf=6, nx=9, ny=8, nz=12, seed=61, vec=vec)
