@@ -24,6 +24,12 @@ struct LcqArgs {
   float rdsup, dsup1, dsup2, mdtau, imax, wj;
 };
 
+// planes of chi and S a lane of the linear column sweep holds in flight ahead of the
+// one it steps onto.  The linear step is short enough for the load latency to show
+// (1.51 -> 1.41 ms per 848M-point ray); the quadratic and hermite steps are bound by
+// their arithmetic, and the same ring made them slower (2.82 -> 3.28 ms quadratic).
+constexpr int kLcPF = 4;
+
 // the monotonised (MC-limited) centred slope of two one-sided differences
 __device__ __forceinline__ float mc_slope(float t1, float t3) {
   const float t2 = 0.5f * (t1 + t3);
@@ -140,10 +146,30 @@ __global__ void __launch_bounds__(256) lcl_sweep(const LclArgs a) {
   const int x = col % a.nx, y = col / a.nx;
   const int64_t o = f * a.cs_f + x * a.cs_x + y * a.cs_y;
   const int64_t go = f * a.gs_f + x * a.gs_x + y * a.gs_y;
+  auto plane = [&](int step) { return o + (int64_t)(a.dir > 0 ? step : a.nz - 1 - step) * a.cs_z; };
   float xxup = a.gchi[go], ssup = a.gS[go], Iup = a.gI[go];
+  // chi and S of the planes ahead, loaded kLcPF steps before their use in a register
+  // ring (planes past the last repeat it and are never used)
+  float qc[kLcPF], qs[kLcPF];
+#pragma unroll
+  for (int k = 0; k < kLcPF; ++k) {
+    const int64_t op = plane(min(k, a.nz - 1));
+    qc[k] = a.chi[op];
+    qs[k] = a.S[op];
+  }
   for (int s = 0; s < a.nz; ++s) {
-    const int64_t oz = o + (int64_t)(a.dir > 0 ? s : a.nz - 1 - s) * a.cs_z;
-    const float xx = a.chi[oz], ss = a.S[oz];
+    const int64_t oz = plane(s);
+    const float xx = qc[0], ss = qs[0];
+#pragma unroll
+    for (int k = 0; k + 1 < kLcPF; ++k) {
+      qc[k] = qc[k + 1];
+      qs[k] = qs[k + 1];
+    }
+    {
+      const int64_t op = plane(min(s + kLcPF, a.nz - 1));
+      qc[kLcPF - 1] = a.chi[op];
+      qs[kLcPF - 1] = a.S[op];
+    }
     const float dt = fmaxf(a.dsup1 * (xxup + xx), a.mdtau);
     float E, W, Ps, Esc;
     dev::lin_weights(dt, E, W, Ps, Esc);
