@@ -26,8 +26,9 @@ struct LcqArgs {
 
 // planes of chi and S a lane of the linear column sweep holds in flight ahead of the
 // one it steps onto.  The linear step is short enough for the load latency to show
-// (1.51 -> 1.41 ms per 848M-point ray); the quadratic and hermite steps are bound by
-// their arithmetic, and the same ring made them slower (2.82 -> 3.28 ms quadratic).
+// (1.58 -> 1.46 ms per 848M-point ray); the quadratic and hermite steps are bound by
+// their arithmetic, and the same ring made them slower (3.21 -> 3.37 ms quadratic,
+// on the same box).
 constexpr int kLcPF = 4;
 
 // the monotonised (MC-limited) centred slope of two one-sided differences
