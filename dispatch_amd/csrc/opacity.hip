@@ -63,7 +63,8 @@ struct Coef {
 // a line that passed, waiting to be deposited: its profile constants, its window
 // fa..fb (1-based), its abscissa (the node-fa offset x0 in grid steps on an
 // equidistant grid, the shifted centre llc in ln(lambda) on any other), its
-// departure factors and whether it also feeds chi_scat (plain: it does not)
+// departure factors and whether it also feeds chi_scat (plain: it does not; always
+// plain in the direct form, which takes no scattering split)
 struct Pend {
   Coef c;
   double llc;
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       if (k < np) {
         Fa = min(Fa, pend[k].fa);
         Fb = max(Fb, pend[k].fb);
-        anyscat = anyscat || !pend[k].plain;
+        anyscat = anyscat || !(DIRECT || pend[k].plain);
       }
     }
     constexpr int U = 8;
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
             const int g = f + j;
             if (g >= pend[k].fa && g <= pend[k].fb) {
               const float lte = val(pend[k], g);
-              if (pend[k].plain) {
+              if (DIRECT || pend[k].plain) {
                 cv[j] = cv[j] + lte * pend[k].bi;
                 ev[j] = ev[j] + lte * pend[k].bj;
               } else {
@@ -396,7 +397,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       for (int k = 0; k < K; ++k) {
         if (k < np && pend[k].fa <= f0 && f0 <= pend[k].fb) {
           const float v = val(pend[k], f0);
-          chi_f0 = pend[k].plain ? chi_f0 + v * pend[k].bi : chi_f0 + v;
+          chi_f0 = (DIRECT || pend[k].plain) ? chi_f0 + v * pend[k].bi : chi_f0 + v;
         }
       }
       float refl_c = a.tol * cont[(f0 - 1) * sf], lref_c;
