@@ -64,7 +64,7 @@ struct Coef {
 // fa..fb (1-based), its abscissa (the node-fa offset x0 in grid steps on an
 // equidistant grid, the shifted centre llc in ln(lambda) on any other), its
 // departure factors and whether it also feeds chi_scat (plain: it does not; always
-// plain in the direct form, which takes no scattering split)
+// plain without the scattering split)
 struct Pend {
   Coef c;
   double llc;
@@ -221,9 +221,14 @@ __device__ __forceinline__ float bot_eval(const Coef &c, float x, const float *T
 // DIRECT: chi and eta are the working opacity itself, for a list of at most K lines
 // alone in its composition (one deposit per cell, after every read of cont): each
 // point's line sum is formed from zero in line order and added to the working value
-// at the deposit, as the increments of the composition would be added.
-template <bool GENERAL, bool DIRECT>
+// at the deposit, as the increments of the composition would be added.  SCAT: the
+// call splits off the scattering part (a line then feeds chi_scat unless its
+// departure factors are given); without it every line is plain, known at compile
+// time, and the scattering path leaves the per-point code.  The direct form never
+// takes the split.
+template <bool GENERAL, bool DIRECT, bool SCAT>
 __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
+  static_assert(!(DIRECT && SCAT), "the direct form takes no scattering split");
   extern __shared__ float tab[];  // TGX then TH2X
   for (int i = threadIdx.x; i <= QLG_NX; i += blockDim.x) {
     tab[i] = a.tgx[i];
@@ -281,7 +286,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       if (k < np) {
         Fa = min(Fa, pend[k].fa);
         Fb = max(Fb, pend[k].fb);
-        anyscat = anyscat || !(DIRECT || pend[k].plain);
+        anyscat = anyscat || !(!SCAT || pend[k].plain);
       }
     }
     constexpr int U = 8;
@@ -318,7 +323,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
             const int g = f + j;
             if (g >= pend[k].fa && g <= pend[k].fb) {
               const float lte = val(pend[k], g);
-              if (DIRECT || pend[k].plain) {
+              if (!SCAT || pend[k].plain) {
                 cv[j] = cv[j] + lte * pend[k].bi;
                 ev[j] = ev[j] + lte * pend[k].bj;
               } else {
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__(256) opacity_kernel(const OpacArgs a) {
       for (int k = 0; k < K; ++k) {
         if (k < np && pend[k].fa <= f0 && f0 <= pend[k].fb) {
           const float v = val(pend[k], f0);
-          chi_f0 = (DIRECT || pend[k].plain) ? chi_f0 + v * pend[k].bi : chi_f0 + v;
+          chi_f0 = (!SCAT || pend[k].plain) ? chi_f0 + v * pend[k].bi : chi_f0 + v;
         }
       }
       float refl_c = a.tol * cont[(f0 - 1) * sf], lref_c;
@@ -612,8 +617,9 @@ int line_opacity(m3drt_ctx *ctx, const m3drt_linelist *ll, const m3drt_opacity_a
   const int threads = 256;
   const int blocks = (o->ncell + threads - 1) / threads;
   const size_t lds = 2 * (QLG_NX + 1) * sizeof(float);
-  auto kern = direct ? (general ? opacity_kernel<true, true> : opacity_kernel<false, true>)
-                    : (general ? opacity_kernel<true, false> : opacity_kernel<false, false>);
+  auto kern = direct ? (general ? opacity_kernel<true, true, false> : opacity_kernel<false, true, false>)
+              : a.scat ? (general ? opacity_kernel<true, false, true> : opacity_kernel<false, false, true>)
+                       : (general ? opacity_kernel<true, false, false> : opacity_kernel<false, false, false>);
   M3DRT_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, ctx->stream, a);
   M3DRT_HIP(hipGetLastError());
